@@ -1,0 +1,126 @@
+"""Seeded synthetic workloads for the RHS + SSP-RK3 path.
+
+A case is a plain dict (the fields of the reference's Params that the path reads, params.hpp:20-43):
+  nx, ny, nz      interior cells of the block
+  dim             3, or 2 (z sweep skipped, weno-5.cpp:28)
+  L               (Lx, Ly, Lz); dx = Lx / nx (grid.cpp:14-16)
+  gamma, mu, Pr   fluid properties; cfl = CFL_max
+  conv            'none' | 'weno5' | 'weno5cw'
+  weights         'shu' | 'borges'
+  diff_order      0 (no diffusion) | 2 | 4 | 6
+  bc              six of 'periodic' | 'neumann', faces West, East, North, South, Front, Back
+
+Primitive fields W are numpy float64 arrays in the reference Array4d layout (NX, NY, NZ, 5),
+NX = nx + 6, variables [rho, p, u, v, w]; halos are left zero like a fresh Domain array.
+
+Nondimensionalisation.  The reference's Lax-Friedrichs split uses alpha = 3.0 everywhere
+(convection_flux.cpp:241).  Its 3-D scheme is only stable at CFL 0.5 when the largest wave speed is
+close to alpha, so every case sets rho0 = 1 and the sound speed c0 = 3 / (1 + M): the initial
+max |u| + c is then 3.  Velocities scale with U0 = M * c0, pressure p0 = rho0 * c0^2 / gamma and
+mu = rho0 * U0 * Lref / Re.
+"""
+import math
+
+import numpy as np
+
+NGC = 3
+ALPHA = 3.0
+
+
+def base_case(nx, ny, nz):
+    return dict(nx=nx, ny=ny, nz=nz, dim=3, L=(1.0, 1.0, 1.0), gamma=1.4, mu=1e-3, Pr=0.72,
+                cfl=0.5, conv="weno5cw", weights="shu", diff_order=4, bc=["periodic"] * 6)
+
+
+def padded_shape(case):
+    return (case["nx"] + 2 * NGC, case["ny"] + 2 * NGC, case["nz"] + 2 * NGC, 5)
+
+
+def _centres(case):
+    Lx, Ly, Lz = case["L"]
+    x = (np.arange(case["nx"]) + 0.5) * Lx / case["nx"]
+    y = (np.arange(case["ny"]) + 0.5) * Ly / case["ny"]
+    z = (np.arange(case["nz"]) + 0.5) * Lz / case["nz"]
+    return np.meshgrid(x, y, z, indexing="ij")
+
+
+def _empty(case):
+    return np.zeros(padded_shape(case), dtype=np.float64)
+
+
+def sound_speed(M):
+    return ALPHA / (1.0 + M)
+
+
+def random_smooth_W(case, seed, c0=2.2):
+    """Smooth waves, a steep tanh front and seeded noise in every variable: exercises the
+    nonlinear WENO weights and all viscous terms.  max|u| + c stays below alpha = 3."""
+    rng = np.random.default_rng(seed)
+    X, Y, Z = _centres(case)
+    Lx, Ly, Lz = case["L"]
+    sh = X.shape
+    x, y, z = X / Lx, Y / Ly, Z / Lz
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    front = np.tanh((x - 0.5) / 0.05)
+    W[I + (0,)] = 1.0 + 0.2 * np.sin(2 * np.pi * (x + y)) + 0.1 * front + 0.02 * rng.standard_normal(sh)
+    W[I + (1,)] = c0 * c0 / case["gamma"] * (1.0 + 0.1 * np.cos(2 * np.pi * (y + z)) + 0.05 * front
+                                             + 0.01 * rng.standard_normal(sh))
+    W[I + (2,)] = 0.3 * np.sin(2 * np.pi * z) + 0.03 * rng.standard_normal(sh)
+    W[I + (3,)] = 0.2 * np.cos(2 * np.pi * x) + 0.03 * rng.standard_normal(sh)
+    W[I + (4,)] = 0.1 * np.sin(2 * np.pi * y) + 0.03 * rng.standard_normal(sh)
+    return W
+
+
+def tgv_W(case, M=0.1, seed=2, c0=None):
+    """Taylor-Green vortex on [0, 2pi]^3 with a seeded 1e-6 uniform density perturbation."""
+    c0 = sound_speed(M) if c0 is None else c0
+    V = M * c0
+    rng = np.random.default_rng(seed)
+    X, Y, Z = _centres(case)
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    W[I + (0,)] = 1.0 + 1e-6 * (2.0 * rng.random(X.shape) - 1.0)
+    W[I + (1,)] = c0 * c0 / case["gamma"] + (V * V / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    W[I + (2,)] = V * np.sin(X) * np.cos(Y) * np.cos(Z)
+    W[I + (3,)] = -V * np.cos(X) * np.sin(Y) * np.cos(Z)
+    return W
+
+
+def shear_W(case, M=0.3, amp=0.05, seed=1, c0=None):
+    """3-D periodic double shear layer on [0,1]^3: u = U0 tanh((|y-0.5|-0.25)/0.02),
+    v = amp * U0 * sin(2 pi x + phi_k) with a seeded phase per z plane."""
+    c0 = sound_speed(M) if c0 is None else c0
+    V = M * c0
+    rng = np.random.default_rng(seed)
+    phase = rng.random(case["nz"]) * 2.0 * np.pi
+    X, Y, Z = _centres(case)
+    Ly = case["L"][1]
+    Lx = case["L"][0]
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    W[I + (0,)] = 1.0
+    W[I + (1,)] = c0 * c0 / case["gamma"]
+    W[I + (2,)] = V * np.tanh((np.abs(Y / Ly - 0.5) - 0.25) / 0.02)
+    W[I + (3,)] = amp * V * np.sin(2 * np.pi * X / Lx + phase[None, None, :])
+    return W
+
+
+def config1(n=64):
+    """Plain method, CPU-runnable: double shear layer, 64^3, [0,1]^3, component-wise WENO-5 (Shu)
+    with LF splitting, CD4 diffusion, Re = 1000, Mach 0.3, CFL 0.5; run for 100 steps."""
+    M, Re = 0.3, 1000.0
+    c0 = sound_speed(M)
+    case = dict(base_case(n, n, n), conv="weno5", weights="shu", diff_order=4,
+                mu=1.0 * M * c0 * 1.0 / Re)
+    return case, shear_W(case, M=M, amp=0.05, seed=1)
+
+
+def config2(n=256):
+    """Plain method, one GPU subdomain: Taylor-Green vortex, 256^3, [0,2pi]^3, characteristic
+    WENO-5 (Shu) + CD4 viscous terms, Mach 0.1, Re = 1600, CFL 0.5; 200 steps."""
+    M, Re = 0.1, 1600.0
+    c0 = sound_speed(M)
+    case = dict(base_case(n, n, n), L=(2 * math.pi,) * 3, conv="weno5cw", weights="shu",
+                diff_order=4, mu=1.0 * M * c0 * 1.0 / Re)
+    return case, tgv_W(case, M=M, seed=2)

@@ -1,0 +1,189 @@
+"""ctypes loader for the CPU oracle (lib/libnsoracle.so) and, where it was built, the reference
+harness (_ref/libnsref_strict.so, _ref/libnsref_fast.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker.  The product path never imports this module.
+
+All arrays are numpy float64 in the reference Array4d layout: shape (NX, NY, NZ, 5) with
+NX = nx + 6 (containers/array_4d.hpp:116-124, array_4d_dimensions.hpp:40-49).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NGC = 3
+
+BC = {"periodic": 0, "neumann": 1}
+CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
+WEIGHTS = {"shu": 0, "borges": 1}
+
+
+class NsoCfg(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("dim", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("mu", ctypes.c_double), ("Pr", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
+        ("bc", ctypes.c_int * 6),
+    ]
+
+
+def make_cfg(case):
+    """case: dict with nx, ny, nz, dim, L (3-tuple), gamma, mu, Pr, cfl, conv, weights,
+    diff_order, bc (6 names, W,E,N,S,F,B)."""
+    c = NsoCfg()
+    c.nx, c.ny, c.nz = case["nx"], case["ny"], case["nz"]
+    c.dim = case.get("dim", 3)
+    c.Lx, c.Ly, c.Lz = case["L"]
+    c.gamma, c.mu, c.Pr, c.cfl = case["gamma"], case["mu"], case["Pr"], case["cfl"]
+    c.conv = CONV[case["conv"]]
+    c.weights = WEIGHTS[case["weights"]]
+    c.diff_order = case["diff_order"]
+    for f, name in enumerate(case["bc"]):
+        c.bc[f] = BC[name]
+    return c
+
+
+def padded_shape(case):
+    return (case["nx"] + 2 * NGC, case["ny"] + 2 * NGC, case["nz"] + 2 * NGC, 5)
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class _Lib:
+    def __init__(self, path, prefix):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        for name in ("rhs", "step"):
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = ctypes.c_int
+        nt = getattr(self.lib, prefix + "num_threads")
+        nt.restype = ctypes.c_int
+
+    def num_threads(self):
+        return getattr(self.lib, self.prefix + "num_threads")()
+
+    def rhs(self, case, U, which=3):
+        """One stage RHS from conservative U (padded, interior used).  Returns (U_filled, W, R)."""
+        cfg = make_cfg(case)
+        U = np.ascontiguousarray(U, dtype=np.float64).copy()
+        W = np.zeros_like(U)
+        R = np.zeros_like(U)
+        rc = getattr(self.lib, self.prefix + "rhs")(ctypes.byref(cfg), _ptr(U), _ptr(W), _ptr(R),
+                                                    ctypes.c_int(which))
+        if rc != 0:
+            raise ValueError("rhs rejected the configuration")
+        return U, W, R
+
+    def step(self, case, W, nsteps):
+        """nsteps SSP-RK3 steps from primitive W.  Returns (W, U, dts)."""
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64).copy()
+        U = np.zeros_like(W)
+        dts = np.zeros(max(nsteps, 1), dtype=np.float64)
+        rc = getattr(self.lib, self.prefix + "step")(ctypes.byref(cfg), _ptr(W), _ptr(U),
+                                                     ctypes.c_int(nsteps), _ptr(dts))
+        if rc != 0:
+            raise ValueError("step rejected the configuration")
+        return W, U, dts[:nsteps]
+
+
+class Oracle(_Lib):
+    def __init__(self):
+        path = os.path.join(HERE, "lib", "libnsoracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle` (or build())")
+        super().__init__(path, "nso_")
+        L = self.lib
+        L.nso_dt.restype = ctypes.c_double
+        L.nso_weno5_left.restype = None
+        L.nso_weno5_right.restype = None
+
+    def dt(self, case, W):
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        return self.lib.nso_dt(ctypes.byref(cfg), _ptr(W))
+
+    def p2c(self, case, W):
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        U = np.zeros_like(W)
+        self.lib.nso_p2c(ctypes.byref(cfg), _ptr(W), _ptr(U))
+        return U
+
+    def c2p(self, case, U):
+        cfg = make_cfg(case)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        W = np.zeros_like(U)
+        self.lib.nso_c2p(ctypes.byref(cfg), _ptr(U), _ptr(W))
+        return W
+
+    def fill_halos(self, case, A):
+        cfg = make_cfg(case)
+        A = np.ascontiguousarray(A, dtype=np.float64).copy()
+        self.lib.nso_fill_halos(ctypes.byref(cfg), _ptr(A))
+        return A
+
+    def weno_left(self, f, weights="shu"):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros(1)
+        self.lib.nso_weno5_left(_ptr(f), ctypes.c_int(WEIGHTS[weights]), _ptr(out))
+        return out[0]
+
+    def weno_right(self, f, weights="shu"):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        out = np.zeros(1)
+        self.lib.nso_weno5_right(_ptr(f), ctypes.c_int(WEIGHTS[weights]), _ptr(out))
+        return out[0]
+
+    def eigenvectors(self, Wavg, gamma, direction):
+        Wavg = np.ascontiguousarray(Wavg, dtype=np.float64)
+        L = np.zeros(25)
+        R = np.zeros(25)
+        self.lib.nso_eigenvectors(_ptr(Wavg), ctypes.c_double(gamma), ctypes.c_int(direction),
+                                  _ptr(L), _ptr(R))
+        return L.reshape(5, 5), R.reshape(5, 5)
+
+    def fd(self, case, f, direction, second, d):
+        cfg = make_cfg(case)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        df = np.zeros_like(f)
+        self.lib.nso_fd(ctypes.byref(cfg), _ptr(f), _ptr(df), ctypes.c_int(direction),
+                        ctypes.c_int(second), ctypes.c_double(d))
+        return df
+
+
+class Reference(_Lib):
+    """The reference's own classes (strict or fast build), if oracle/_ref was built here."""
+
+    def __init__(self, flavour="strict"):
+        path = os.path.join(HERE, "_ref", f"libnsref_{flavour}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        super().__init__(path, "ref_")
+
+
+def reference_available(flavour="strict"):
+    return os.path.exists(os.path.join(HERE, "_ref", f"libnsref_{flavour}.so"))
+
+
+def interior(A):
+    return A[NGC:-NGC, NGC:-NGC, NGC:-NGC, :]
+
+
+def faces_mask(case):
+    """Boolean mask (NX,NY,NZ) of interior + face halos (no edges / corners)."""
+    NX, NY, NZ, _ = padded_shape(case)
+    ii = np.zeros(NX, bool); ii[NGC:-NGC] = True
+    jj = np.zeros(NY, bool); jj[NGC:-NGC] = True
+    kk = np.zeros(NZ, bool); kk[NGC:-NGC] = True
+    I, J, K = np.meshgrid(ii, jj, kk, indexing="ij")
+    n_in = I.astype(int) + J.astype(int) + K.astype(int)
+    return n_in >= 2
