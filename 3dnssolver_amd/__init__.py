@@ -1,0 +1,215 @@
+"""MI355X-native RHS + SSP-RK3 path of the compressible Navier-Stokes block solver.
+
+Python view of the C ABI in include/nsgpu.h (lib/libnsgpu.so, HIP for gfx950), shaped like the
+reference's operator interface (paths relative to the reference's src/):
+
+  Solver.apply_boundary(A)          Domain::apply_boundary_conditions + Communicator::communicate
+                                    (domain.cpp:18-28, communicator.cpp:6-144)
+  Solver.convection_solve(W, U, R)  ConvectionScheme::solve (convection_scheme.hpp:26): R += conv
+  Solver.diffusion_solve(W, R)      DiffusionScheme::solve (diffusion_scheme.hpp:14-17): R += diff
+  Solver.rhs(U, which)              one stage body of TimeScheme::step (time_scheme.cpp:33-41)
+  Solver.step(n)                    TimeScheme::step (time_scheme.cpp:3-70), state on the GPU
+  Solver.dt                         TimeScheme::get_dt (time_scheme.hpp:31)
+
+Arrays are numpy float64 in the reference Array4d layout (NX, NY, NZ, 5), NX = nx + 6.
+There is no CPU fallback: without the HIP library or a GPU every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnsgpu.so")
+
+BC = {"periodic": 0, "neumann": 1}
+CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
+WEIGHTS = {"shu": 0, "borges": 1}
+MATH = {"exact": 0, "fast": 1}
+STATE = {"conservative": 0, "primitive": 1}
+
+EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len", "nsgpu_set_state",
+           "nsgpu_get_state", "nsgpu_apply_boundary", "nsgpu_convection_solve",
+           "nsgpu_diffusion_solve", "nsgpu_rhs", "nsgpu_step", "nsgpu_get_dt", "nsgpu_reduce",
+           "nsgpu_stream", "nsgpu_timing", "nsgpu_timer_count", "nsgpu_timer_name",
+           "nsgpu_timer_read"]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("ngc", ctypes.c_int),
+        ("dim", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("Ly", ctypes.c_double), ("Lz", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("mu", ctypes.c_double), ("Pr", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
+        ("bc", ctypes.c_int * 6), ("math", ctypes.c_int),
+    ]
+
+
+class NsgpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nsgpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads lib/libnsgpu.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with `python 3dnssolver_amd/build.py`")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    D = ctypes.POINTER(ctypes.c_double)
+    L.nsgpu_create.argtypes = [ctypes.POINTER(Config), ctypes.c_int, ctypes.POINTER(P)]
+    L.nsgpu_destroy.argtypes = [P]
+    L.nsgpu_destroy.restype = None
+    L.nsgpu_strerror.restype = ctypes.c_char_p
+    L.nsgpu_array_len.argtypes = [P]
+    L.nsgpu_array_len.restype = ctypes.c_size_t
+    L.nsgpu_set_state.argtypes = [P, D, ctypes.c_int]
+    L.nsgpu_get_state.argtypes = [P, D, ctypes.c_int]
+    L.nsgpu_apply_boundary.argtypes = [P, D]
+    L.nsgpu_convection_solve.argtypes = [P, D, D, D]
+    L.nsgpu_diffusion_solve.argtypes = [P, D, D]
+    L.nsgpu_rhs.argtypes = [P, D, D, D, ctypes.c_int]
+    L.nsgpu_step.argtypes = [P, ctypes.c_int, D, D]
+    L.nsgpu_get_dt.argtypes = [P, D]
+    L.nsgpu_reduce.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_uint)]
+    L.nsgpu_stream.argtypes = [P]
+    L.nsgpu_stream.restype = P
+    L.nsgpu_timing.argtypes = [P, ctypes.c_int]
+    L.nsgpu_timer_name.restype = ctypes.c_char_p
+    L.nsgpu_timer_read.argtypes = [P, ctypes.c_int, D, ctypes.POINTER(ctypes.c_long)]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_config(case, math="exact"):
+    c = Config()
+    c.nx, c.ny, c.nz = case["nx"], case["ny"], case["nz"]
+    c.ngc = 3
+    c.dim = case.get("dim", 3)
+    c.Lx, c.Ly, c.Lz = case["L"]
+    c.gamma, c.mu, c.Pr, c.cfl = case["gamma"], case["mu"], case["Pr"], case["cfl"]
+    c.conv = CONV[case["conv"]]
+    c.weights = WEIGHTS[case["weights"]]
+    c.diff_order = case["diff_order"]
+    for f, name in enumerate(case["bc"]):
+        c.bc[f] = BC[name]
+    c.math = MATH[math]
+    return c
+
+
+class Solver:
+    """One block on one GPU (one nsgpu context)."""
+
+    def __init__(self, case, math="exact", device=0):
+        self.L = load_library()
+        self.case = dict(case)
+        self.cfg = make_config(case, math)
+        self.shape = (case["nx"] + 6, case["ny"] + 6, case["nz"] + 6, 5)
+        h = ctypes.c_void_p()
+        self._check(self.L.nsgpu_create(ctypes.byref(self.cfg), device, ctypes.byref(h)))
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nsgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"array shape {a.shape} != {self.shape}")
+        return a
+
+    def set_state(self, A, kind="primitive"):
+        A = self._arr(A)
+        self._check(self.L.nsgpu_set_state(self.h, _ptr(A), STATE[kind]))
+
+    def get_state(self, kind="primitive"):
+        A = np.zeros(self.shape)
+        self._check(self.L.nsgpu_get_state(self.h, _ptr(A), STATE[kind]))
+        return A
+
+    def apply_boundary(self, A):
+        A = self._arr(A).copy()
+        self._check(self.L.nsgpu_apply_boundary(self.h, _ptr(A)))
+        return A
+
+    def convection_solve(self, W, U, R):
+        W, U, R = self._arr(W), self._arr(U), self._arr(R).copy()
+        self._check(self.L.nsgpu_convection_solve(self.h, _ptr(W), _ptr(U), _ptr(R)))
+        return R
+
+    def diffusion_solve(self, W, R):
+        W, R = self._arr(W), self._arr(R).copy()
+        self._check(self.L.nsgpu_diffusion_solve(self.h, _ptr(W), _ptr(R)))
+        return R
+
+    def rhs(self, U, which=3):
+        """Returns (U with halos filled, W = c2p(U), R)."""
+        U = self._arr(U).copy()
+        W = np.zeros(self.shape)
+        R = np.zeros(self.shape)
+        self._check(self.L.nsgpu_rhs(self.h, _ptr(U), _ptr(W), _ptr(R), which))
+        return U, W, R
+
+    def step(self, nsteps=1, time=0.0):
+        """Advances the resident primitive state; returns (time, last dt)."""
+        t = np.array([time], dtype=np.float64)
+        dt = np.zeros(1)
+        self._check(self.L.nsgpu_step(self.h, nsteps, _ptr(t), _ptr(dt)))
+        return float(t[0]), float(dt[0])
+
+    @property
+    def dt(self):
+        dt = np.zeros(1)
+        self._check(self.L.nsgpu_get_dt(self.h, _ptr(dt)))
+        return float(dt[0])
+
+    def reduce(self):
+        """Returns (conservation sums[5], CFL dt, flags, status code)."""
+        s = np.zeros(5)
+        dt = np.zeros(1)
+        fl = ctypes.c_uint(0)
+        rc = self.L.nsgpu_reduce(self.h, _ptr(s), _ptr(dt), ctypes.byref(fl))
+        if rc not in (0, -4, -5, -6):
+            self._check(rc)
+        return s, float(dt[0]), fl.value, rc
+
+    def stream(self):
+        return self.L.nsgpu_stream(self.h)
+
+    def timing(self, enable=True):
+        self._check(self.L.nsgpu_timing(self.h, 1 if enable else 0))
+
+    def timers(self):
+        out = {}
+        for i in range(self.L.nsgpu_timer_count()):
+            ms = ctypes.c_double()
+            n = ctypes.c_long()
+            self._check(self.L.nsgpu_timer_read(self.h, i, ctypes.byref(ms), ctypes.byref(n)))
+            out[self.L.nsgpu_timer_name(i).decode()] = (ms.value, n.value)
+        return out

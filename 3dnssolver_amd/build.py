@@ -1,0 +1,51 @@
+"""Builds the HIP C-ABI library lib/libnsgpu.so for gfx950 (MI355X) with hipcc.
+
+ns_kernels.hip is compiled twice: once with -ffp-contract=off (EXACT: the reference's operation
+order, bitwise) and once with -ffp-contract=fast (FAST).  Every HIP compile disables packed FP32
+(-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "lib")
+ARCH = os.environ.get("NSGPU_ARCH", "gfx950")
+
+COMMON = ["-std=c++17", "-O3", "-fPIC", f"--offload-arch={ARCH}", "-fno-gpu-flush-denormals-to-zero",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    units = [
+        ("ns_kernels.hip", "kernels_exact.o", ["-DNS_FAST=0", "-ffp-contract=off"]),
+        ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=fast"]),
+        ("nsgpu_capi.hip", "capi.o", ["-ffp-contract=off"]),
+    ]
+    for src, obj, extra in units:
+        o = os.path.join(OUT, obj)
+        cmd = ["hipcc", "-c", *COMMON, *extra, os.path.join(CSRC, src), "-o", o]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(o)
+    lib = os.path.join(OUT, "libnsgpu.so")
+    tmp = lib + ".tmp"
+    _run(["hipcc", "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])
+    os.replace(tmp, lib)
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

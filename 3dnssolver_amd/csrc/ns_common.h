@@ -1,0 +1,83 @@
+// ns_common.h -- block geometry, device layout and the kernel launch table shared by the C-ABI
+// (nsgpu_capi.hip) and the kernel translation units (ns_kernels.hip, compiled once per math mode).
+//
+// Device layout: structure of arrays.  A 5-variable field is 5 planes of the padded block, each
+// plane row-major [NX][NY][NZ] with k fastest (the reference Array4d order with q moved out,
+// containers/array_4d.hpp:116-124), plane stride P = NX*NY*NZ, NX = nx + 2*NGC.  Loads along k
+// are unit-stride per variable, so a wavefront reading 64 consecutive k of one variable reads one
+// contiguous 512-byte run.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nsgpu {
+
+constexpr int NGC = 3;
+constexpr int NQ  = 5;
+
+struct Geo {
+    int    nx, ny, nz;  // interior cells
+    int    NX, NY, NZ;  // padded extents
+    int    sx, sy;      // strides of i and j (k stride is 1)
+    long   P;           // plane stride
+    double dx, dy, dz;
+    int    dim;
+};
+
+// per-face boundary action (resolved on the host from nsgpu_config::bc):
+// 0 = periodic wrap, 1 = Neumann copy
+struct Bnd {
+    int periodic[3];
+    int face_kind[6];  // W,E,N,S,F,B
+};
+
+struct Phys {
+    double gamma, mu, Pr, cfl;
+    int    conv;        // 0 none, 1 component-wise, 2 characteristic
+    int    borges;      // 0 Shu, 1 Borges
+    int    diff_order;  // 0, 2, 4, 6
+};
+
+// SSP-RK3 stage coefficients are formed on the device from dt (time_scheme.cpp:37-63)
+struct StageArgs {
+    const double* Un;     // step-start state (interior)
+    const double* Ucur;   // stage input state (halo-filled)
+    double*       Uout;   // stage output (interior)
+    double*       Wout;   // c2p(Uout) on the last stage (interior), else nullptr
+    int           stage;  // 0, 1, 2
+};
+
+// kernel launchers of one math mode (EXACT or FAST); all enqueue on `s`
+struct Launchers {
+    void (*reset_min)(unsigned long long* bits, hipStream_t s);
+    void (*step_begin)(const double* W, double* Ua, double* Un, unsigned long long* dtmin,
+                       Geo g, Phys ph, hipStream_t s);
+    void (*halo)(double* A, int nvar, Geo g, Bnd b, hipStream_t s);
+    void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
+    void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
+                      hipStream_t s);
+    void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
+    void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);
+    // R (interior) = R_in (or 0) + convection(fx,fy,fz) + diffusion(W, Dx, Dy); then either
+    // stores R (st == nullptr) or applies the SSP-RK3 stage update
+    void (*residual)(const double* fx, const double* fy, const double* fz, const double* W,
+                     const double* Dx, const double* Dy, const double* Rin, double* Rout,
+                     const StageArgs* st, const unsigned long long* dtmin, int which, Geo g,
+                     Phys ph, hipStream_t s);
+    void (*aos_to_soa)(const double* aos, double* soa, Geo g, hipStream_t s);
+    void (*soa_to_aos)(const double* soa, double* aos, Geo g, hipStream_t s);
+    void (*reduce)(const double* U, const double* W, double* partial, int nblocks_max,
+                   unsigned long long* dtmin, unsigned* flags, Geo g, Phys ph, hipStream_t s,
+                   int* nblocks_out);
+    void (*reduce_final)(const double* partial, int nblocks, double* sums, hipStream_t s);
+    void (*time_accum)(double* t, const unsigned long long* dtmin, double cfl, hipStream_t s);
+};
+
+namespace exact {
+const Launchers& launchers();
+}
+namespace fast {
+const Launchers& launchers();
+}
+
+}  // namespace nsgpu

@@ -1,0 +1,914 @@
+// ns_kernels.hip -- HIP kernels (gfx950) of the RHS + SSP-RK3 path.
+//
+// This translation unit is compiled twice:
+//   NS_FAST=0 with -ffp-contract=off   -> namespace nsgpu::exact
+//       every expression is evaluated in the reference's order with one rounding per operation
+//       (IEEE division and square root), so results are bitwise those of a strict build of the
+//       reference.  Multiplications by the structural zeros / ones of the axis-aligned
+//       eigenvector bases are skipped: x + (+-0) == x and 1*x == x exactly, so only the sign of
+//       an exact zero can differ.
+//   NS_FAST=1 with -ffp-contract=fast  -> namespace nsgpu::fast
+//       fused multiply-add, reciprocal hoisting, one division per WENO reconstruction.
+//
+// Citations are paths relative to the reference's src/ directory.
+#include "ns_common.h"
+
+#ifndef NS_FAST
+#error "compile with -DNS_FAST=0 or -DNS_FAST=1"
+#endif
+#if NS_FAST
+#define NS_MODE fast
+#else
+#define NS_MODE exact
+#endif
+
+namespace nsgpu {
+namespace NS_MODE {
+
+// ------------------------------------------------------------------------------------------
+// constants (convection_constants.hpp:4-12, cd_constants.hpp:9-27)
+// ------------------------------------------------------------------------------------------
+#define EPS_SHU 1E-6
+#define EPS_BORGES 1E-14
+#define ONEPERSIX (1. / 6.)
+#define THIRTEENPERTWELVE (13. / 12.)
+#define ONEPERFOUR (1. / 4.)
+#define LF_ALPHA 3.0  // ConvectionFlux::max_eigenvalue stores 3.0 (convection_flux.cpp:241)
+
+__constant__ double c_fd1[3][7] = {
+    {-1. / 2, 0., 1. / 2.},
+    {1. / 12., -2. / 3., 0., 2. / 3., -1. / 12.},
+    {-1. / 60., 3. / 20., -3. / 4., 0., 3. / 4., -3. / 20., 1. / 60.}};
+__constant__ double c_fd2[3][7] = {
+    {1., -2., 1.},
+    {-1. / 12., 4. / 3., -5. / 2., 4. / 3., -1. / 12.},
+    {1. / 90., -3. / 20., 3. / 2., -49. / 18., 3. / 2., -3. / 20., 1. / 90.}};
+
+// ------------------------------------------------------------------------------------------
+// WENO-5 (weno-5.cpp:222-322)
+// ------------------------------------------------------------------------------------------
+#if !NS_FAST
+template <bool BORGES>
+__device__ __forceinline__ double weno_combine(double a, double b, double c, double d, double e,
+                                               double d0, double d1, double d2, double f0,
+                                               double f1, double f2) {
+    double t1, t2;
+    t1        = a - 2. * b + c;
+    t2        = a - 4. * b + 3. * c;
+    double b0 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = b - 2. * c + d;
+    t2        = b - d;
+    double b1 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = c - 2. * d + e;
+    t2        = 3. * c - 4. * d + e;
+    double b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    double a0, a1, a2;
+    if constexpr (BORGES) {
+        double tau5 = fabs(b0 - b2);
+        a0          = d0 * (1.0 + (tau5 / (b0 + EPS_BORGES)));
+        a1          = d1 * (1.0 + (tau5 / (b1 + EPS_BORGES)));
+        a2          = d2 * (1.0 + (tau5 / (b2 + EPS_BORGES)));
+    } else {
+        a0 = d0 / ((EPS_SHU + b0) * (EPS_SHU + b0));
+        a1 = d1 / ((EPS_SHU + b1) * (EPS_SHU + b1));
+        a2 = d2 / ((EPS_SHU + b2) * (EPS_SHU + b2));
+    }
+    double al = a0 + a1 + a2;
+    return (a0 / al) * f0 + (a1 / al) * f1 + (a2 / al) * f2;
+}
+#else
+// One division per reconstruction: with s_k = (eps + b_k)^2 the Shu weights are
+// w_k = d_k prod_{j!=k} s_j / sum_m d_m prod_{j!=m} s_j; Borges: a_k = d_k (1 + tau/(b_k+eps))
+// = d_k (b_k + eps + tau) / (b_k + eps), same clearing of denominators.
+template <bool BORGES>
+__device__ __forceinline__ double weno_combine(double a, double b, double c, double d, double e,
+                                               double d0, double d1, double d2, double f0,
+                                               double f1, double f2) {
+    double t1, t2;
+    t1        = a - 2. * b + c;
+    t2        = a - 4. * b + 3. * c;
+    double b0 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = b - 2. * c + d;
+    t2        = b - d;
+    double b1 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = c - 2. * d + e;
+    t2        = 3. * c - 4. * d + e;
+    double b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    double p0, p1, p2;
+    if constexpr (BORGES) {
+        double tau5 = fabs(b0 - b2);
+        double e0 = b0 + EPS_BORGES, e1 = b1 + EPS_BORGES, e2 = b2 + EPS_BORGES;
+        p0 = d0 * (e0 + tau5) * (e1 * e2);
+        p1 = d1 * (e1 + tau5) * (e0 * e2);
+        p2 = d2 * (e2 + tau5) * (e0 * e1);
+    } else {
+        double s0 = (EPS_SHU + b0) * (EPS_SHU + b0);
+        double s1 = (EPS_SHU + b1) * (EPS_SHU + b1);
+        double s2 = (EPS_SHU + b2) * (EPS_SHU + b2);
+        p0        = d0 * (s1 * s2);
+        p1        = d1 * (s0 * s2);
+        p2        = d2 * (s0 * s1);
+    }
+    return (p0 * f0 + p1 * f1 + p2 * f2) / (p0 + p1 + p2);
+}
+#endif
+
+// Weno5::WENO5_left (weno-5.cpp:305-322), ideal weights d_left = {0.1, 0.6, 0.3}
+template <bool BORGES>
+__device__ __forceinline__ double weno_left(double a, double b, double c, double d, double e) {
+    double f0 = (2. * a - 7. * b + 11. * c) * ONEPERSIX;
+    double f1 = (-b + 5. * c + 2. * d) * ONEPERSIX;
+    double f2 = (2. * c + 5. * d - e) * ONEPERSIX;
+    return weno_combine<BORGES>(a, b, c, d, e, 0.1, 0.6, 0.3, f0, f1, f2);
+}
+
+// Weno5::WENO5_right (weno-5.cpp:285-303), ideal weights d_right = {0.3, 0.6, 0.1}
+template <bool BORGES>
+__device__ __forceinline__ double weno_right(double a, double b, double c, double d, double e) {
+    double f0 = (-a + 5. * b + 2. * c) * ONEPERSIX;
+    double f1 = (2. * b + 5. * c - d) * ONEPERSIX;
+    double f2 = (11. * c - 7. * d + 2. * e) * ONEPERSIX;
+    return weno_combine<BORGES>(a, b, c, d, e, 0.3, 0.6, 0.1, f0, f1, f2);
+}
+
+// ------------------------------------------------------------------------------------------
+// Euler flux and Lax-Friedrichs split (convection_flux.cpp:23-61, weno-5.cpp:94-110)
+// ------------------------------------------------------------------------------------------
+
+// Reads the 5 primitive and 5 conservative values of one cell and forms
+// fL = (F + alpha U) / 2, fR = (F - alpha U) / 2 for the direction DIR.
+template <int DIR>
+__device__ __forceinline__ void split_cell(const double* __restrict__ U, const double* __restrict__ W,
+                                           long c, long P, double gamma, double fl[5],
+                                           double fr[5]) {
+    double rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
+    double qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
+    double cs  = sqrt(gamma * p / rho);
+    double H   = cs * cs / (gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+    double F[5];
+    F[0] = rho * qn;
+    F[1] = rho * qn * H;
+    F[2] = (DIR == 0) ? rho * qn * u + p : rho * qn * u;
+    F[3] = (DIR == 1) ? rho * qn * v + p : rho * qn * v;
+    F[4] = (DIR == 2) ? rho * qn * w + p : rho * qn * w;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double uq = U[c + q * P];
+        fl[q]     = (F[q] + LF_ALPHA * uq) * 0.5;
+        fr[q]     = (F[q] - LF_ALPHA * uq) * 0.5;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Characteristic projection (convection_flux.cpp:63-175, weno-5cw.cpp:4-28)
+// ------------------------------------------------------------------------------------------
+
+// axis bases of ConvectionFlux::orthogonal_unit_vectors: n = e_DIR,
+// x: l = (0,0,1), m = (0,-1,0); y: l = (0,0,1), m = (1,0,0); z: l = (0,-1,0), m = (1,0,0)
+template <int DIR, int A> struct Basis;
+template <int A> struct Basis<0, A> {
+    static constexpr int n = (A == 0), l = (A == 2), m = (A == 1) ? -1 : 0;
+};
+template <int A> struct Basis<1, A> {
+    static constexpr int n = (A == 1), l = (A == 2), m = (A == 0);
+};
+template <int A> struct Basis<2, A> {
+    static constexpr int n = (A == 2), l = (A == 1) ? -1 : 0, m = (A == 0);
+};
+
+// acc + K*x for a structural constant K in {-1, 0, 1}
+template <int K> __device__ __forceinline__ double cadd(double acc, double x) {
+    if constexpr (K == 1) return acc + x;
+    else if constexpr (K == -1) return acc - x;
+    else return acc;
+}
+
+// the eigenvector entries that depend on the face state (ConvectionFlux::eigenvectors,
+// convection_flux.cpp:117-175, c^2 = gamma / (p rho) as written there)
+struct Eig {
+    double vel[3];
+    double c, H, qn, ql, qm, q2h, K_pc;
+    double L0[5], L2[5], L3[5];  // rows 0, 2, 3 of L (all entries depend on the state)
+};
+
+template <int DIR>
+__device__ __forceinline__ void eigen(double rho, double p, double u, double v, double w,
+                                      double gamma, Eig& E) {
+    E.vel[0]    = u;
+    E.vel[1]    = v;
+    E.vel[2]    = w;
+    double c2   = gamma / (p * rho);
+    double c    = sqrt(c2);
+    double qn   = E.vel[DIR];
+    // ql = u*lx + v*ly + w*lz, qm = u*mx + v*my + w*mz with one nonzero entry each
+    double ql   = (DIR == 2) ? -v : w;
+    double qm   = (DIR == 0) ? -v : u;
+    double q2   = u * u + v * v + w * w;
+    double H    = c2 / (gamma - 1.0) + 0.5 * q2;
+    double K    = gamma - 1.0;
+    double K_pc = K / (2.0 * c2);
+    double t    = 0.5 * K_pc * q2;
+    double qn2c = qn / (2.0 * c);
+    double i2c  = 1.0 / (2.0 * c);  // n_a / (2c) for the normal component
+    E.c = c, E.H = H, E.qn = qn, E.ql = ql, E.qm = qm, E.q2h = 0.5 * q2, E.K_pc = K_pc;
+    E.L0[0] = t + qn2c;
+    E.L3[0] = t - qn2c;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double kv = K_pc * E.vel[a];
+        if (a == DIR) {
+            E.L0[1 + a] = -(kv + i2c);
+            E.L3[1 + a] = -(kv - i2c);
+        } else {
+            E.L0[1 + a] = -(kv);
+            E.L3[1 + a] = -(kv);
+        }
+        E.L2[1 + a] = K * E.vel[a] / c2;
+    }
+    E.L0[4] = K_pc;
+    E.L3[4] = K_pc;
+    E.L2[0] = 1.0 - K_pc * q2;
+    E.L2[4] = -K / c2;
+}
+
+// row q of L times the split stencil values s[p] (p in conservative order), summed in p order
+// from 0.0 (matrix::square_matrix_matrix_product, matrix.cpp:12-20)
+template <int DIR, int Q>
+__device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
+    if constexpr (Q == 0) {
+        return E.L0[0] * s[0] + E.L0[1] * s[1] + E.L0[2] * s[2] + E.L0[3] * s[3] + E.L0[4] * s[4];
+    } else if constexpr (Q == 2) {
+        return E.L2[0] * s[0] + E.L2[1] * s[1] + E.L2[2] * s[2] + E.L2[3] * s[3] + E.L2[4] * s[4];
+    } else if constexpr (Q == 3) {
+        return E.L3[0] * s[0] + E.L3[1] * s[1] + E.L3[2] * s[2] + E.L3[3] * s[3] + E.L3[4] * s[4];
+    } else if constexpr (Q == 1) {  // {-qm, mx, my, mz, 0}
+        double acc = (-E.qm) * s[0];
+        acc        = cadd<Basis<DIR, 0>::m>(acc, s[1]);
+        acc        = cadd<Basis<DIR, 1>::m>(acc, s[2]);
+        acc        = cadd<Basis<DIR, 2>::m>(acc, s[3]);
+        return acc;
+    } else {  // Q == 4: {-ql, lx, ly, lz, 0}
+        double acc = (-E.ql) * s[0];
+        acc        = cadd<Basis<DIR, 0>::l>(acc, s[1]);
+        acc        = cadd<Basis<DIR, 1>::l>(acc, s[2]);
+        acc        = cadd<Basis<DIR, 2>::l>(acc, s[3]);
+        return acc;
+    }
+}
+
+// R times the characteristic fluxes ft (matrix::square_matrix_vector_product, matrix.cpp:4-10)
+template <int DIR, int A>
+__device__ __forceinline__ double Rvel(const Eig& E, const double ft[5]) {
+    // row 1+A: {vel_A - c n_A, m_A, vel_A, vel_A + c n_A, l_A}
+    constexpr int n = Basis<DIR, A>::n;
+    double        va = E.vel[A];
+    double        e0 = n ? va - E.c : va;
+    double        e3 = n ? va + E.c : va;
+    double        acc = e0 * ft[0];
+    acc               = cadd<Basis<DIR, A>::m>(acc, ft[1]);
+    acc               = acc + va * ft[2];
+    acc               = acc + e3 * ft[3];
+    acc               = cadd<Basis<DIR, A>::l>(acc, ft[4]);
+    return acc;
+}
+
+template <int DIR>
+__device__ __forceinline__ void Rmul(const Eig& E, const double ft[5], double out[5]) {
+    out[0] = ft[0] + ft[2] + ft[3];  // {1, 0, 1, 1, 0}
+    out[1] = Rvel<DIR, 0>(E, ft);
+    out[2] = Rvel<DIR, 1>(E, ft);
+    out[3] = Rvel<DIR, 2>(E, ft);
+    double qc = E.qn * E.c;          // {H - qn c, qm, q2/2, H + qn c, ql}
+    out[4] = (E.H - qc) * ft[0] + E.qm * ft[1] + E.q2h * ft[2] + (E.H + qc) * ft[3] + E.ql * ft[4];
+}
+
+template <int DIR, bool BORGES, int Q>
+__device__ __forceinline__ double char_component(const Eig& E, const double fl[6][5],
+                                                 const double fr[6][5]) {
+    // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
+    double a[5], b[5];
+#pragma unroll
+    for (int ii = 0; ii < 5; ii++) {
+        a[ii] = Lrow<DIR, Q>(E, fl[ii]);
+        b[ii] = Lrow<DIR, Q>(E, fr[ii + 1]);
+    }
+    return weno_left<BORGES>(a[0], a[1], a[2], a[3], a[4]) +
+           weno_right<BORGES>(b[0], b[1], b[2], b[3], b[4]);
+}
+
+// ------------------------------------------------------------------------------------------
+// face-flux kernel: one thread per face i+1/2 (stored at cell i) of the sweep direction
+// (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
+// ------------------------------------------------------------------------------------------
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(256) void k_face_flux(const double* __restrict__ U,
+                                                   const double* __restrict__ W,
+                                                   double* __restrict__ f, Geo g, double gamma) {
+    int k = threadIdx.x + blockIdx.x * blockDim.x;
+    int j = threadIdx.y + blockIdx.y * blockDim.y;
+    int i = blockIdx.z;
+    // face ranges: the sweep index runs from begin-1 to end-1, the others over the interior
+    int nk = g.nz + (DIR == 2), nj = g.ny + (DIR == 1);
+    if (k >= nk || j >= nj) return;
+    i += NGC - (DIR == 0);
+    j += NGC - (DIR == 1);
+    k += NGC - (DIR == 2);
+    const long P = g.P;
+    const long c = (long)i * g.sx + (long)j * g.sy + k;
+    const long s = (DIR == 0) ? g.sx : (DIR == 1) ? g.sy : 1;
+
+    // split fluxes of cells i-2 .. i+3
+    double fl[6][5], fr[6][5];
+#pragma unroll
+    for (int ii = 0; ii < 6; ii++) split_cell<DIR>(U, W, c + (ii - 2) * s, P, gamma, fl[ii], fr[ii]);
+
+    double out[5];
+    if constexpr (CONV == 1) {
+#pragma unroll
+        for (int q = 0; q < 5; q++)
+            out[q] = weno_left<BORGES>(fl[0][q], fl[1][q], fl[2][q], fl[3][q], fl[4][q]) +
+                     weno_right<BORGES>(fr[1][q], fr[2][q], fr[3][q], fr[4][q], fr[5][q]);
+    } else {
+        double Wa[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) Wa[q] = 0.5 * (W[c + q * P] + W[c + s + q * P]);
+        Eig E;
+        eigen<DIR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
+        double ft[5];
+        ft[0] = char_component<DIR, BORGES, 0>(E, fl, fr);
+        ft[1] = char_component<DIR, BORGES, 1>(E, fl, fr);
+        ft[2] = char_component<DIR, BORGES, 2>(E, fl, fr);
+        ft[3] = char_component<DIR, BORGES, 3>(E, fl, fr);
+        ft[4] = char_component<DIR, BORGES, 4>(E, fl, fr);
+        Rmul<DIR>(E, ft, out);
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) f[c + q * P] = out[q];
+}
+
+// ------------------------------------------------------------------------------------------
+// equation of state (eq_of_state.cpp:32-101)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void c2p(const double u[5], double gamma, double w[5]) {
+    w[0]       = u[0];
+    w[2]       = u[2] / u[0];
+    w[3]       = u[3] / u[0];
+    w[4]       = u[4] / u[0];
+    double kin = 0.5 * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
+    w[1]       = (gamma - 1.0) * (u[1] - u[0] * kin);
+}
+
+__device__ __forceinline__ void p2c(const double w[5], double gamma, double u[5]) {
+    u[0]       = w[0];
+    u[2]       = w[0] * w[2];
+    u[3]       = w[0] * w[3];
+    u[4]       = w[0] * w[4];
+    double kin = 0.5 * w[0] * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
+    u[1]       = w[1] / (gamma - 1.0) + kin;
+}
+
+__device__ __forceinline__ double local_dt(const double w[5], double gamma, double dx, double dy,
+                                           double dz) {
+    double c  = sqrt(gamma * w[1] / w[0]);
+    double ax = fabs(w[2] + c), bx = fabs(w[2] - c);
+    double ay = fabs(w[3] + c), by = fabs(w[3] - c);
+    double az = fabs(w[4] + c), bz = fabs(w[4] - c);
+    double ex  = dx / ((ax < bx) ? bx : ax);  // std::max
+    double ey  = dy / ((ay < by) ? by : ay);
+    double ez  = dz / ((az < bz) ? bz : az);
+    double tmp = (ey < ex) ? ey : ex;  // std::min
+    return (ez < tmp) ? ez : tmp;
+}
+
+// min over a workgroup of positive doubles, then one atomicMin on the bit pattern (positive
+// IEEE doubles order like their unsigned 64-bit patterns; NaN sorts above every finite value,
+// matching `if (dt < max_dt)` in time_scheme.cpp:119 which never picks a NaN)
+__device__ __forceinline__ void block_min_atomic(double v, unsigned long long* out) {
+    __shared__ double red[16];
+    for (int o = 32; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, 64);
+        v        = (t < v) ? t : v;
+    }
+    int lane = (threadIdx.x + threadIdx.y * blockDim.x) & 63;
+    int wid  = (threadIdx.x + threadIdx.y * blockDim.x) >> 6;
+    int nw   = (blockDim.x * blockDim.y + 63) >> 6;
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    if (wid == 0) {
+        double x = (lane < nw) ? red[lane] : 1E300;
+        for (int o = 8; o > 0; o >>= 1) {
+            double t = __shfl_xor(x, o, 64);
+            x        = (t < x) ? t : x;
+        }
+        if (lane == 0 && x < 1E300) atomicMin(out, (unsigned long long)__double_as_longlong(x));
+    }
+}
+
+__global__ void k_reset_min(unsigned long long* bits) {
+    *bits = (unsigned long long)__double_as_longlong(1E10);  // max_dt = 1E10 (time_scheme.cpp:104)
+}
+
+// step prologue: dt from W (time_scheme.cpp:16, :98-136), U = p2c(W) (:25), Uprev = U (:27)
+__global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W,
+                                                    double* __restrict__ Ua,
+                                                    double* __restrict__ Un,
+                                                    unsigned long long* dtmin, Geo g, double gamma) {
+    int  k     = threadIdx.x + blockIdx.x * blockDim.x;
+    int  j     = threadIdx.y + blockIdx.y * blockDim.y;
+    int  i     = blockIdx.z;
+    bool valid = (k < g.nz && j < g.ny);
+    double dt  = 1E300;
+    if (valid) {
+        long   c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+        double w[5], u[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) w[q] = W[c + q * g.P];
+        dt = local_dt(w, gamma, g.dx, g.dy, g.dz);
+        p2c(w, gamma, u);
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            Ua[c + q * g.P] = u[q];
+            Un[c + q * g.P] = u[q];
+        }
+    }
+    block_min_atomic(dt, dtmin);
+}
+
+// ------------------------------------------------------------------------------------------
+// halo fill: Neumann faces (boundary_neumann.cpp:22-132) and the periodic face exchange of one
+// rank (communicator.cpp:16-144 with the slicers of array_4d.cpp:39-164).  The six face regions
+// are disjoint and read interior cells only, so one launch covers them in any order.
+// ------------------------------------------------------------------------------------------
+__global__ void k_halo(double* A, int nvar, Geo g, Bnd b) {
+    int t2   = threadIdx.x + blockIdx.x * blockDim.x;  // fastest tangential index
+    int t1   = threadIdx.y + blockIdx.y * blockDim.y;
+    int face = blockIdx.z / NGC;
+    int h    = blockIdx.z % NGC;
+    int axis = face >> 1;
+    int n1, n2;  // tangential extents
+    if (axis == 0) n1 = g.ny, n2 = g.nz;
+    else if (axis == 1) n1 = g.nx, n2 = g.nz;
+    else n1 = g.nx, n2 = g.ny;
+    if (t1 >= n1 || t2 >= n2) return;
+    int  n   = (axis == 0) ? g.nx : (axis == 1) ? g.ny : g.nz;
+    // face order W,E,N,S,F,B: low side for W (0), N (2), B (5); high side for E, S, F
+    bool low = (face == 0 || face == 2 || face == 5);
+    int  dst = low ? h : NGC + n + h;
+    int  src;
+    if (b.periodic[axis]) src = low ? n + h : NGC + h;
+    else src = low ? NGC : NGC + n - 1;
+    long ct, cd, cs;
+    if (axis == 0) {
+        ct = (long)(t1 + NGC) * g.sy + (t2 + NGC);
+        cd = ct + (long)dst * g.sx;
+        cs = ct + (long)src * g.sx;
+    } else if (axis == 1) {
+        ct = (long)(t1 + NGC) * g.sx + (t2 + NGC);
+        cd = ct + (long)dst * g.sy;
+        cs = ct + (long)src * g.sy;
+    } else {
+        ct = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy;
+        cd = ct + dst;
+        cs = ct + src;
+    }
+    for (int q = 0; q < nvar; q++) A[cd + q * g.P] = A[cs + q * g.P];
+}
+
+// W = c2p(U) on the interior and the face halos (edges and corners are never read)
+__global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
+                                                   double* __restrict__ W, Geo g, double gamma) {
+    int k = threadIdx.x + blockIdx.x * blockDim.x;
+    int j = threadIdx.y + blockIdx.y * blockDim.y;
+    int i = blockIdx.z;
+    if (k >= g.NZ || j >= g.NY) return;
+    int outside = (i < NGC || i >= NGC + g.nx) + (j < NGC || j >= NGC + g.ny) +
+                  (k < NGC || k >= NGC + g.nz);
+    if (outside > 1) return;
+    long   c = (long)i * g.sx + (long)j * g.sy + k;
+    double u[5], w[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) u[q] = U[c + q * g.P];
+    c2p(u, gamma, w);
+#pragma unroll
+    for (int q = 0; q < 5; q++) W[c + q * g.P] = w[q];
+}
+
+// ------------------------------------------------------------------------------------------
+// viscous terms (centered_diffusion.cpp:3-182, cd-n.cpp:91-165, diffusion_scheme.cpp:3-58)
+// ------------------------------------------------------------------------------------------
+
+// first derivatives of rho, p, u, v along x and y on the interior:
+// sum over the stencil of f(. + s + k) * c_k / d, in k order, the zero centre term skipped
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
+                                                double* __restrict__ Dx, double* __restrict__ Dy,
+                                                Geo g) {
+    int k = threadIdx.x + blockIdx.x * blockDim.x;
+    int j = threadIdx.y + blockIdx.y * blockDim.y;
+    int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    constexpr int  H  = ORDER / 2;
+    constexpr int  OI = ORDER / 2 - 1;
+    const long     c  = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double* f  = W + q * g.P + c;
+        double        sx = 0., sy = 0.;
+        bool          first = true;
+#pragma unroll
+        for (int t = 0; t <= 2 * H; t++) {
+            if (t == H) continue;
+            double co = c_fd1[OI][t];
+            double vx = f[(long)(t - H) * g.sx] * co / g.dx;
+            double vy = f[(long)(t - H) * g.sy] * co / g.dy;
+            if (first) {
+                sx = vx, sy = vy, first = false;
+            } else {
+                sx += vx, sy += vy;
+            }
+        }
+        Dx[c + q * g.P] = sx;
+        Dy[c + q * g.P] = sy;
+    }
+}
+
+// periodic exchange of the derivative halos that the cross derivatives read: the j halos of
+// dW/dx and the i halos of dW/dy (centered_diffusion.cpp:22-24).  At a physical wall the halos
+// stay zero: the arrays are zeroed once at allocation and never written there.
+__global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
+    int k    = threadIdx.x + blockIdx.x * blockDim.x;
+    int t1   = threadIdx.y + blockIdx.y * blockDim.y;
+    int side = blockIdx.z / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
+    int h    = blockIdx.z % NGC;
+    bool low = ((blockIdx.z / NGC) & 1) == 0;
+    if (k >= g.nz) return;
+    if (side == 0) {
+        if (!b.periodic[1] || t1 >= g.nx) return;
+        long base = (long)(t1 + NGC) * g.sx + (k + NGC);
+        int  dst  = low ? h : NGC + g.ny + h;
+        int  src  = low ? g.ny + h : NGC + h;
+        for (int q = 0; q < 4; q++)
+            Dx[base + (long)dst * g.sy + q * g.P] = Dx[base + (long)src * g.sy + q * g.P];
+    } else {
+        if (!b.periodic[0] || t1 >= g.ny) return;
+        long base = (long)(t1 + NGC) * g.sy + (k + NGC);
+        int  dst  = low ? h : NGC + g.nx + h;
+        int  src  = low ? g.nx + h : NGC + h;
+        for (int q = 0; q < 4; q++)
+            Dy[base + (long)dst * g.sx + q * g.P] = Dy[base + (long)src * g.sx + q * g.P];
+    }
+}
+
+// one stencil sum along `stride` of the first- (SECOND=0) or second-derivative coefficients
+template <int ORDER, bool SECOND>
+__device__ __forceinline__ double fd_sum(const double* f, long stride, double d) {
+    constexpr int H  = ORDER / 2;
+    constexpr int OI = ORDER / 2 - 1;
+    double        s  = 0.;
+    bool          first = true;
+#pragma unroll
+    for (int t = 0; t <= 2 * H; t++) {
+        if (!SECOND && t == H) continue;
+        double co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
+        double v  = f[(long)(t - H) * stride] * co / d;
+        if (first) {
+            s = v, first = false;
+        } else {
+            s += v;
+        }
+    }
+    return s;
+}
+
+// CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
+// constant-viscosity zero terms (dmu = dlamb = 0) dropped: each is an exact +-0 addend.
+template <int ORDER>
+__device__ __forceinline__ void viscous(const double* __restrict__ W,
+                                        const double* __restrict__ Dx,
+                                        const double* __restrict__ Dy, long c, const Geo& g,
+                                        const Phys& ph, double& ene, double& x_mom,
+                                        double& y_mom) {
+    const long P   = g.P;
+    const int  rho = 0, p = 1, u = 2, v = 3;
+    double     dxx = g.dx * g.dx, dyy = g.dy * g.dy;
+    // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79)
+    double du_dxdx   = fd_sum<ORDER, true>(W + u * P + c, g.sx, dxx);
+    double du_dydy   = fd_sum<ORDER, true>(W + u * P + c, g.sx, dyy);
+    double dv_dxdx   = fd_sum<ORDER, true>(W + v * P + c, g.sx, dxx);
+    double dv_dydy   = fd_sum<ORDER, true>(W + v * P + c, g.sx, dyy);
+    double drho_dxdx = fd_sum<ORDER, true>(W + rho * P + c, g.sx, dxx);
+    double drho_dydy = fd_sum<ORDER, true>(W + rho * P + c, g.sx, dyy);
+    double dp_dxdx   = fd_sum<ORDER, true>(W + p * P + c, g.sx, dxx);
+    double dp_dydy   = fd_sum<ORDER, true>(W + p * P + c, g.sx, dyy);
+    // cross derivatives: dW_dxdy = df_dy(dW_dx), dW_dydx = df_dx(dW_dy)
+    double du_dxdy = fd_sum<ORDER, false>(Dx + u * P + c, g.sy, g.dy);
+    double dv_dxdy = fd_sum<ORDER, false>(Dx + v * P + c, g.sy, g.dy);
+    double du_dydx = fd_sum<ORDER, false>(Dy + u * P + c, g.sx, g.dx);
+    double dv_dydx = fd_sum<ORDER, false>(Dy + v * P + c, g.sx, g.dx);
+
+    double du_dx = Dx[u * P + c], dv_dx = Dx[v * P + c], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
+    double du_dy = Dy[u * P + c], dv_dy = Dy[v * P + c], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
+    double U = W[u * P + c], V = W[v * P + c], Pp = W[p * P + c], RHO = W[rho * P + c];
+
+    double mu   = ph.mu;
+    double lamb = (-2. / 3.) * mu;
+    double Dtauxx_dx = 2.0 * mu * du_dxdx + lamb * (du_dxdx + dv_dydx);
+    double Dtauyy_dy = 2.0 * mu * dv_dydy + lamb * (du_dxdy + dv_dydy);
+    double Dtauxy_dx = mu * (du_dydx + dv_dxdx);
+    double Dtauyx_dy = mu * (du_dydy + dv_dxdy);
+    double Tauxx     = 2.0 * mu * du_dx + lamb * (du_dx + dv_dy);
+    double Tauyy     = Tauxx;  // tauxx(...) as written at centered_diffusion.cpp:143
+    double Tauxy     = mu * (du_dy + dv_dx);
+    double Tauyx     = Tauxy;
+    double Dtauvx_dx = U * Dtauxx_dx + V * Dtauxy_dx + Tauxx * du_dx + Tauxy * dv_dy;
+    double Dtauvy_dy = U * Dtauyx_dy + V * Dtauyy_dy + Tauyy * dv_dy + Tauyx * du_dy;
+    double one_rho   = 1. / RHO;
+    double one_rho2  = 1. / (RHO * RHO);
+    double one_rho3  = 1. / (RHO * RHO * RHO);
+    double temp2x = mu * (2.0 * one_rho2 * drho_dx * dp_dx + one_rho * dp_dxdx +
+                          one_rho3 * Pp * drho_dx * drho_dx - one_rho2 * Pp * drho_dxdx);
+    double temp3  = ph.gamma / (ph.Pr * (ph.gamma - 1.0));
+    double Dqx_dx = -temp3 * temp2x;
+    // the y heat-flux term multiplies by the enum value p = 1 (centered_diffusion.cpp:166)
+    double temp2y = mu * (2.0 * one_rho2 * drho_dy * dp_dy + one_rho * dp_dydy +
+                          one_rho3 * drho_dy * drho_dy - one_rho2 * Pp * drho_dydy);
+    double Dqy_dy = -temp3 * temp2y;
+    x_mom         = -Dtauxx_dx - Dtauyx_dy;
+    y_mom         = -Dtauyy_dy - Dtauxy_dx;
+    ene           = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
+}
+
+// residual + stage update at one interior cell:
+// R = R_in + (fx_i - fx_{i-1})/dx + (fy_j - fy_{j-1})/dy + (fz_k - fz_{k-1})/dz
+// (Weno5::compute_residual, weno-5.cpp:39-73), then the viscous terms (centered_diffusion.cpp:
+// 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96).
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_residual(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
+    const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
+    const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
+    int k = threadIdx.x + blockIdx.x * blockDim.x;
+    int j = threadIdx.y + blockIdx.y * blockDim.y;
+    int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+    const long P = g.P;
+    double     R[5];
+    bool       have[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        have[q] = (Rin != nullptr);
+        R[q]    = Rin ? Rin[c + q * P] : 0.0;
+    }
+    if ((which & 1) && ph.conv) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double a = (fx[c + q * P] - fx[c - g.sx + q * P]) / g.dx;
+            R[q]     = have[q] ? R[q] + a : a;
+            R[q] += (fy[c + q * P] - fy[c - g.sy + q * P]) / g.dy;
+            if (g.dim == 3) R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) / g.dz;
+            have[q] = true;
+        }
+    }
+    if constexpr (ORDER > 0) {
+        if (which & 2) {
+            double ene, xm, ym;
+            viscous<ORDER>(W, Dx, Dy, c, g, ph, ene, xm, ym);
+            R[1] = have[1] ? R[1] + ene : ene;
+            R[2] = have[2] ? R[2] + xm : xm;
+            R[3] = have[3] ? R[3] + ym : ym;
+            have[1] = have[2] = have[3] = true;
+        }
+    }
+    if (st.Uout == nullptr) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) Rout[c + q * P] = have[q] ? R[q] : 0.0;
+        return;
+    }
+    // SSP-RK3 stage (time_scheme.cpp:37-63): dt = CFL * min local dt (time_scheme.cpp:129)
+    double dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
+    double m1, m2, m3;
+    if (st.stage == 0) {
+        m1 = 1.0, m2 = 0.0, m3 = -dt;
+    } else if (st.stage == 1) {
+        m1 = 3.0 / 4.0, m2 = 1.0 / 4.0, m3 = -(1. / 4.) * dt;
+    } else {
+        m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
+    }
+    double un[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        double r = have[q] ? R[q] : 0.0;
+        un[q]    = (st.Un[c + q * P] * m1) + (st.Ucur[c + q * P] * m2) + (r * m3);
+        st.Uout[c + q * P] = un[q];
+    }
+    if (st.Wout) {  // conservative_to_primitive after the last stage (time_scheme.cpp:69)
+        double w[5];
+        c2p(un, ph.gamma, w);
+#pragma unroll
+        for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
+    }
+}
+
+__global__ void k_time_accum(double* t, const unsigned long long* dtmin, double cfl) {
+    *t = *t + __longlong_as_double((long long)*dtmin) * cfl;  // time = time + dt (driver.cpp:57)
+}
+
+// ------------------------------------------------------------------------------------------
+// host layout conversion: reference AoS [NX][NY][NZ][5] <-> device SoA planes
+// ------------------------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long P) {
+    long c = threadIdx.x + (long)blockIdx.x * blockDim.x;
+    if (c >= P) return;
+#pragma unroll
+    for (int q = 0; q < 5; q++) soa[c + q * P] = aos[c * 5 + q];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long P) {
+    long c = threadIdx.x + (long)blockIdx.x * blockDim.x;
+    if (c >= P) return;
+#pragma unroll
+    for (int q = 0; q < 5; q++) aos[c * 5 + q] = soa[c + q * P];
+}
+
+// ------------------------------------------------------------------------------------------
+// monitoring reductions: conservation sums (fixed-order two-pass tree, deterministic), CFL
+// minimum and health flags
+// ------------------------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict__ U,
+                                                        const double* __restrict__ W,
+                                                        double* __restrict__ partial,
+                                                        unsigned long long* dtmin,
+                                                        unsigned* flags, Geo g, Phys ph) {
+    __shared__ double sh[5][RED_THREADS];
+    long   n     = (long)g.nx * g.ny * g.nz;
+    double acc[5] = {0, 0, 0, 0, 0};
+    double dtl    = 1E300;
+    unsigned fl   = 0;
+    for (long t = threadIdx.x + (long)blockIdx.x * RED_THREADS; t < n; t += (long)gridDim.x * RED_THREADS) {
+        int  k = (int)(t % g.nz);
+        long r = t / g.nz;
+        int  j = (int)(r % g.ny);
+        int  i = (int)(r / g.ny);
+        long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+        double w[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double u = U[c + q * g.P];
+            acc[q] += u;
+            w[q] = W[c + q * g.P];
+            if (!isfinite(u) || !isfinite(w[q])) fl |= 1u;
+        }
+        if (!(w[0] > 0.0) || !(w[1] > 0.0)) fl |= 2u;
+        else {
+            double cs = sqrt(ph.gamma * w[1] / w[0]);
+            double m  = fmax(fabs(w[2]), fmax(fabs(w[3]), fabs(w[4]))) + cs;
+            if (m > LF_ALPHA) fl |= 4u;
+            double d = local_dt(w, ph.gamma, g.dx, g.dy, g.dz);
+            dtl      = (d < dtl) ? d : dtl;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+    if (fl) atomicOr(flags, fl);
+    block_min_atomic(dtl, dtmin);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partial,
+                                                              int nb, double* __restrict__ sums) {
+    __shared__ double sh[5][RED_THREADS];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = threadIdx.x; b < nb; b += RED_THREADS)
+        for (int q = 0; q < 5; q++) acc[q] += partial[b * 5 + q];
+    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) sums[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// ------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------
+static dim3 cell_grid(int n2, int n1, int n0, dim3 blk) {
+    return dim3((n2 + blk.x - 1) / blk.x, (n1 + blk.y - 1) / blk.y, n0);
+}
+static const dim3 BLK(64, 4, 1);
+
+static void l_reset_min(unsigned long long* bits, hipStream_t s) {
+    k_reset_min<<<1, 1, 0, s>>>(bits);
+}
+
+static void l_step_begin(const double* W, double* Ua, double* Un, unsigned long long* dtmin,
+                         Geo g, Phys ph, hipStream_t s) {
+    k_step_begin<<<cell_grid(g.nz, g.ny, g.nx, BLK), BLK, 0, s>>>(W, Ua, Un, dtmin, g, ph.gamma);
+}
+
+static void l_halo(double* A, int nvar, Geo g, Bnd b, hipStream_t s) {
+    int  m1 = g.ny > g.nx ? g.ny : g.nx;
+    int  m2 = g.nz > g.ny ? g.nz : g.ny;
+    dim3 blk(64, 4, 1);
+    dim3 grd((m2 + 63) / 64, (m1 + 3) / 4, 6 * NGC);
+    k_halo<<<grd, blk, 0, s>>>(A, nvar, g, b);
+}
+
+static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t s) {
+    k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
+}
+
+template <int DIR>
+static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
+                          hipStream_t s) {
+    dim3 grd = cell_grid(g.nz + (DIR == 2), g.ny + (DIR == 1), g.nx + (DIR == 0), BLK);
+    if (ph.conv == 1) {
+        if (ph.borges) k_face_flux<DIR, 1, true><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+        else k_face_flux<DIR, 1, false><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+    } else {
+        if (ph.borges) k_face_flux<DIR, 2, true><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+        else k_face_flux<DIR, 2, false><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+    }
+}
+
+static void l_face_flux(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
+                        hipStream_t s) {
+    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, s);
+    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, s);
+    else face_flux_dir<2>(U, W, f, g, ph, s);
+}
+
+static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {
+    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
+    if (ph.diff_order == 2) k_deriv1<2><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
+    else if (ph.diff_order == 4) k_deriv1<4><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
+    else k_deriv1<6><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
+}
+
+static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s) {
+    int  m1 = g.nx > g.ny ? g.nx : g.ny;
+    dim3 blk(64, 4, 1);
+    dim3 grd((g.nz + 63) / 64, (m1 + 3) / 4, 4 * NGC);
+    k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b);
+}
+
+static void l_residual(const double* fx, const double* fy, const double* fz, const double* W,
+                       const double* Dx, const double* Dy, const double* Rin, double* Rout,
+                       const StageArgs* st, const unsigned long long* dtmin, int which, Geo g,
+                       Phys ph, hipStream_t s) {
+    StageArgs a{};
+    if (st) a = *st;
+    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
+    switch (ph.diff_order) {
+    case 2: k_residual<2><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
+    case 4: k_residual<4><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
+    case 6: k_residual<6><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
+    default: k_residual<0><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
+    }
+}
+
+static void l_aos_to_soa(const double* aos, double* soa, Geo g, hipStream_t s) {
+    k_aos_to_soa<<<(unsigned)((g.P + 255) / 256), 256, 0, s>>>(aos, soa, g.P);
+}
+static void l_soa_to_aos(const double* soa, double* aos, Geo g, hipStream_t s) {
+    k_soa_to_aos<<<(unsigned)((g.P + 255) / 256), 256, 0, s>>>(soa, aos, g.P);
+}
+
+static void l_reduce(const double* U, const double* W, double* partial, int nblocks_max,
+                     unsigned long long* dtmin, unsigned* flags, Geo g, Phys ph, hipStream_t s,
+                     int* nblocks_out) {
+    long n  = (long)g.nx * g.ny * g.nz;
+    long nb = (n + RED_THREADS - 1) / RED_THREADS;
+    if (nb > nblocks_max) nb = nblocks_max;
+    k_reduce<<<(unsigned)nb, RED_THREADS, 0, s>>>(U, W, partial, dtmin, flags, g, ph);
+    *nblocks_out = (int)nb;
+}
+
+static void l_reduce_final(const double* partial, int nb, double* sums, hipStream_t s) {
+    k_reduce_final<<<1, RED_THREADS, 0, s>>>(partial, nb, sums);
+}
+
+static void l_time_accum(double* t, const unsigned long long* dtmin, double cfl, hipStream_t s) {
+    k_time_accum<<<1, 1, 0, s>>>(t, dtmin, cfl);
+}
+
+const Launchers& launchers() {
+    static const Launchers L = {l_reset_min,  l_step_begin, l_halo,         l_c2p_faces,
+                                l_face_flux,  l_deriv1,     l_deriv_halo,   l_residual,
+                                l_aos_to_soa, l_soa_to_aos, l_reduce,       l_reduce_final,
+                                l_time_accum};
+    return L;
+}
+
+}  // namespace NS_MODE
+}  // namespace nsgpu

@@ -1,0 +1,167 @@
+"""Benchmark of the RHS + SSP-RK3 path on one MI355X.
+
+Workload: the Taylor-Green vortex on a 256^3 periodic block over [0, 2pi]^3, characteristic-wise
+WENO-5 (Shu weights, Lax-Friedrichs split) + CD4 viscous terms, FP64, SSP-RK3 at CFL 0.5.
+A step is one full SSP-RK3 step (3 RHS evaluations + CFL reduction + stage updates) with the state
+resident in HBM.  Metric: Gcell-RK3-steps/s = nx*ny*nz*steps / wall time of the timed steps.
+
+Prints one JSON line (see the keys at the end of main()).
+"""
+import argparse
+import importlib
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# algorithmic work per unit (one cell advanced one SSP-RK3 step, characteristic WENO-5 + CD4,
+# Shu weights; add/sub/mul = 1, fma = 2, div = 1, sqrt = 1): 12,050 FP64 flop.  Per face of one
+# characteristic sweep: 1,190 flop; per face of one component-wise sweep: 725 flop.
+FLOP_PER_CELL_STEP = {"weno5cw": 12050.0, "weno5": 7860.0}
+FLOP_PER_FACE = {"weno5cw": 1190.0, "weno5": 725.0}
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_baseline_child(n, steps):
+    """Times the reference (oracle/_ref/libnsref_fast.so: the reference's own translation units,
+    its optimisation flags) or, when that was not built, the oracle, on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    C = importlib.import_module("3dnssolver_amd.cases")
+    if pyoracle.reference_available("fast"):
+        lib, kind = pyoracle.Reference("fast"), "reference"
+    else:
+        lib, kind = pyoracle.Oracle(), "port"
+    warm_case, warm_W = C.config2(32)
+    lib.step(warm_case, warm_W, 1)
+    case, W = C.config2(n)
+    t0 = time.perf_counter()
+    lib.step(case, W, steps)
+    el = time.perf_counter() - t0
+    print(json.dumps({"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-RK3-steps/s",
+                      "cores": lib.num_threads(), "kind": kind,
+                      "sample": f"{steps} SSP-RK3 step(s) of the same workload at {n}^3 "
+                                f"({el:.1f} s wall, OpenMP)"}))
+
+
+def run_cpu_baseline(n, steps, timeout):
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
+                            "--cpu-n", str(n), "--cpu-steps", str(steps)],
+                           capture_output=True, text=True, timeout=timeout)
+        lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+        if r.returncode == 0 and lines:
+            return json.loads(lines[-1])
+        sys.stderr.write(r.stderr[-2000:])
+    except subprocess.TimeoutExpired:
+        sys.stderr.write("cpu baseline timed out\n")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--math", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--conv", choices=["weno5cw", "weno5"], default="weno5cw")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-baseline-child", action="store_true")
+    ap.add_argument("--timing-steps", type=int, default=2)
+    args = ap.parse_args()
+
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_n, args.cpu_steps)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
+              f"see DESIGN.md for how the block would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    ns = importlib.import_module("3dnssolver_amd")
+    C = importlib.import_module("3dnssolver_amd.cases")
+    case, W0 = C.config2(args.n)
+    case["conv"] = args.conv
+    n = args.n
+    print(f"[bench] {n}^3 {args.conv} cd4 math={args.math}: setup", file=sys.stderr, flush=True)
+    s = ns.Solver(case, math=args.math)
+    s.set_state(W0, "primitive")
+    if args.warmup:
+        s.step(args.warmup)
+    print(f"[bench] warmup done, timing {args.steps} steps", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    _, dt = s.step(args.steps)          # returns after hipStreamSynchronize
+    el = time.perf_counter() - t0
+    sums, dtmin, flags, rc = s.reduce()
+    cells = n ** 3
+    value = cells * args.steps / el / 1e9
+    ms_per_step = el / args.steps * 1e3
+
+    # per-kernel durations with HIP events on the context's stream (separate pass)
+    s.timing(True)
+    s.step(args.timing_steps)
+    timers = s.timers()
+    s.timing(False)
+    s.close()
+    flux = {k: v for k, v in timers.items() if k.startswith("face_flux")}
+    dom = max(flux, key=lambda k: flux[k][0]) if flux else max(timers, key=lambda k: timers[k][0])
+    dom_ms, dom_n = timers[dom]
+    faces = (n + 1) * n * n
+    flop_launch = FLOP_PER_FACE[args.conv] * faces
+    achieved = flop_launch / (dom_ms / dom_n * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        tr = json.load(open(tpath))
+        traffic = tr.get(f"{dom}_{args.math}_{n}")
+    step_frac = value * 1e9 * FLOP_PER_CELL_STEP[args.conv] / (FP64_PEAK_TFLOPS * 1e12)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = run_cpu_baseline(args.cpu_n, args.cpu_steps, timeout=240)
+
+    out = {
+        "metric": "Gcell-RK3-steps/s at 256^3 + % binding roofline",
+        "value": round(value, 6),
+        "unit": "Gcell-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Taylor-Green vortex {n}^3 periodic, [0,2pi]^3, Mach 0.1, Re 1600, "
+                        f"{'characteristic' if args.conv == 'weno5cw' else 'component-wise'} "
+                        f"WENO-5 (Shu, LF split) + CD4 viscous, SSP-RK3, CFL 0.5",
+            "grid": [n, n, n], "math": args.math,
+            "state_resident_in_hbm": True,
+        },
+        "roofline": {
+            "bound": "mfma", "compute": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)",
+            "kernel": dom, "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
+            "flop_per_launch": flop_launch, "avg_launch_ms": round(dom_ms / dom_n, 5),
+            "step_frac": round(step_frac, 5),
+        },
+        "kernel_ms_per_step": {k: round(v[0] / args.timing_steps, 4) for k, v in timers.items()},
+        "health": {"status": rc, "flags": flags, "dt_last": dt},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+/*
+ * nsgpu.h -- C ABI of the MI355X-native right-hand-side + SSP-RK3 path of the compressible
+ * Navier-Stokes block solver.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ types or exceptions cross it.  One context
+ * binds one GPU and one HIP stream and owns all device memory of one block.  Host arrays are
+ * caller-owned and use the reference Array4d layout: row-major [NX][NY][NZ][5], q fastest,
+ * NX = nx + 2*3 (containers/array_4d.hpp:116-124, array_4d_dimensions.hpp:40-49); they are read
+ * or written synchronously before a call returns.  Variables: conservative U =
+ * [rho, rhoE, rho*u, rho*v, rho*w], primitive W = [rho, p, u, v, w] (eq_of_state.cpp:32-87).
+ *
+ * Reference interfaces replaced (paths relative to the reference's src/):
+ *   nsgpu_apply_boundary    Domain::apply_boundary_conditions + Communicator::communicate
+ *                           (domain.hpp:34, domain.cpp:18-28; communicator.hpp:21,
+ *                           communicator.cpp:6-144) on one block: Neumann faces, periodic wrap
+ *   nsgpu_convection_solve  ConvectionScheme::solve(grid, W, U, R) (convection_scheme.hpp:26;
+ *                           Weno5 weno-5.hpp:19-22, Weno5cw weno-5cw.hpp): adds into R
+ *   nsgpu_diffusion_solve   DiffusionScheme::solve(grid, comm, W, R) (diffusion_scheme.hpp:14-17;
+ *                           CenteredDiffusion centered_diffusion.hpp): adds into R
+ *   nsgpu_rhs               one stage body of TimeScheme::step (time_scheme.cpp:33-41)
+ *   nsgpu_step              TimeScheme::step (time_scheme.hpp:25-29, time_scheme.cpp:3-70),
+ *                           state resident on the GPU between calls
+ *   nsgpu_get_dt            TimeScheme::get_dt (time_scheme.hpp:31)
+ *   nsgpu_set_state /       Domain::get_NS_primitive_array / get_NS_conservative_array
+ *   nsgpu_get_state         (domain.hpp:36-40): upload / download of the block state
+ *   nsgpu_reduce            no reference counterpart: conservation sums, CFL minimum and health
+ *                           flags for callers that monitor a run
+ */
+#ifndef NSGPU_H
+#define NSGPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nsgpu_ctx nsgpu_ctx;
+
+/* status codes: 0 = OK, < 0 = error */
+enum {
+    NSGPU_OK              = 0,
+    NSGPU_ERR_INVALID     = -1, /* bad configuration or argument */
+    NSGPU_ERR_HIP         = -2, /* a HIP runtime call failed */
+    NSGPU_ERR_NODEVICE    = -3, /* no GPU / device index out of range */
+    NSGPU_ERR_NONFINITE   = -4, /* state holds NaN / Inf (nsgpu_reduce health check) */
+    NSGPU_ERR_POSITIVITY  = -5, /* rho <= 0 or p <= 0 */
+    NSGPU_ERR_ALPHA       = -6, /* max |u|+c exceeds the Lax-Friedrichs constant 3.0 */
+    NSGPU_ERR_UNSUPPORTED = -7  /* valid request the library does not implement */
+};
+
+/* boundary kind per face; faces in the reference order W,E,N,S,F,B (domain.cpp:18-28):
+ * West = i-low, East = i-high, North = j-low, South = j-high, Front = k-high, Back = k-low.
+ * An axis is periodic when its West / South / Front entry says so (decomposition.cpp:145-161);
+ * each face of a non-periodic axis must then be NEUMANN (a reference face with another kind has
+ * no boundary object, domain.cpp:87-137). */
+enum { NSGPU_BC_PERIODIC = 0, NSGPU_BC_NEUMANN = 1 };
+enum { NSGPU_CONV_NONE = 0, NSGPU_CONV_WENO5 = 1, NSGPU_CONV_WENO5CW = 2 };
+enum { NSGPU_WEIGHTS_SHU = 0, NSGPU_WEIGHTS_BORGES = 1 };
+/* arithmetic of the kernels:
+ *   EXACT  the reference's FP64 operation order, no contraction, IEEE division and sqrt:
+ *          results are bitwise those of a strict (-ffp-contract=off) build of the reference
+ *   FAST   FP64 with fused multiply-add, hoisted reciprocals and one division per WENO
+ *          reconstruction; agrees with EXACT to round-off (tolerances in the tests) */
+enum { NSGPU_MATH_EXACT = 0, NSGPU_MATH_FAST = 1 };
+enum { NSGPU_STATE_CONSERVATIVE = 0, NSGPU_STATE_PRIMITIVE = 1 };
+enum { NSGPU_RHS_CONVECTION = 1, NSGPU_RHS_DIFFUSION = 2, NSGPU_RHS_BOTH = 3 };
+
+typedef struct {
+    int    nx, ny, nz;  /* interior cells of the block (each >= 3) */
+    int    ngc;         /* halo layers: must be 3 (params.hpp:16-18) */
+    int    dim;         /* 3, or 2: the z sweep is skipped (weno-5.cpp:28) */
+    double Lx, Ly, Lz;  /* block lengths: dx = Lx / nx (grid.cpp:14-16) */
+    double gamma, mu, Pr, cfl;
+    int    conv;        /* NSGPU_CONV_* */
+    int    weights;     /* NSGPU_WEIGHTS_* */
+    int    diff_order;  /* 0: no diffusion; 2, 4, 6: CenteredDiffusion order */
+    int    bc[6];       /* NSGPU_BC_*, faces W,E,N,S,F,B */
+    int    math;        /* NSGPU_MATH_* */
+} nsgpu_config;
+
+/* Create a context on GPU `device` (HIP ordinal).  Allocates the block's device arrays. */
+int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out);
+void nsgpu_destroy(nsgpu_ctx* ctx);
+const char* nsgpu_strerror(int status);
+
+/* number of doubles of one padded host array: (nx+6)*(ny+6)*(nz+6)*5 */
+size_t nsgpu_array_len(const nsgpu_ctx* ctx);
+
+/* Upload / download the block state (whole padded array).  PRIMITIVE is the state TimeScheme::step
+ * advances (the reference keeps W between steps and re-derives U from it, time_scheme.cpp:25);
+ * CONSERVATIVE reads the U of the last step (interior). */
+int nsgpu_set_state(nsgpu_ctx* ctx, const double* aos, int kind);
+int nsgpu_get_state(nsgpu_ctx* ctx, double* aos, int kind);
+
+/* Boundary conditions + single-block halo exchange on a 5-variable array, in place. */
+int nsgpu_apply_boundary(nsgpu_ctx* ctx, double* aos);
+
+/* ConvectionScheme::solve: R += convective residual of (W, U), halos of W and U as given. */
+int nsgpu_convection_solve(nsgpu_ctx* ctx, const double* W_aos, const double* U_aos, double* R_aos);
+/* DiffusionScheme::solve: R += viscous residual of W (halos as given); the first-derivative
+ * arrays get the periodic exchange, or zero halos at physical walls. */
+int nsgpu_diffusion_solve(nsgpu_ctx* ctx, const double* W_aos, double* R_aos);
+
+/* Stage body: U (interior) -> boundary + exchange in place -> W = c2p(U) -> R = 0 + terms of
+ * `which`.  W_out may be NULL. */
+int nsgpu_rhs(nsgpu_ctx* ctx, double* U_aos, double* W_out, double* R_out, int which);
+
+/* nsteps x TimeScheme::step on the resident primitive state.  Adds each step's dt to *time_io
+ * (may be NULL) and stores the last dt in *dt_last (may be NULL).  Synchronises once at the end. */
+int nsgpu_step(nsgpu_ctx* ctx, int nsteps, double* time_io, double* dt_last);
+int nsgpu_get_dt(nsgpu_ctx* ctx, double* dt);
+
+/* Conservation sums over the interior of the resident conservative state (sum of U_q * dx*dy*dz),
+ * the CFL time-step minimum of the resident primitive state, and health flags
+ * (bit 0 non-finite, bit 1 rho<=0 or p<=0, bit 2 max|u|+c > 3). */
+int nsgpu_reduce(nsgpu_ctx* ctx, double sums[5], double* dt_min, unsigned* flags);
+
+/* The context's HIP stream (a hipStream_t), for callers that time or order work around it. */
+void* nsgpu_stream(nsgpu_ctx* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream: enable, then read the accumulated
+ * milliseconds and launch counts of the kernel families (names via nsgpu_timer_name). */
+int nsgpu_timing(nsgpu_ctx* ctx, int enable);
+int nsgpu_timer_count(void);
+const char* nsgpu_timer_name(int i);
+int nsgpu_timer_read(nsgpu_ctx* ctx, int i, double* ms, long* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,238 @@
+"""Parity of the HIP path (through the C ABI) with the oracle and the reference's golden fixtures.
+
+EXACT math must agree bitwise (interior values; signed zeros compare equal).  FAST math is held to
+1e-12 relative per RHS evaluation and 1e-10 after 100 steps
+(max |a-b| / max |b| per variable).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import GOLDEN_RHS, cases_module, load_golden, package
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_err(a, b):
+    """max |a-b| / max |b| per variable (last axis), as an array of 5."""
+    a = np.asarray(a).reshape(-1, 5)
+    b = np.asarray(b).reshape(-1, 5)
+    den = np.abs(b).max(axis=0)
+    den[den == 0] = 1.0
+    return np.abs(a - b).max(axis=0) / den
+
+
+@pytest.fixture(scope="module")
+def ns():
+    return package()
+
+
+@pytest.mark.parametrize("name", GOLDEN_RHS)
+def test_rhs_matches_golden_bitwise(ns, name):
+    case, g = load_golden(name)
+    print(f"[gpu] rhs golden {name}", flush=True)
+    s = ns.Solver(case, math="exact")
+    for which, key in ((1, "R_conv"), (2, "R_diff"), (3, "R")):
+        Uf, W, R = s.rhs(g["U_in"], which)
+        np.testing.assert_array_equal(po.interior(R), po.interior(g[key]))
+    m = po.faces_mask(case)
+    np.testing.assert_array_equal(Uf[m], g["U_filled"][m])
+    np.testing.assert_array_equal(W[m], g["W"][m])
+    s.close()
+
+
+@pytest.mark.parametrize("name", ["G2_weno5cw_cd4_periodic", "G5_walls", "G6_mixed"])
+def test_operator_level_solve_adds_into_R(ns, name):
+    case, g = load_golden(name)
+    s = ns.Solver(case, math="exact")
+    zero = np.zeros_like(g["R"])
+    Rc = s.convection_solve(g["W"], g["U_filled"], zero)
+    np.testing.assert_array_equal(po.interior(Rc), po.interior(g["R_conv"]))
+    Rd = s.diffusion_solve(g["W"], zero)
+    np.testing.assert_array_equal(po.interior(Rd), po.interior(g["R_diff"]))
+    # the time scheme's order: convection then diffusion accumulate into the same R
+    R = s.diffusion_solve(g["W"], s.convection_solve(g["W"], g["U_filled"], zero))
+    np.testing.assert_array_equal(po.interior(R), po.interior(g["R"]))
+    s.close()
+
+
+@pytest.mark.parametrize("bc", [["periodic"] * 6, ["neumann"] * 6,
+                                ["periodic", "periodic", "neumann", "neumann", "periodic", "periodic"]])
+def test_apply_boundary_matches_oracle(ns, oracle, bc):
+    C = cases_module()
+    case = dict(C.base_case(11, 7, 9), bc=bc)
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal(po.padded_shape(case))
+    s = ns.Solver(case)
+    B = s.apply_boundary(A)
+    Bo = oracle.fill_halos(case, A)
+    np.testing.assert_array_equal(B, Bo)  # edges / corners untouched on both sides
+    s.close()
+
+
+def test_steps_match_golden_bitwise(ns):
+    case, g = load_golden("G7_steps_weno5cw_tgv16")
+    print("[gpu] steps golden G7", flush=True)
+    s = ns.Solver(case, math="exact")
+    s.set_state(g["W_in"], "primitive")
+    t = 0.0
+    for k in range(g["W_steps"].shape[0]):
+        t, dt = s.step(1, t)
+        assert dt == g["dts"][k]
+        np.testing.assert_array_equal(po.interior(s.get_state("primitive")), g["W_steps"][k])
+    assert t == pytest.approx(g["dts"].sum(), rel=1e-15)
+    s.close()
+
+
+@pytest.mark.parametrize("conv,weights,order,bc,dim", [
+    ("weno5", "shu", 4, ["periodic"] * 6, 3),
+    ("weno5cw", "shu", 4, ["periodic"] * 6, 3),
+    ("weno5cw", "borges", 6, ["neumann"] * 6, 3),
+    ("weno5", "borges", 2, ["neumann", "neumann", "periodic", "periodic", "neumann", "neumann"], 3),
+    ("weno5cw", "shu", 4, ["periodic"] * 6, 2),
+    ("none", "shu", 4, ["periodic"] * 6, 3),
+    ("weno5cw", "shu", 0, ["periodic"] * 6, 3),
+])
+def test_multi_step_matches_oracle_bitwise(ns, oracle, conv, weights, order, bc, dim):
+    C = cases_module()
+    case = dict(C.base_case(19, 13, 11), conv=conv, weights=weights, diff_order=order, bc=bc,
+                dim=dim, L=(1.0, 0.8, 0.7))
+    W0 = C.random_smooth_W(case, 31)
+    print(f"[gpu] steps {conv} {weights} cd{order} {bc[0]}/{bc[2]}/{bc[4]} dim{dim}", flush=True)
+    Wo, Uo, dto = oracle.step(case, W0, 5)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    dts = []
+    for _ in range(5):
+        _, dt = s.step(1)
+        dts.append(dt)
+    np.testing.assert_array_equal(np.array(dts), dto)
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    np.testing.assert_array_equal(po.interior(s.get_state("conservative")), po.interior(Uo))
+    s.close()
+
+
+def test_config1_100_steps_32cubed_bitwise(ns, oracle):
+    # config-1 physics (shear layer, component-wise WENO-5, CD4) at 32^3 for 100 steps
+    C = cases_module()
+    case, W0 = C.config1(32)
+    print("[gpu] config1 32^3 x 100 steps", flush=True)
+    Wo, _, dto = oracle.step(case, W0, 100)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    t, dt = s.step(100)
+    assert dt == dto[-1]
+    assert t == pytest.approx(dto.sum(), rel=1e-14)
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    s.close()
+
+
+@pytest.mark.parametrize("name", GOLDEN_RHS)
+def test_fast_math_rhs_within_1e12(ns, name):
+    case, g = load_golden(name)
+    s = ns.Solver(case, math="fast")
+    _, _, R = s.rhs(g["U_in"], 3)
+    err = rel_err(po.interior(R), po.interior(g["R"]))
+    assert err.max() <= 1e-12, err
+    s.close()
+
+
+def test_fast_math_100_steps_within_1e10(ns, oracle):
+    C = cases_module()
+    case, W0 = C.config2(24)
+    print("[gpu] fast math TGV 24^3 x 100 steps", flush=True)
+    Wo, _, _ = oracle.step(case, W0, 100)
+    s = ns.Solver(case, math="fast")
+    s.set_state(W0, "primitive")
+    s.step(100)
+    err = rel_err(po.interior(s.get_state("primitive")), po.interior(Wo))
+    assert err.max() <= 1e-10, err
+    s.close()
+
+
+def test_reduce_sums_and_flags(ns):
+    C = cases_module()
+    case, W0 = C.config2(16)
+    s = ns.Solver(case)
+    s.set_state(W0, "primitive")
+    s.step(2)
+    sums, dt, flags, rc = s.reduce()
+    U = s.get_state("conservative")
+    vol = np.prod([case["L"][a] / case[n] for a, n in enumerate(("nx", "ny", "nz"))])
+    np.testing.assert_allclose(sums, po.interior(U).sum(axis=(0, 1, 2)) * vol, rtol=1e-13)
+    assert rc == 0 and flags == 0
+    W = s.get_state("primitive")
+    assert dt == pytest.approx(po.Oracle().dt(case, W), rel=1e-15)
+    # mass and momentum are conserved to round-off on a periodic block
+    s2, _, _, _ = s.reduce()
+    s.step(3)
+    s3, _, _, _ = s.reduce()
+    assert abs(s3[0] - s2[0]) < 1e-13 * abs(s2[0])
+    s.close()
+
+
+def test_reduce_flags_forced_from_host(ns):
+    # failure detection: a negative density and a NaN planted in the uploaded state
+    C = cases_module()
+    case, W0 = C.config2(16)
+    s = ns.Solver(case)
+    W = W0.copy()
+    W[5, 5, 5, 0] = -1.0
+    s.set_state(W, "primitive")
+    _, _, flags, rc = s.reduce()
+    assert flags & 2 and rc == -5
+    W[6, 6, 6, 2] = np.nan
+    s.set_state(W, "primitive")
+    _, _, flags, rc = s.reduce()
+    assert flags & 1 and rc == -4
+    s.close()
+
+
+def _length_with_spacing(h, n):
+    """A length L with L / n == h exactly (IEEE double division, as on the device)."""
+    L = h * n
+    for _ in range(64):
+        q = L / n
+        if q == h:
+            return L
+        L = np.nextafter(L, np.inf if q < h else -np.inf)
+    raise AssertionError("no length reproduces the spacing")
+
+
+def _window_check(ns, oracle, case, W, math, tol, nwin=3, w=6, m=4, seed=0):
+    """RHS at full size on the GPU vs the oracle on periodic crops around a few windows: a cell
+    at distance >= m from the crop edge sees exactly its neighbourhood in the full block."""
+    s = ns.Solver(case, math=math)
+    Ufull = oracle.p2c(case, W)
+    _, _, R = s.rhs(Ufull, 3)
+    s.close()
+    rng = np.random.default_rng(seed)
+    n = case["nx"]
+    worst = 0.0
+    for _ in range(nwin):
+        i0, j0, k0 = rng.integers(0, n, 3)
+        idx = [(np.arange(-m, w + m) + o) % n + 3 for o in (i0, j0, k0)]
+        nsub = w + 2 * m
+        sub = dict(case, nx=nsub, ny=nsub, nz=nsub,
+                   L=tuple(_length_with_spacing(case["L"][a] / case[nn], nsub)
+                           for a, nn in enumerate(("nx", "ny", "nz"))))
+        Us = np.zeros(po.padded_shape(sub))
+        Us[3:-3, 3:-3, 3:-3] = Ufull[np.ix_(idx[0], idx[1], idx[2])]
+        _, _, Rs = oracle.rhs(sub, Us, 3)
+        got = R[np.ix_(idx[0][m:m + w], idx[1][m:m + w], idx[2][m:m + w])]
+        want = Rs[3 + m:3 + m + w, 3 + m:3 + m + w, 3 + m:3 + m + w]
+        if tol == 0:
+            np.testing.assert_array_equal(got, want)
+        else:
+            worst = max(worst, np.abs(got - want).max() / np.abs(po.interior(R)).max())
+    return worst
+
+
+def test_full_size_256_rhs_windows(ns, oracle):
+    # config 2 at its real size (256^3, characteristic WENO-5 + CD4): exact kernels agree bitwise
+    # with the oracle on cropped windows; the grid spacing does not enter the crop except via dx,
+    # which the crop reproduces exactly (L_sub / n_sub == L / n)
+    C = cases_module()
+    case, W = C.config2(256)
+    print("[gpu] 256^3 window parity (exact)", flush=True)
+    _window_check(ns, oracle, case, W, "exact", 0)
