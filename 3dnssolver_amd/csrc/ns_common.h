@@ -20,39 +20,46 @@ struct Geo {
     int    NX, NY, NZ;  // padded extents
     int    sx, sy;      // strides of i and j (k stride is 1)
     long   P;           // plane stride
-    double dx, dy, dz;
+    double dx, dy, dz;  // Grid::dx() etc. (grid.cpp:14-16)
+    double dxx, dyy;    // dx*dx, dy*dy (CDN second-derivative divisors, cd-n.cpp:67-79)
+    double idx, idy, idz, idxx, idyy;  // reciprocals (FAST math only)
     int    dim;
+    int    square_xy;   // dx*dx == dy*dy bitwise: the two x-stencil second derivatives coincide
 };
 
-// per-face boundary action (resolved on the host from nsgpu_config::bc):
-// 0 = periodic wrap, 1 = Neumann copy
+// per-face boundary action (resolved on the host from nsgpu_config::bc)
 struct Bnd {
     int periodic[3];
-    int face_kind[6];  // W,E,N,S,F,B
+    int face_kind[6];  // W,E,N,S,F,B: 0 periodic, 1 Neumann
 };
 
 struct Phys {
     double gamma, mu, Pr, cfl;
+    double lamb;        // (-2./3.) * mu            (centered_diffusion.cpp:101)
+    double two_mu;      // 2.0 * mu                 (diffusion_scheme.cpp:5, :27)
+    double heat;        // gamma / (Pr * (gamma - 1.0))  (centered_diffusion.cpp:161)
     int    conv;        // 0 none, 1 component-wise, 2 characteristic
     int    borges;      // 0 Shu, 1 Borges
     int    diff_order;  // 0, 2, 4, 6
 };
 
-// SSP-RK3 stage coefficients are formed on the device from dt (time_scheme.cpp:37-63)
+// SSP-RK3 stage (time_scheme.cpp:37-63); the coefficients are formed on the device from dt
 struct StageArgs {
     const double* Un;     // step-start state (interior)
     const double* Ucur;   // stage input state (halo-filled)
     double*       Uout;   // stage output (interior)
-    double*       Wout;   // c2p(Uout) on the last stage (interior), else nullptr
+    double*       Wout;   // c2p(Uout) (interior)
     int           stage;  // 0, 1, 2
 };
 
 // kernel launchers of one math mode (EXACT or FAST); all enqueue on `s`
 struct Launchers {
     void (*reset_min)(unsigned long long* bits, hipStream_t s);
-    void (*step_begin)(const double* W, double* Ua, double* Un, unsigned long long* dtmin,
-                       Geo g, Phys ph, hipStream_t s);
-    void (*halo)(double* A, int nvar, Geo g, Bnd b, hipStream_t s);
+    // dt from W, U = p2c(W) -> Ua and Un, Wst = c2p(U) (interior)
+    void (*step_begin)(const double* W, double* Ua, double* Un, double* Wst,
+                       unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s);
+    // boundary conditions + periodic exchange on nA planes of A and nB planes of B (B may be null)
+    void (*halo)(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s);
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
                       hipStream_t s);
@@ -71,6 +78,7 @@ struct Launchers {
                    int* nblocks_out);
     void (*reduce_final)(const double* partial, int nblocks, double* sums, hipStream_t s);
     void (*time_accum)(double* t, const unsigned long long* dtmin, double cfl, hipStream_t s);
+    void (*zero_flags)(unsigned* flags, hipStream_t s);
 };
 
 namespace exact {

@@ -8,7 +8,8 @@
 //       eigenvector bases are skipped: x + (+-0) == x and 1*x == x exactly, so only the sign of
 //       an exact zero can differ.
 //   NS_FAST=1 with -ffp-contract=fast  -> namespace nsgpu::fast
-//       fused multiply-add, reciprocal hoisting, one division per WENO reconstruction.
+//       fused multiply-add, reciprocals hoisted out of the stencils, no square root in the flux
+//       split, one division per WENO reconstruction.
 //
 // Citations are paths relative to the reference's src/ directory.
 #include "ns_common.h"
@@ -47,7 +48,6 @@ __constant__ double c_fd2[3][7] = {
 // ------------------------------------------------------------------------------------------
 // WENO-5 (weno-5.cpp:222-322)
 // ------------------------------------------------------------------------------------------
-#if !NS_FAST
 template <bool BORGES>
 __device__ __forceinline__ double weno_combine(double a, double b, double c, double d, double e,
                                                double d0, double d1, double d2, double f0,
@@ -62,6 +62,7 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
     t1        = c - 2. * d + e;
     t2        = 3. * c - 4. * d + e;
     double b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+#if !NS_FAST
     double a0, a1, a2;
     if constexpr (BORGES) {
         double tau5 = fabs(b0 - b2);
@@ -75,25 +76,10 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
     }
     double al = a0 + a1 + a2;
     return (a0 / al) * f0 + (a1 / al) * f1 + (a2 / al) * f2;
-}
 #else
-// One division per reconstruction: with s_k = (eps + b_k)^2 the Shu weights are
-// w_k = d_k prod_{j!=k} s_j / sum_m d_m prod_{j!=m} s_j; Borges: a_k = d_k (1 + tau/(b_k+eps))
-// = d_k (b_k + eps + tau) / (b_k + eps), same clearing of denominators.
-template <bool BORGES>
-__device__ __forceinline__ double weno_combine(double a, double b, double c, double d, double e,
-                                               double d0, double d1, double d2, double f0,
-                                               double f1, double f2) {
-    double t1, t2;
-    t1        = a - 2. * b + c;
-    t2        = a - 4. * b + 3. * c;
-    double b0 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
-    t1        = b - 2. * c + d;
-    t2        = b - d;
-    double b1 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
-    t1        = c - 2. * d + e;
-    t2        = 3. * c - 4. * d + e;
-    double b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    // one division: with s_k = (eps + b_k)^2 the Shu weights are
+    // w_k = d_k prod_{j!=k} s_j / sum_m d_m prod_{j!=m} s_j; Borges a_k = d_k (1 + tau/e_k)
+    // = d_k (e_k + tau) / e_k with e_k = b_k + eps, same clearing of denominators.
     double p0, p1, p2;
     if constexpr (BORGES) {
         double tau5 = fabs(b0 - b2);
@@ -110,8 +96,8 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
         p2        = d2 * (s0 * s1);
     }
     return (p0 * f0 + p1 * f1 + p2 * f2) / (p0 + p1 + p2);
-}
 #endif
+}
 
 // Weno5::WENO5_left (weno-5.cpp:305-322), ideal weights d_left = {0.1, 0.6, 0.3}
 template <bool BORGES>
@@ -132,30 +118,32 @@ __device__ __forceinline__ double weno_right(double a, double b, double c, doubl
 }
 
 // ------------------------------------------------------------------------------------------
-// Euler flux and Lax-Friedrichs split (convection_flux.cpp:23-61, weno-5.cpp:94-110)
+// Euler flux and Lax-Friedrichs split (convection_flux.cpp:23-61, weno-5.cpp:94-110):
+// s[0..4] = fL = (F + alpha U)/2, s[5..9] = fR = (F - alpha U)/2 of one cell, direction DIR
 // ------------------------------------------------------------------------------------------
-
-// Reads the 5 primitive and 5 conservative values of one cell and forms
-// fL = (F + alpha U) / 2, fR = (F - alpha U) / 2 for the direction DIR.
 template <int DIR>
 __device__ __forceinline__ void split_cell(const double* __restrict__ U, const double* __restrict__ W,
-                                           long c, long P, double gamma, double fl[5],
-                                           double fr[5]) {
+                                           long c, long P, double gamma, double s[10]) {
     double rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
     double qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
-    double cs  = sqrt(gamma * p / rho);
-    double H   = cs * cs / (gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+#if !NS_FAST
+    double cs = sqrt(gamma * p / rho);
+    double H  = cs * cs / (gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+#else
+    double H = (gamma / (gamma - 1.0)) * (p / rho) + 0.5 * (u * u + v * v + w * w);
+#endif
     double F[5];
-    F[0] = rho * qn;
-    F[1] = rho * qn * H;
-    F[2] = (DIR == 0) ? rho * qn * u + p : rho * qn * u;
-    F[3] = (DIR == 1) ? rho * qn * v + p : rho * qn * v;
-    F[4] = (DIR == 2) ? rho * qn * w + p : rho * qn * w;
+    double m = rho * qn;
+    F[0]     = m;
+    F[1]     = m * H;
+    F[2]     = (DIR == 0) ? m * u + p : m * u;
+    F[3]     = (DIR == 1) ? m * v + p : m * v;
+    F[4]     = (DIR == 2) ? m * w + p : m * w;
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         double uq = U[c + q * P];
-        fl[q]     = (F[q] + LF_ALPHA * uq) * 0.5;
-        fr[q]     = (F[q] - LF_ALPHA * uq) * 0.5;
+        s[q]      = (F[q] + LF_ALPHA * uq) * 0.5;
+        s[5 + q]  = (F[q] - LF_ALPHA * uq) * 0.5;
     }
 }
 
@@ -187,30 +175,46 @@ template <int K> __device__ __forceinline__ double cadd(double acc, double x) {
 // convection_flux.cpp:117-175, c^2 = gamma / (p rho) as written there)
 struct Eig {
     double vel[3];
-    double c, H, qn, ql, qm, q2h, K_pc;
+    double c, H, qn, ql, qm, q2h;
     double L0[5], L2[5], L3[5];  // rows 0, 2, 3 of L (all entries depend on the state)
 };
 
 template <int DIR>
 __device__ __forceinline__ void eigen(double rho, double p, double u, double v, double w,
                                       double gamma, Eig& E) {
-    E.vel[0]    = u;
-    E.vel[1]    = v;
-    E.vel[2]    = w;
+    E.vel[0]  = u;
+    E.vel[1]  = v;
+    E.vel[2]  = w;
+    double qn = E.vel[DIR];
+    // ql = u*lx + v*ly + w*lz, qm = u*mx + v*my + w*mz with one nonzero entry each
+    double ql = (DIR == 2) ? -v : w;
+    double qm = (DIR == 0) ? -v : u;
+    double q2 = u * u + v * v + w * w;
+    double K  = gamma - 1.0;
+#if !NS_FAST
     double c2   = gamma / (p * rho);
     double c    = sqrt(c2);
-    double qn   = E.vel[DIR];
-    // ql = u*lx + v*ly + w*lz, qm = u*mx + v*my + w*mz with one nonzero entry each
-    double ql   = (DIR == 2) ? -v : w;
-    double qm   = (DIR == 0) ? -v : u;
-    double q2   = u * u + v * v + w * w;
     double H    = c2 / (gamma - 1.0) + 0.5 * q2;
-    double K    = gamma - 1.0;
     double K_pc = K / (2.0 * c2);
-    double t    = 0.5 * K_pc * q2;
     double qn2c = qn / (2.0 * c);
     double i2c  = 1.0 / (2.0 * c);  // n_a / (2c) for the normal component
-    E.c = c, E.H = H, E.qn = qn, E.ql = ql, E.qm = qm, E.q2h = 0.5 * q2, E.K_pc = K_pc;
+#pragma unroll
+    for (int a = 0; a < 3; a++) E.L2[1 + a] = K * E.vel[a] / c2;
+    E.L2[4] = -K / c2;
+#else
+    double ic2  = p * rho / gamma;  // 1 / c^2
+    double c2   = gamma / (p * rho);
+    double c    = sqrt(c2);
+    double H    = c2 / K + 0.5 * q2;
+    double K_pc = 0.5 * K * ic2;
+    double i2c  = 0.5 * c * ic2;    // 1 / (2c)
+    double qn2c = qn * i2c;
+#pragma unroll
+    for (int a = 0; a < 3; a++) E.L2[1 + a] = K * E.vel[a] * ic2;
+    E.L2[4] = -K * ic2;
+#endif
+    double t = 0.5 * K_pc * q2;
+    E.c = c, E.H = H, E.qn = qn, E.ql = ql, E.qm = qm, E.q2h = 0.5 * q2;
     E.L0[0] = t + qn2c;
     E.L3[0] = t - qn2c;
 #pragma unroll
@@ -223,15 +227,13 @@ __device__ __forceinline__ void eigen(double rho, double p, double u, double v, 
             E.L0[1 + a] = -(kv);
             E.L3[1 + a] = -(kv);
         }
-        E.L2[1 + a] = K * E.vel[a] / c2;
     }
     E.L0[4] = K_pc;
     E.L3[4] = K_pc;
     E.L2[0] = 1.0 - K_pc * q2;
-    E.L2[4] = -K / c2;
 }
 
-// row q of L times the split stencil values s[p] (p in conservative order), summed in p order
+// row Q of L times the split stencil values s[p] (p in conservative order), summed in p order
 // from 0.0 (matrix::square_matrix_matrix_product, matrix.cpp:12-20)
 template <int DIR, int Q>
 __device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
@@ -260,10 +262,10 @@ __device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
 template <int DIR, int A>
 __device__ __forceinline__ double Rvel(const Eig& E, const double ft[5]) {
     // row 1+A: {vel_A - c n_A, m_A, vel_A, vel_A + c n_A, l_A}
-    constexpr int n = Basis<DIR, A>::n;
-    double        va = E.vel[A];
-    double        e0 = n ? va - E.c : va;
-    double        e3 = n ? va + E.c : va;
+    constexpr int n   = Basis<DIR, A>::n;
+    double        va  = E.vel[A];
+    double        e0  = n ? va - E.c : va;
+    double        e3  = n ? va + E.c : va;
     double        acc = e0 * ft[0];
     acc               = cadd<Basis<DIR, A>::m>(acc, ft[1]);
     acc               = acc + va * ft[2];
@@ -274,76 +276,200 @@ __device__ __forceinline__ double Rvel(const Eig& E, const double ft[5]) {
 
 template <int DIR>
 __device__ __forceinline__ void Rmul(const Eig& E, const double ft[5], double out[5]) {
-    out[0] = ft[0] + ft[2] + ft[3];  // {1, 0, 1, 1, 0}
-    out[1] = Rvel<DIR, 0>(E, ft);
-    out[2] = Rvel<DIR, 1>(E, ft);
-    out[3] = Rvel<DIR, 2>(E, ft);
-    double qc = E.qn * E.c;          // {H - qn c, qm, q2/2, H + qn c, ql}
+    out[0]    = ft[0] + ft[2] + ft[3];  // {1, 0, 1, 1, 0}
+    out[1]    = Rvel<DIR, 0>(E, ft);
+    out[2]    = Rvel<DIR, 1>(E, ft);
+    out[3]    = Rvel<DIR, 2>(E, ft);
+    double qc = E.qn * E.c;             // {H - qn c, qm, q2/2, H + qn c, ql}
     out[4] = (E.H - qc) * ft[0] + E.qm * ft[1] + E.q2h * ft[2] + (E.H + qc) * ft[3] + E.ql * ft[4];
 }
 
-template <int DIR, bool BORGES, int Q>
-__device__ __forceinline__ double char_component(const Eig& E, const double fl[6][5],
-                                                 const double fr[6][5]) {
+// ------------------------------------------------------------------------------------------
+// one face flux from the split values of its six stencil cells.  S(ii, v) returns split value v
+// (0..4 fL, 5..9 fR) of cell i-2+ii, ii = 0..5; Wl, Wr are the primitives of cells i and i+1.
+// (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
+// ------------------------------------------------------------------------------------------
+template <int DIR, bool BORGES, int Q, class S>
+__device__ __forceinline__ double char_component(const Eig& E, const S& st) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
     double a[5], b[5];
 #pragma unroll
     for (int ii = 0; ii < 5; ii++) {
-        a[ii] = Lrow<DIR, Q>(E, fl[ii]);
-        b[ii] = Lrow<DIR, Q>(E, fr[ii + 1]);
+        double sl[5], sr[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+            sl[p] = st(ii, p);
+            sr[p] = st(ii + 1, 5 + p);
+        }
+        a[ii] = Lrow<DIR, Q>(E, sl);
+        b[ii] = Lrow<DIR, Q>(E, sr);
     }
     return weno_left<BORGES>(a[0], a[1], a[2], a[3], a[4]) +
            weno_right<BORGES>(b[0], b[1], b[2], b[3], b[4]);
 }
 
-// ------------------------------------------------------------------------------------------
-// face-flux kernel: one thread per face i+1/2 (stored at cell i) of the sweep direction
-// (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
-// ------------------------------------------------------------------------------------------
-template <int DIR, int CONV, bool BORGES>
-__global__ __launch_bounds__(256) void k_face_flux(const double* __restrict__ U,
-                                                   const double* __restrict__ W,
-                                                   double* __restrict__ f, Geo g, double gamma) {
-    int k = threadIdx.x + blockIdx.x * blockDim.x;
-    int j = threadIdx.y + blockIdx.y * blockDim.y;
-    int i = blockIdx.z;
-    // face ranges: the sweep index runs from begin-1 to end-1, the others over the interior
-    int nk = g.nz + (DIR == 2), nj = g.ny + (DIR == 1);
-    if (k >= nk || j >= nj) return;
-    i += NGC - (DIR == 0);
-    j += NGC - (DIR == 1);
-    k += NGC - (DIR == 2);
-    const long P = g.P;
-    const long c = (long)i * g.sx + (long)j * g.sy + k;
-    const long s = (DIR == 0) ? g.sx : (DIR == 1) ? g.sy : 1;
-
-    // split fluxes of cells i-2 .. i+3
-    double fl[6][5], fr[6][5];
-#pragma unroll
-    for (int ii = 0; ii < 6; ii++) split_cell<DIR>(U, W, c + (ii - 2) * s, P, gamma, fl[ii], fr[ii]);
-
-    double out[5];
+template <int DIR, int CONV, bool BORGES, class S>
+__device__ __forceinline__ void face_flux(const S& st, const double Wl[5], const double Wr[5],
+                                          double gamma, double out[5]) {
     if constexpr (CONV == 1) {
 #pragma unroll
         for (int q = 0; q < 5; q++)
-            out[q] = weno_left<BORGES>(fl[0][q], fl[1][q], fl[2][q], fl[3][q], fl[4][q]) +
-                     weno_right<BORGES>(fr[1][q], fr[2][q], fr[3][q], fr[4][q], fr[5][q]);
+            out[q] = weno_left<BORGES>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q)) +
+                     weno_right<BORGES>(st(1, 5 + q), st(2, 5 + q), st(3, 5 + q), st(4, 5 + q),
+                                        st(5, 5 + q));
     } else {
         double Wa[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) Wa[q] = 0.5 * (W[c + q * P] + W[c + s + q * P]);
+        for (int q = 0; q < 5; q++) Wa[q] = 0.5 * (Wl[q] + Wr[q]);
         Eig E;
         eigen<DIR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
         double ft[5];
-        ft[0] = char_component<DIR, BORGES, 0>(E, fl, fr);
-        ft[1] = char_component<DIR, BORGES, 1>(E, fl, fr);
-        ft[2] = char_component<DIR, BORGES, 2>(E, fl, fr);
-        ft[3] = char_component<DIR, BORGES, 3>(E, fl, fr);
-        ft[4] = char_component<DIR, BORGES, 4>(E, fl, fr);
+        ft[0] = char_component<DIR, BORGES, 0>(E, st);
+        ft[1] = char_component<DIR, BORGES, 1>(E, st);
+        ft[2] = char_component<DIR, BORGES, 2>(E, st);
+        ft[3] = char_component<DIR, BORGES, 3>(E, st);
+        ft[4] = char_component<DIR, BORGES, 4>(E, st);
         Rmul<DIR>(E, ft, out);
     }
+}
+
+struct RegSplit {  // split values held in registers: s[ii][v]
+    const double (*s)[10];
+    __device__ double operator()(int ii, int v) const { return s[ii][v]; }
+};
+
+template <int LANES>
+struct LdsSplit {  // split values in LDS, layout [cell][v][lane]
+    const double* base;  // &tile[first cell][0][lane]
+    __device__ double operator()(int ii, int v) const { return base[(ii * 10 + v) * LANES]; }
+};
+
+// ------------------------------------------------------------------------------------------
+// face-flux kernels.  Face i+1/2 of the sweep direction is stored at cell i.  The sweep index
+// of the faces runs from NGC-1 to NGC+n-1; the tile kernels cover the first n faces, the plane
+// kernel the last one (or, for operator-level use, any plane range).
+// ------------------------------------------------------------------------------------------
+constexpr int TILE_S = 8;    // x/y tiles: faces along the sweep per block (one wave each)
+constexpr int TILE_L = 64;   // lanes along k
+
+// x or y sweep: lanes along k (unit stride), 8 waves at 8 consecutive sweep positions.  The split
+// values of the 13 cells the tile needs are computed once into LDS.
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(512) void k_flux_tile_xy(const double* __restrict__ U,
+                                                      const double* __restrict__ W,
+                                                      double* __restrict__ f, Geo g,
+                                                      double gamma) {
+    constexpr int NC = TILE_S + 5;
+    __shared__ double tile[NC * 10 * TILE_L];
+    const int lane = threadIdx.x, w = threadIdx.y;
+    const int k    = NGC + blockIdx.x * TILE_L + lane;
+    const int o    = NGC + blockIdx.y;              // j for the x sweep, i for the y sweep
+    const int s0   = blockIdx.z * TILE_S;           // first face of the tile (0-based)
+    const int n    = (DIR == 0) ? g.nx : g.ny;
+    const long ss  = (DIR == 0) ? g.sx : g.sy;      // stride along the sweep
+    const long so  = (DIR == 0) ? g.sy : g.sx;      // stride of the other index
+    const bool kin = (k < NGC + g.nz);
+    // cells s0 .. s0+12 (padded sweep index) feed faces s0+2 .. s0+9
+    for (int t = w; t < NC; t += TILE_S) {
+        int cs = s0 + t;
+        if (kin && cs <= n + 2 * NGC - 1) {
+            double sv[10];
+            split_cell<DIR>(U, W, (long)cs * ss + (long)o * so + k, g.P, gamma, sv);
 #pragma unroll
-    for (int q = 0; q < 5; q++) f[c + q * P] = out[q];
+            for (int v = 0; v < 10; v++) tile[(t * 10 + v) * TILE_L + lane] = sv[v];
+        }
+    }
+    __syncthreads();
+    const int fidx = s0 + w;  // face index 0..n-1
+    if (!kin || fidx >= n) return;
+    const int  i = fidx + NGC - 1;  // face i+1/2
+    const long c = (long)i * ss + (long)o * so + k;
+    double     Wl[5], Wr[5];
+    if constexpr (CONV == 2) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + ss + q * g.P];
+    }
+    LdsSplit<TILE_L> st{&tile[(w * 10) * TILE_L + lane]};
+    double           out[5];
+    face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
+#pragma unroll
+    for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
+}
+
+// z sweep: lanes along k = the sweep (64 faces per row segment), 8 rows (j) per block; each row
+// stages the split values of its 69 cells in LDS.
+template <int CONV, bool BORGES>
+__global__ __launch_bounds__(512) void k_flux_tile_z(const double* __restrict__ U,
+                                                     const double* __restrict__ W,
+                                                     double* __restrict__ f, Geo g,
+                                                     double gamma) {
+    constexpr int NC = TILE_L + 5;
+    __shared__ double tile[8][NC * 10];
+    const int lane = threadIdx.x, r = threadIdx.y;
+    const int j    = NGC + blockIdx.y * 8 + r;
+    const int i    = NGC + blockIdx.z;
+    const int f0   = blockIdx.x * TILE_L;  // first face of the segment (0-based)
+    const bool rowok = (j < NGC + g.ny);   // every wave reaches the barrier below
+    const long row = (long)i * g.sx + (long)j * g.sy;
+    double*    T   = tile[r];
+    for (int t = lane; t < NC; t += TILE_L) {
+        int cs = f0 + t;  // padded k of the cell
+        if (rowok && cs <= g.nz + 2 * NGC - 1) {
+            double sv[10];
+            split_cell<2>(U, W, row + cs, g.P, gamma, sv);
+#pragma unroll
+            for (int v = 0; v < 10; v++) T[v * NC + t] = sv[v];
+        }
+    }
+    __syncthreads();  // rows are independent; each wave only reads its own row
+    const int fidx = f0 + lane;
+    if (!rowok || fidx >= g.nz) return;
+    const int  kf = fidx + NGC - 1;
+    const long c  = row + kf;
+    double     Wl[5], Wr[5];
+    if constexpr (CONV == 2) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + 1 + q * g.P];
+    }
+    struct {
+        const double* b;
+        __device__ double operator()(int ii, int v) const { return b[v * NC + ii]; }
+    } st{&T[lane]};
+    double out[5];
+    face_flux<2, CONV, BORGES>(st, Wl, Wr, gamma, out);
+#pragma unroll
+    for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
+}
+
+// one plane of faces (sweep index `i`), one thread per face, split values in registers
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
+                                                    const double* __restrict__ W,
+                                                    double* __restrict__ f, Geo g, double gamma,
+                                                    int i) {
+    // tangential indices (t1 slow, t2 fast) over the interior
+    int t2 = threadIdx.x + blockIdx.x * blockDim.x;
+    int t1 = threadIdx.y + blockIdx.y * blockDim.y;
+    int n1 = (DIR == 0) ? g.ny : g.nx;
+    int n2 = (DIR == 2) ? g.ny : g.nz;
+    if (t1 >= n1 || t2 >= n2) return;
+    long c, s;
+    if (DIR == 0) c = (long)i * g.sx + (long)(t1 + NGC) * g.sy + (t2 + NGC), s = g.sx;
+    else if (DIR == 1) c = (long)(t1 + NGC) * g.sx + (long)i * g.sy + (t2 + NGC), s = g.sy;
+    else c = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy + i, s = 1;
+    double sv[6][10];
+#pragma unroll
+    for (int ii = 0; ii < 6; ii++) split_cell<DIR>(U, W, c + (ii - 2) * s, g.P, gamma, sv[ii]);
+    double Wl[5], Wr[5];
+    if constexpr (CONV == 2) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + s + q * g.P];
+    }
+    RegSplit st{sv};
+    double   out[5];
+    face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
+#pragma unroll
+    for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -389,8 +515,9 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* o
         double t = __shfl_xor(v, o, 64);
         v        = (t < v) ? t : v;
     }
-    int lane = (threadIdx.x + threadIdx.y * blockDim.x) & 63;
-    int wid  = (threadIdx.x + threadIdx.y * blockDim.x) >> 6;
+    int tid  = threadIdx.x + threadIdx.y * blockDim.x;
+    int lane = tid & 63;
+    int wid  = tid >> 6;
     int nw   = (blockDim.x * blockDim.y + 63) >> 6;
     if (lane == 0) red[wid] = v;
     __syncthreads();
@@ -408,27 +535,36 @@ __global__ void k_reset_min(unsigned long long* bits) {
     *bits = (unsigned long long)__double_as_longlong(1E10);  // max_dt = 1E10 (time_scheme.cpp:104)
 }
 
-// step prologue: dt from W (time_scheme.cpp:16, :98-136), U = p2c(W) (:25), Uprev = U (:27)
+__global__ void k_zero_flags(unsigned* flags) { *flags = 0u; }
+
+// step prologue: dt from W (time_scheme.cpp:16, :98-136), U = p2c(W) (:25), Uprev = U (:27),
+// and the stage-1 primitives c2p(U) (:36).  Grid-stride over the interior.
 __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W,
                                                     double* __restrict__ Ua,
                                                     double* __restrict__ Un,
+                                                    double* __restrict__ Wst,
                                                     unsigned long long* dtmin, Geo g, double gamma) {
-    int  k     = threadIdx.x + blockIdx.x * blockDim.x;
-    int  j     = threadIdx.y + blockIdx.y * blockDim.y;
-    int  i     = blockIdx.z;
-    bool valid = (k < g.nz && j < g.ny);
-    double dt  = 1E300;
-    if (valid) {
+    long   n  = (long)g.nx * g.ny * g.nz;
+    double dt = 1E300;
+    for (long t = threadIdx.x + (long)blockIdx.x * blockDim.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        int    k = (int)(t % g.nz);
+        long   r = t / g.nz;
+        int    j = (int)(r % g.ny);
+        int    i = (int)(r / g.ny);
         long   c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
-        double w[5], u[5];
+        double w[5], u[5], w2[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) w[q] = W[c + q * g.P];
-        dt = local_dt(w, gamma, g.dx, g.dy, g.dz);
+        double d = local_dt(w, gamma, g.dx, g.dy, g.dz);
+        dt       = (d < dt) ? d : dt;
         p2c(w, gamma, u);
+        c2p(u, gamma, w2);
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            Ua[c + q * g.P] = u[q];
-            Un[c + q * g.P] = u[q];
+            Ua[c + q * g.P]  = u[q];
+            Un[c + q * g.P]  = u[q];
+            Wst[c + q * g.P] = w2[q];
         }
     }
     block_min_atomic(dt, dtmin);
@@ -437,9 +573,10 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
 // ------------------------------------------------------------------------------------------
 // halo fill: Neumann faces (boundary_neumann.cpp:22-132) and the periodic face exchange of one
 // rank (communicator.cpp:16-144 with the slicers of array_4d.cpp:39-164).  The six face regions
-// are disjoint and read interior cells only, so one launch covers them in any order.
+// are disjoint and read interior cells only, so one launch covers them in any order.  The same
+// copy applied to the primitive planes gives c2p(U) on the halo bit for bit (c2p is per cell).
 // ------------------------------------------------------------------------------------------
-__global__ void k_halo(double* A, int nvar, Geo g, Bnd b) {
+__global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
     int t2   = threadIdx.x + blockIdx.x * blockDim.x;  // fastest tangential index
     int t1   = threadIdx.y + blockIdx.y * blockDim.y;
     int face = blockIdx.z / NGC;
@@ -471,7 +608,8 @@ __global__ void k_halo(double* A, int nvar, Geo g, Bnd b) {
         cd = ct + dst;
         cs = ct + src;
     }
-    for (int q = 0; q < nvar; q++) A[cd + q * g.P] = A[cs + q * g.P];
+    for (int q = 0; q < nA; q++) A[cd + q * g.P] = A[cs + q * g.P];
+    for (int q = 0; q < nB; q++) B[cd + q * g.P] = B[cs + q * g.P];
 }
 
 // W = c2p(U) on the interior and the face halos (edges and corners are never read)
@@ -497,8 +635,38 @@ __global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
 // viscous terms (centered_diffusion.cpp:3-182, cd-n.cpp:91-165, diffusion_scheme.cpp:3-58)
 // ------------------------------------------------------------------------------------------
 
-// first derivatives of rho, p, u, v along x and y on the interior:
-// sum over the stencil of f(. + s + k) * c_k / d, in k order, the zero centre term skipped
+// one stencil sum along `stride` of the first- (SECOND=0) or second-derivative coefficients:
+// EXACT: sum of f(. + s + k) * c_k / d in k order, the zero centre term of the first derivative
+// skipped (an exact +-0 addend); FAST: sum of f * (c_k / d) with the reciprocal of d hoisted.
+template <int ORDER, bool SECOND>
+__device__ __forceinline__ double fd_sum(const double* f, long stride, double d, double id) {
+    constexpr int H     = ORDER / 2;
+    constexpr int OI    = ORDER / 2 - 1;
+    double        s     = 0.;
+    bool          first = true;
+#pragma unroll
+    for (int t = 0; t <= 2 * H; t++) {
+        if (!SECOND && t == H) continue;
+        double co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
+#if !NS_FAST
+        double v = f[(long)(t - H) * stride] * co / d;
+#else
+        double v = f[(long)(t - H) * stride] * (co * id);
+        (void)d;
+#endif
+        if (first) {
+            s = v, first = false;
+        } else {
+            s += v;
+        }
+    }
+#if !NS_FAST
+    (void)id;
+#endif
+    return s;
+}
+
+// first derivatives of rho, p, u, v along x and y on the interior
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
@@ -507,28 +675,12 @@ __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
     int j = threadIdx.y + blockIdx.y * blockDim.y;
     int i = blockIdx.z;
     if (k >= g.nz || j >= g.ny) return;
-    constexpr int  H  = ORDER / 2;
-    constexpr int  OI = ORDER / 2 - 1;
-    const long     c  = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+    const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        const double* f  = W + q * g.P + c;
-        double        sx = 0., sy = 0.;
-        bool          first = true;
-#pragma unroll
-        for (int t = 0; t <= 2 * H; t++) {
-            if (t == H) continue;
-            double co = c_fd1[OI][t];
-            double vx = f[(long)(t - H) * g.sx] * co / g.dx;
-            double vy = f[(long)(t - H) * g.sy] * co / g.dy;
-            if (first) {
-                sx = vx, sy = vy, first = false;
-            } else {
-                sx += vx, sy += vy;
-            }
-        }
-        Dx[c + q * g.P] = sx;
-        Dy[c + q * g.P] = sy;
+        const double* f = W + q * g.P + c;
+        Dx[c + q * g.P] = fd_sum<ORDER, false>(f, g.sx, g.dx, g.idx);
+        Dy[c + q * g.P] = fd_sum<ORDER, false>(f, g.sy, g.dy, g.idy);
     }
 }
 
@@ -536,11 +688,11 @@ __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
 // dW/dx and the i halos of dW/dy (centered_diffusion.cpp:22-24).  At a physical wall the halos
 // stay zero: the arrays are zeroed once at allocation and never written there.
 __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
-    int k    = threadIdx.x + blockIdx.x * blockDim.x;
-    int t1   = threadIdx.y + blockIdx.y * blockDim.y;
-    int side = blockIdx.z / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
-    int h    = blockIdx.z % NGC;
-    bool low = ((blockIdx.z / NGC) & 1) == 0;
+    int  k    = threadIdx.x + blockIdx.x * blockDim.x;
+    int  t1   = threadIdx.y + blockIdx.y * blockDim.y;
+    int  side = blockIdx.z / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
+    int  h    = blockIdx.z % NGC;
+    bool low  = ((blockIdx.z / NGC) & 1) == 0;
     if (k >= g.nz) return;
     if (side == 0) {
         if (!b.periodic[1] || t1 >= g.nx) return;
@@ -559,30 +711,9 @@ __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
     }
 }
 
-// one stencil sum along `stride` of the first- (SECOND=0) or second-derivative coefficients
-template <int ORDER, bool SECOND>
-__device__ __forceinline__ double fd_sum(const double* f, long stride, double d) {
-    constexpr int H  = ORDER / 2;
-    constexpr int OI = ORDER / 2 - 1;
-    double        s  = 0.;
-    bool          first = true;
-#pragma unroll
-    for (int t = 0; t <= 2 * H; t++) {
-        if (!SECOND && t == H) continue;
-        double co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
-        double v  = f[(long)(t - H) * stride] * co / d;
-        if (first) {
-            s = v, first = false;
-        } else {
-            s += v;
-        }
-    }
-    return s;
-}
-
 // CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
 // constant-viscosity zero terms (dmu = dlamb = 0) dropped: each is an exact +-0 addend.
-template <int ORDER>
+template <int ORDER, bool SQ>
 __device__ __forceinline__ void viscous(const double* __restrict__ W,
                                         const double* __restrict__ Dx,
                                         const double* __restrict__ Dy, long c, const Geo& g,
@@ -590,49 +721,59 @@ __device__ __forceinline__ void viscous(const double* __restrict__ W,
                                         double& y_mom) {
     const long P   = g.P;
     const int  rho = 0, p = 1, u = 2, v = 3;
-    double     dxx = g.dx * g.dx, dyy = g.dy * g.dy;
-    // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79)
-    double du_dxdx   = fd_sum<ORDER, true>(W + u * P + c, g.sx, dxx);
-    double du_dydy   = fd_sum<ORDER, true>(W + u * P + c, g.sx, dyy);
-    double dv_dxdx   = fd_sum<ORDER, true>(W + v * P + c, g.sx, dxx);
-    double dv_dydy   = fd_sum<ORDER, true>(W + v * P + c, g.sx, dyy);
-    double drho_dxdx = fd_sum<ORDER, true>(W + rho * P + c, g.sx, dxx);
-    double drho_dydy = fd_sum<ORDER, true>(W + rho * P + c, g.sx, dyy);
-    double dp_dxdx   = fd_sum<ORDER, true>(W + p * P + c, g.sx, dxx);
-    double dp_dydy   = fd_sum<ORDER, true>(W + p * P + c, g.sx, dyy);
+    // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79); when
+    // dx*dx == dy*dy the two sums are the same operations on the same operands
+    double du_dxdx   = fd_sum<ORDER, true>(W + u * P + c, g.sx, g.dxx, g.idxx);
+    double dv_dxdx   = fd_sum<ORDER, true>(W + v * P + c, g.sx, g.dxx, g.idxx);
+    double drho_dxdx = fd_sum<ORDER, true>(W + rho * P + c, g.sx, g.dxx, g.idxx);
+    double dp_dxdx   = fd_sum<ORDER, true>(W + p * P + c, g.sx, g.dxx, g.idxx);
+    double du_dydy, dv_dydy, drho_dydy, dp_dydy;
+    if constexpr (SQ) {
+        du_dydy = du_dxdx, dv_dydy = dv_dxdx, drho_dydy = drho_dxdx, dp_dydy = dp_dxdx;
+    } else {
+        du_dydy   = fd_sum<ORDER, true>(W + u * P + c, g.sx, g.dyy, g.idyy);
+        dv_dydy   = fd_sum<ORDER, true>(W + v * P + c, g.sx, g.dyy, g.idyy);
+        drho_dydy = fd_sum<ORDER, true>(W + rho * P + c, g.sx, g.dyy, g.idyy);
+        dp_dydy   = fd_sum<ORDER, true>(W + p * P + c, g.sx, g.dyy, g.idyy);
+    }
     // cross derivatives: dW_dxdy = df_dy(dW_dx), dW_dydx = df_dx(dW_dy)
-    double du_dxdy = fd_sum<ORDER, false>(Dx + u * P + c, g.sy, g.dy);
-    double dv_dxdy = fd_sum<ORDER, false>(Dx + v * P + c, g.sy, g.dy);
-    double du_dydx = fd_sum<ORDER, false>(Dy + u * P + c, g.sx, g.dx);
-    double dv_dydx = fd_sum<ORDER, false>(Dy + v * P + c, g.sx, g.dx);
+    double du_dxdy = fd_sum<ORDER, false>(Dx + u * P + c, g.sy, g.dy, g.idy);
+    double dv_dxdy = fd_sum<ORDER, false>(Dx + v * P + c, g.sy, g.dy, g.idy);
+    double du_dydx = fd_sum<ORDER, false>(Dy + u * P + c, g.sx, g.dx, g.idx);
+    double dv_dydx = fd_sum<ORDER, false>(Dy + v * P + c, g.sx, g.dx, g.idx);
 
     double du_dx = Dx[u * P + c], dv_dx = Dx[v * P + c], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
     double du_dy = Dy[u * P + c], dv_dy = Dy[v * P + c], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
     double U = W[u * P + c], V = W[v * P + c], Pp = W[p * P + c], RHO = W[rho * P + c];
 
-    double mu   = ph.mu;
-    double lamb = (-2. / 3.) * mu;
-    double Dtauxx_dx = 2.0 * mu * du_dxdx + lamb * (du_dxdx + dv_dydx);
-    double Dtauyy_dy = 2.0 * mu * dv_dydy + lamb * (du_dxdy + dv_dydy);
+    double mu        = ph.mu;
+    double lamb      = ph.lamb;
+    double Dtauxx_dx = ph.two_mu * du_dxdx + lamb * (du_dxdx + dv_dydx);
+    double Dtauyy_dy = ph.two_mu * dv_dydy + lamb * (du_dxdy + dv_dydy);
     double Dtauxy_dx = mu * (du_dydx + dv_dxdx);
     double Dtauyx_dy = mu * (du_dydy + dv_dxdy);
-    double Tauxx     = 2.0 * mu * du_dx + lamb * (du_dx + dv_dy);
+    double Tauxx     = ph.two_mu * du_dx + lamb * (du_dx + dv_dy);
     double Tauyy     = Tauxx;  // tauxx(...) as written at centered_diffusion.cpp:143
     double Tauxy     = mu * (du_dy + dv_dx);
     double Tauyx     = Tauxy;
     double Dtauvx_dx = U * Dtauxx_dx + V * Dtauxy_dx + Tauxx * du_dx + Tauxy * dv_dy;
     double Dtauvy_dy = U * Dtauyx_dy + V * Dtauyy_dy + Tauyy * dv_dy + Tauyx * du_dy;
-    double one_rho   = 1. / RHO;
-    double one_rho2  = 1. / (RHO * RHO);
-    double one_rho3  = 1. / (RHO * RHO * RHO);
+#if !NS_FAST
+    double one_rho  = 1. / RHO;
+    double one_rho2 = 1. / (RHO * RHO);
+    double one_rho3 = 1. / (RHO * RHO * RHO);
+#else
+    double one_rho  = 1. / RHO;
+    double one_rho2 = one_rho * one_rho;
+    double one_rho3 = one_rho2 * one_rho;
+#endif
     double temp2x = mu * (2.0 * one_rho2 * drho_dx * dp_dx + one_rho * dp_dxdx +
                           one_rho3 * Pp * drho_dx * drho_dx - one_rho2 * Pp * drho_dxdx);
-    double temp3  = ph.gamma / (ph.Pr * (ph.gamma - 1.0));
-    double Dqx_dx = -temp3 * temp2x;
+    double Dqx_dx = -ph.heat * temp2x;
     // the y heat-flux term multiplies by the enum value p = 1 (centered_diffusion.cpp:166)
     double temp2y = mu * (2.0 * one_rho2 * drho_dy * dp_dy + one_rho * dp_dydy +
                           one_rho3 * drho_dy * drho_dy - one_rho2 * Pp * drho_dydy);
-    double Dqy_dy = -temp3 * temp2y;
+    double Dqy_dy = -ph.heat * temp2y;
     x_mom         = -Dtauxx_dx - Dtauyx_dy;
     y_mom         = -Dtauyy_dy - Dtauxy_dx;
     ene           = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
@@ -641,8 +782,9 @@ __device__ __forceinline__ void viscous(const double* __restrict__ W,
 // residual + stage update at one interior cell:
 // R = R_in + (fx_i - fx_{i-1})/dx + (fy_j - fy_{j-1})/dy + (fz_k - fz_{k-1})/dz
 // (Weno5::compute_residual, weno-5.cpp:39-73), then the viscous terms (centered_diffusion.cpp:
-// 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96).
-template <int ORDER>
+// 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96)
+// and the primitives of the new state (eq_of_state.cpp:61-87).
+template <int ORDER, bool SQ>
 __global__ __launch_bounds__(256) void k_residual(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
@@ -664,20 +806,32 @@ __global__ __launch_bounds__(256) void k_residual(
     if ((which & 1) && ph.conv) {
 #pragma unroll
         for (int q = 0; q < 5; q++) {
+#if !NS_FAST
             double a = (fx[c + q * P] - fx[c - g.sx + q * P]) / g.dx;
-            R[q]     = have[q] ? R[q] + a : a;
-            R[q] += (fy[c + q * P] - fy[c - g.sy + q * P]) / g.dy;
-            if (g.dim == 3) R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) / g.dz;
+            double b = (fy[c + q * P] - fy[c - g.sy + q * P]) / g.dy;
+#else
+            double a = (fx[c + q * P] - fx[c - g.sx + q * P]) * g.idx;
+            double b = (fy[c + q * P] - fy[c - g.sy + q * P]) * g.idy;
+#endif
+            R[q] = have[q] ? R[q] + a : a;
+            R[q] += b;
+            if (g.dim == 3) {
+#if !NS_FAST
+                R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) / g.dz;
+#else
+                R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) * g.idz;
+#endif
+            }
             have[q] = true;
         }
     }
     if constexpr (ORDER > 0) {
         if (which & 2) {
             double ene, xm, ym;
-            viscous<ORDER>(W, Dx, Dy, c, g, ph, ene, xm, ym);
-            R[1] = have[1] ? R[1] + ene : ene;
-            R[2] = have[2] ? R[2] + xm : xm;
-            R[3] = have[3] ? R[3] + ym : ym;
+            viscous<ORDER, SQ>(W, Dx, Dy, c, g, ph, ene, xm, ym);
+            R[1]    = have[1] ? R[1] + ene : ene;
+            R[2]    = have[2] ? R[2] + xm : xm;
+            R[3]    = have[3] ? R[3] + ym : ym;
             have[1] = have[2] = have[3] = true;
         }
     }
@@ -696,19 +850,21 @@ __global__ __launch_bounds__(256) void k_residual(
     } else {
         m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
     }
-    double un[5];
+    double un[5], w[5];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         double r = have[q] ? R[q] : 0.0;
-        un[q]    = (st.Un[c + q * P] * m1) + (st.Ucur[c + q * P] * m2) + (r * m3);
+#if !NS_FAST
+        un[q] = (st.Un[c + q * P] * m1) + (st.Ucur[c + q * P] * m2) + (r * m3);
+#else
+        // explicit order-preserving form; FMA contraction is allowed in this mode
+        un[q] = st.Un[c + q * P] * m1 + st.Ucur[c + q * P] * m2 + r * m3;
+#endif
         st.Uout[c + q * P] = un[q];
     }
-    if (st.Wout) {  // conservative_to_primitive after the last stage (time_scheme.cpp:69)
-        double w[5];
-        c2p(un, ph.gamma, w);
+    c2p(un, ph.gamma, w);
 #pragma unroll
-        for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
-    }
+    for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
 }
 
 __global__ void k_time_accum(double* t, const unsigned long long* dtmin, double cfl) {
@@ -743,16 +899,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict
                                                         unsigned long long* dtmin,
                                                         unsigned* flags, Geo g, Phys ph) {
     __shared__ double sh[5][RED_THREADS];
-    long   n     = (long)g.nx * g.ny * g.nz;
-    double acc[5] = {0, 0, 0, 0, 0};
-    double dtl    = 1E300;
-    unsigned fl   = 0;
-    for (long t = threadIdx.x + (long)blockIdx.x * RED_THREADS; t < n; t += (long)gridDim.x * RED_THREADS) {
-        int  k = (int)(t % g.nz);
-        long r = t / g.nz;
-        int  j = (int)(r % g.ny);
-        int  i = (int)(r / g.ny);
-        long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+    long     n      = (long)g.nx * g.ny * g.nz;
+    double   acc[5] = {0, 0, 0, 0, 0};
+    double   dtl    = 1E300;
+    unsigned fl     = 0;
+    for (long t = threadIdx.x + (long)blockIdx.x * RED_THREADS; t < n;
+         t += (long)gridDim.x * RED_THREADS) {
+        int    k = (int)(t % g.nz);
+        long   r = t / g.nz;
+        int    j = (int)(r % g.ny);
+        int    i = (int)(r / g.ny);
+        long   c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
         double w[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
@@ -786,7 +943,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partial,
                                                               int nb, double* __restrict__ sums) {
     __shared__ double sh[5][RED_THREADS];
-    double acc[5] = {0, 0, 0, 0, 0};
+    double            acc[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nb; b += RED_THREADS)
         for (int q = 0; q < 5; q++) acc[q] += partial[b * 5 + q];
     for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
@@ -810,34 +967,59 @@ static const dim3 BLK(64, 4, 1);
 static void l_reset_min(unsigned long long* bits, hipStream_t s) {
     k_reset_min<<<1, 1, 0, s>>>(bits);
 }
+static void l_zero_flags(unsigned* flags, hipStream_t s) { k_zero_flags<<<1, 1, 0, s>>>(flags); }
 
-static void l_step_begin(const double* W, double* Ua, double* Un, unsigned long long* dtmin,
-                         Geo g, Phys ph, hipStream_t s) {
-    k_step_begin<<<cell_grid(g.nz, g.ny, g.nx, BLK), BLK, 0, s>>>(W, Ua, Un, dtmin, g, ph.gamma);
+static void l_step_begin(const double* W, double* Ua, double* Un, double* Wst,
+                         unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s) {
+    long n  = (long)g.nx * g.ny * g.nz;
+    long nb = (n + 255) / 256;
+    if (nb > 2048) nb = 2048;
+    k_step_begin<<<(unsigned)nb, 256, 0, s>>>(W, Ua, Un, Wst, dtmin, g, ph.gamma);
 }
 
-static void l_halo(double* A, int nvar, Geo g, Bnd b, hipStream_t s) {
+static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s) {
     int  m1 = g.ny > g.nx ? g.ny : g.nx;
     int  m2 = g.nz > g.ny ? g.nz : g.ny;
     dim3 blk(64, 4, 1);
     dim3 grd((m2 + 63) / 64, (m1 + 3) / 4, 6 * NGC);
-    k_halo<<<grd, blk, 0, s>>>(A, nvar, g, b);
+    k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b);
 }
 
 static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t s) {
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
 
+template <int DIR, int CONV, bool B>
+static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
+                     hipStream_t s) {
+    dim3 blk(TILE_L, 8, 1);
+    if constexpr (DIR == 2) {
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
+        k_flux_tile_z<CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma);
+    } else {
+        int  n = (DIR == 0) ? g.nx : g.ny;
+        int  o = (DIR == 0) ? g.ny : g.nx;
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S - 1) / TILE_S);
+        k_flux_tile_xy<DIR, CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma);
+    }
+    // the last face plane (sweep index NGC + n - 1)
+    int  n  = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
+    int  n1 = (DIR == 0) ? g.ny : g.nx;
+    int  n2 = (DIR == 2) ? g.ny : g.nz;
+    dim3 pb(64, 4, 1);
+    dim3 pg((n2 + 63) / 64, (n1 + 3) / 4, 1);
+    k_flux_plane<DIR, CONV, B><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1);
+}
+
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
                           hipStream_t s) {
-    dim3 grd = cell_grid(g.nz + (DIR == 2), g.ny + (DIR == 1), g.nx + (DIR == 0), BLK);
     if (ph.conv == 1) {
-        if (ph.borges) k_face_flux<DIR, 1, true><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
-        else k_face_flux<DIR, 1, false><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+        if (ph.borges) flux_dir<DIR, 1, true>(U, W, f, g, ph.gamma, s);
+        else flux_dir<DIR, 1, false>(U, W, f, g, ph.gamma, s);
     } else {
-        if (ph.borges) k_face_flux<DIR, 2, true><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
-        else k_face_flux<DIR, 2, false><<<grd, BLK, 0, s>>>(U, W, f, g, ph.gamma);
+        if (ph.borges) flux_dir<DIR, 2, true>(U, W, f, g, ph.gamma, s);
+        else flux_dir<DIR, 2, false>(U, W, f, g, ph.gamma, s);
     }
 }
 
@@ -862,18 +1044,29 @@ static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s) {
     k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b);
 }
 
+template <int ORDER>
+static void residual_order(const double* fx, const double* fy, const double* fz, const double* W,
+                           const double* Dx, const double* Dy, const double* Rin, double* Rout,
+                           StageArgs a, const unsigned long long* dtmin, int which, Geo g,
+                           Phys ph, hipStream_t s) {
+    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
+    if (g.square_xy)
+        k_residual<ORDER, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    else
+        k_residual<ORDER, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+}
+
 static void l_residual(const double* fx, const double* fy, const double* fz, const double* W,
                        const double* Dx, const double* Dy, const double* Rin, double* Rout,
                        const StageArgs* st, const unsigned long long* dtmin, int which, Geo g,
                        Phys ph, hipStream_t s) {
     StageArgs a{};
     if (st) a = *st;
-    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
     switch (ph.diff_order) {
-    case 2: k_residual<2><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
-    case 4: k_residual<4><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
-    case 6: k_residual<6><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
-    default: k_residual<0><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph); break;
+    case 2: residual_order<2>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    case 4: residual_order<4>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    case 6: residual_order<6>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    default: residual_order<0>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
     }
 }
 
@@ -903,10 +1096,10 @@ static void l_time_accum(double* t, const unsigned long long* dtmin, double cfl,
 }
 
 const Launchers& launchers() {
-    static const Launchers L = {l_reset_min,  l_step_begin, l_halo,         l_c2p_faces,
-                                l_face_flux,  l_deriv1,     l_deriv_halo,   l_residual,
-                                l_aos_to_soa, l_soa_to_aos, l_reduce,       l_reduce_final,
-                                l_time_accum};
+    static const Launchers L = {l_reset_min,    l_step_begin, l_halo,       l_c2p_faces,
+                                l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
+                                l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
+                                l_time_accum,   l_zero_flags};
     return L;
 }
 

@@ -39,6 +39,7 @@ struct nsgpu_ctx {
     // device buffers (SoA)
     double*            W  = nullptr;  // primitive state (5 planes)
     double*            W2 = nullptr;  // next primitive state (written by the last stage)
+    double*            Ws[2] = {nullptr, nullptr};  // stage primitives (ping-pong)
     double*            Un = nullptr;  // step-start conservative state
     double*            Ua = nullptr;  // stage states (ping-pong)
     double*            Ub = nullptr;
@@ -197,13 +198,21 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     g.dx = cfg->Lx / cfg->nx;  // Grid::compute_step_sizes (grid.cpp:14-16)
     g.dy = cfg->Ly / cfg->ny;
     g.dz = cfg->Lz / cfg->nz;
-    g.dim = cfg->dim;
+    g.dxx = g.dx * g.dx;  // CDN::ddf_dxdx / ddf_dydy divisors (cd-n.cpp:67-79)
+    g.dyy = g.dy * g.dy;
+    g.idx = 1.0 / g.dx, g.idy = 1.0 / g.dy, g.idz = 1.0 / g.dz;
+    g.idxx = 1.0 / g.dxx, g.idyy = 1.0 / g.dyy;
+    g.dim       = cfg->dim;
+    g.square_xy = (g.dxx == g.dyy);
     static const int lead[3] = {0, 3, 4};
     for (int a = 0; a < 3; a++) c->b.periodic[a] = (cfg->bc[lead[a]] == NSGPU_BC_PERIODIC);
     for (int f = 0; f < 6; f++) c->b.face_kind[f] = cfg->bc[f];
     c->ph.gamma = cfg->gamma, c->ph.mu = cfg->mu, c->ph.Pr = cfg->Pr, c->ph.cfl = cfg->cfl;
     c->ph.conv = cfg->conv, c->ph.borges = (cfg->weights == NSGPU_WEIGHTS_BORGES);
     c->ph.diff_order = cfg->diff_order;
+    c->ph.lamb       = (-2. / 3.) * cfg->mu;                           // centered_diffusion.cpp:101
+    c->ph.two_mu     = 2.0 * cfg->mu;
+    c->ph.heat       = cfg->gamma / (cfg->Pr * (cfg->gamma - 1.0));    // centered_diffusion.cpp:161
     c->L = (cfg->math == NSGPU_MATH_FAST) ? &fast::launchers() : &exact::launchers();
     c->plane_bytes = (size_t)g.P * sizeof(double);
     size_t five = 5 * c->plane_bytes, four = 4 * c->plane_bytes;
@@ -211,6 +220,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (!rc) rc = alloc(&c->W, five);
     if (!rc) rc = alloc(&c->W2, five);
+    for (int t = 0; t < 2 && !rc; t++) rc = alloc(&c->Ws[t], five);
     if (!rc) rc = alloc(&c->Un, five);
     if (!rc) rc = alloc(&c->Ua, five);
     if (!rc) rc = alloc(&c->Ub, five);
@@ -240,7 +250,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->W,  c->W2, c->Un, c->Ua, c->Ub, c->F[0], c->F[1], c->F[2], c->Dx, c->Dy,
+    double* bufs[] = {c->W,  c->W2, c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub, c->F[0], c->F[1], c->F[2], c->Dx, c->Dy,
                       c->R,  c->stage_aos, c->partial, (double*)c->dtmin, (double*)c->flags,
                       c->dtime, c->sums};
     for (double* p : bufs)
@@ -285,7 +295,7 @@ int nsgpu_apply_boundary(nsgpu_ctx* c, double* aos) {
     HIPCHK(hipSetDevice(c->device));
     int rc = upload_soa(c, aos, c->R);
     if (rc) return rc;
-    c->L->halo(c->R, 5, c->g, c->b, c->stream);
+    c->L->halo(c->R, 5, nullptr, 0, c->g, c->b, c->stream);
     return download_soa(c, c->R, aos);
 }
 
@@ -327,7 +337,7 @@ int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int whi
     if (rc) return rc;
     {
         Scope s(c, T_HALO);
-        c->L->halo(c->Ua, 5, c->g, c->b, c->stream);
+        c->L->halo(c->Ua, 5, nullptr, 0, c->g, c->b, c->stream);
     }
     {
         Scope s(c, T_C2P);
@@ -356,32 +366,34 @@ int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
         {
             Scope sc(c, T_STEP_BEGIN);
             L.reset_min(&c->dtmin[0], c->stream);
-            L.step_begin(c->W, c->Ua, c->Un, &c->dtmin[0], c->g, c->ph, c->stream);
+            L.step_begin(c->W, c->Ua, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
         }
-        double* cur = c->Ua;
-        double* nxt = c->Ub;
+        double* cur  = c->Ua;
+        double* nxt  = c->Ub;
+        double* wcur = c->Ws[0];
+        double* wnxt = c->Ws[1];
         for (int st = 0; st < 3; st++) {
             {
+                // halos of U and of its primitives: the same copies (c2p acts per cell)
                 Scope sc(c, T_HALO);
-                L.halo(cur, 5, c->g, c->b, c->stream);
+                L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
             }
-            {
-                Scope sc(c, T_C2P);
-                L.c2p_faces(cur, c->W, c->g, c->ph, c->stream);
-            }
-            convection_fluxes(c, cur, c->W);
-            diffusion_derivs(c, c->W);
-            // the last stage writes c2p(U) into the other primitive buffer: the residual reads
-            // the stage primitives at neighbouring cells while it runs
-            StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : nullptr, st};
+            convection_fluxes(c, cur, wcur);
+            diffusion_derivs(c, wcur);
+            // the stage writes U and c2p(U) of the next stage; the last stage writes the
+            // primitive state into the other state buffer (the residual reads wcur meanwhile)
+            StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st};
             {
                 Scope sc(c, T_RESIDUAL);
-                L.residual(c->F[0], c->F[1], c->F[2], c->W, c->Dx, c->Dy, nullptr, nullptr, &a,
+                L.residual(c->F[0], c->F[1], c->F[2], wcur, c->Dx, c->Dy, nullptr, nullptr, &a,
                            &c->dtmin[0], NSGPU_RHS_BOTH, c->g, c->ph, c->stream);
             }
             double* t = cur;
             cur       = nxt;
             nxt       = t;
+            t         = wcur;
+            wcur      = wnxt;
+            wnxt      = t;
         }
         c->Ulast = cur;
         double* t = c->W;
@@ -412,7 +424,7 @@ int nsgpu_reduce(nsgpu_ctx* c, double sums[5], double* dt_min, unsigned* flags) 
     if (!c) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     c->L->reset_min(&c->dtmin[1], c->stream);
-    HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(unsigned), c->stream));
+    c->L->zero_flags(c->flags, c->stream);
     int nb = 0;
     c->L->reduce(c->Ulast, c->W, c->partial, c->partial_max, &c->dtmin[1], c->flags, c->g, c->ph,
                  c->stream, &nb);
