@@ -31,7 +31,7 @@ STATE = {"conservative": 0, "primitive": 1}
 EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len", "nsgpu_set_state",
            "nsgpu_get_state", "nsgpu_apply_boundary", "nsgpu_convection_solve",
            "nsgpu_diffusion_solve", "nsgpu_rhs", "nsgpu_step", "nsgpu_get_dt", "nsgpu_reduce",
-           "nsgpu_stream", "nsgpu_timing", "nsgpu_timer_count", "nsgpu_timer_name",
+           "nsgpu_stream", "nsgpu_selftest", "nsgpu_timing", "nsgpu_timer_count", "nsgpu_timer_name",
            "nsgpu_timer_read"]
 
 
@@ -56,11 +56,13 @@ class NsgpuError(RuntimeError):
 _lib = None
 
 
-def load_library(path=LIB_PATH):
-    """Loads lib/libnsgpu.so (raises if it was not built: there is no fallback)."""
+def load_library(path=None):
+    """Loads lib/libnsgpu.so, or the library named by $NSGPU_LIB (a tuning build); raises if it
+    was not built: there is no fallback."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("NSGPU_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is missing: build it with `python 3dnssolver_amd/build.py`")
     L = ctypes.CDLL(path)
@@ -83,6 +85,8 @@ def load_library(path=LIB_PATH):
     L.nsgpu_reduce.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_uint)]
     L.nsgpu_stream.argtypes = [P]
     L.nsgpu_stream.restype = P
+    L.nsgpu_selftest.argtypes = [ctypes.c_int, ctypes.c_long, ctypes.c_ulonglong,
+                                 ctypes.POINTER(ctypes.c_long)]
     L.nsgpu_timing.argtypes = [P, ctypes.c_int]
     L.nsgpu_timer_name.restype = ctypes.c_char_p
     L.nsgpu_timer_read.argtypes = [P, ctypes.c_int, D, ctypes.POINTER(ctypes.c_long)]
@@ -109,6 +113,16 @@ def make_config(case, math="exact"):
         c.bc[f] = BC[name]
     c.math = MATH[math]
     return c
+
+
+def selftest(n=1 << 26, seed=1, device=0):
+    """Number of random operand pairs where the kernels' division differs from IEEE `/`."""
+    L = load_library()
+    out = ctypes.c_long(-1)
+    rc = L.nsgpu_selftest(device, n, seed, ctypes.byref(out))
+    if rc != 0:
+        raise NsgpuError(rc, L.nsgpu_strerror(rc).decode())
+    return out.value
 
 
 class Solver:

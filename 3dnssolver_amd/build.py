@@ -25,22 +25,25 @@ def _run(cmd):
     return r
 
 
-def build(verbose=False):
-    os.makedirs(OUT, exist_ok=True)
+def build(verbose=False, defines=(), out=None):
+    """Compiles and links the library; `defines` are extra -D flags for the kernel units (tuning
+    variants), `out` an alternative output directory."""
+    OUT_ = out or OUT
+    os.makedirs(OUT_, exist_ok=True)
     objs = []
     units = [
-        ("ns_kernels.hip", "kernels_exact.o", ["-DNS_FAST=0", "-ffp-contract=off"]),
-        ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=fast"]),
+        ("ns_kernels.hip", "kernels_exact.o", ["-DNS_FAST=0", "-ffp-contract=off", *defines]),
+        ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=fast", *defines]),
         ("nsgpu_capi.hip", "capi.o", ["-ffp-contract=off"]),
     ]
     for src, obj, extra in units:
-        o = os.path.join(OUT, obj)
+        o = os.path.join(OUT_, obj)
         cmd = ["hipcc", "-c", *COMMON, *extra, os.path.join(CSRC, src), "-o", o]
         if verbose:
             print(" ".join(cmd))
         _run(cmd)
         objs.append(o)
-    lib = os.path.join(OUT, "libnsgpu.so")
+    lib = os.path.join(OUT_, "libnsgpu.so")
     tmp = lib + ".tmp"
     _run(["hipcc", "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, lib)
@@ -48,4 +51,10 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import argparse
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    print(build(verbose=True, defines=[f"-D{d}" for d in a.defines], out=a.out))

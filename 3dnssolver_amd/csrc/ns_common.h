@@ -79,6 +79,8 @@ struct Launchers {
     void (*reduce_final)(const double* partial, int nblocks, double* sums, hipStream_t s);
     void (*time_accum)(double* t, const unsigned long long* dtmin, double cfl, hipStream_t s);
     void (*zero_flags)(unsigned* flags, hipStream_t s);
+    // counts the operand pairs where the division routine of the kernels differs from `/`
+    void (*selftest_div)(long n, unsigned long long seed, unsigned long long* count, hipStream_t s);
 };
 
 namespace exact {

@@ -17,6 +17,12 @@
 #ifndef NS_FAST
 #error "compile with -DNS_FAST=0 or -DNS_FAST=1"
 #endif
+#ifndef NS_ROW_FENCE
+#define NS_ROW_FENCE 1
+#endif
+#ifndef NS_XCD_MAP
+#define NS_XCD_MAP 0
+#endif
 #if NS_FAST
 #define NS_MODE fast
 #else
@@ -46,6 +52,42 @@ __constant__ double c_fd2[3][7] = {
     {1. / 90., -3. / 20., 3. / 2., -49. / 18., 3. / 2., -3. / 20., 1. / 90.}};
 
 // ------------------------------------------------------------------------------------------
+// FP64 division.  IEEE division on gfx950 is div_scale x2, rcp, 2 Newton steps, q = n r,
+// remainder, div_fmas, div_fixup.  div_scale leaves its operands unchanged and div_fmas is a plain
+// fma unless an exponent is near the ends of the range (|exponents| < ~900 and an exponent
+// difference < ~700 never scale), and div_fixup only rewrites NaN / Inf / zero / overflowing
+// quotients.  On the operands of this path (physical states, grid spacings, smoothness indicators)
+// the sequence below is therefore the same arithmetic without the 3 range-fixing instructions:
+// the correctly rounded quotient.  nsgpu_selftest checks it against the IEEE operator on random
+// operands over exponents [-300, 300]; only the sign of an exact-zero quotient can differ.
+// ------------------------------------------------------------------------------------------
+#ifndef NS_DIV_IEEE
+#define NS_DIV_IEEE 0
+#endif
+// refined reciprocal of a divisor: depends on d only, so quotients by one divisor share it
+struct Rcp {
+    double d, r;
+};
+__device__ __forceinline__ Rcp rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r        = __builtin_fma(r, e, r);
+    e        = __builtin_fma(-d, r, 1.0);
+    r        = __builtin_fma(r, e, r);
+    return {d, r};
+}
+// n / d from the refined reciprocal: q = n r, remainder n - d q (exact), corrected quotient
+__device__ __forceinline__ double dv(double n, const Rcp& R) {
+#if NS_DIV_IEEE
+    return n / R.d;
+#endif
+    double q   = n * R.r;
+    double rem = __builtin_fma(-R.d, q, n);
+    return __builtin_fma(rem, R.r, q);
+}
+__device__ __forceinline__ double dv(double n, double d) { return dv(n, rcp(d)); }
+
+// ------------------------------------------------------------------------------------------
 // WENO-5 (weno-5.cpp:222-322)
 // ------------------------------------------------------------------------------------------
 template <bool BORGES>
@@ -66,16 +108,17 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
     double a0, a1, a2;
     if constexpr (BORGES) {
         double tau5 = fabs(b0 - b2);
-        a0          = d0 * (1.0 + (tau5 / (b0 + EPS_BORGES)));
-        a1          = d1 * (1.0 + (tau5 / (b1 + EPS_BORGES)));
-        a2          = d2 * (1.0 + (tau5 / (b2 + EPS_BORGES)));
+        a0          = d0 * (1.0 + dv(tau5, b0 + EPS_BORGES));
+        a1          = d1 * (1.0 + dv(tau5, b1 + EPS_BORGES));
+        a2          = d2 * (1.0 + dv(tau5, b2 + EPS_BORGES));
     } else {
-        a0 = d0 / ((EPS_SHU + b0) * (EPS_SHU + b0));
-        a1 = d1 / ((EPS_SHU + b1) * (EPS_SHU + b1));
-        a2 = d2 / ((EPS_SHU + b2) * (EPS_SHU + b2));
+        a0 = dv(d0, (EPS_SHU + b0) * (EPS_SHU + b0));
+        a1 = dv(d1, (EPS_SHU + b1) * (EPS_SHU + b1));
+        a2 = dv(d2, (EPS_SHU + b2) * (EPS_SHU + b2));
     }
     double al = a0 + a1 + a2;
-    return (a0 / al) * f0 + (a1 / al) * f1 + (a2 / al) * f2;
+    Rcp    ra = rcp(al);
+    return dv(a0, ra) * f0 + dv(a1, ra) * f1 + dv(a2, ra) * f2;
 #else
     // one division: with s_k = (eps + b_k)^2 the Shu weights are
     // w_k = d_k prod_{j!=k} s_j / sum_m d_m prod_{j!=m} s_j; Borges a_k = d_k (1 + tau/e_k)
@@ -95,7 +138,7 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
         p1        = d1 * (s0 * s2);
         p2        = d2 * (s0 * s1);
     }
-    return (p0 * f0 + p1 * f1 + p2 * f2) / (p0 + p1 + p2);
+    return dv(p0 * f0 + p1 * f1 + p2 * f2, p0 + p1 + p2);
 #endif
 }
 
@@ -127,10 +170,10 @@ __device__ __forceinline__ void split_cell(const double* __restrict__ U, const d
     double rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
     double qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
 #if !NS_FAST
-    double cs = sqrt(gamma * p / rho);
-    double H  = cs * cs / (gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+    double cs = sqrt(dv(gamma * p, rho));
+    double H  = dv(cs * cs, gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
 #else
-    double H = (gamma / (gamma - 1.0)) * (p / rho) + 0.5 * (u * u + v * v + w * w);
+    double H = dv(gamma, gamma - 1.0) * dv(p, rho) + 0.5 * (u * u + v * v + w * w);
 #endif
     double F[5];
     double m = rho * qn;
@@ -192,20 +235,22 @@ __device__ __forceinline__ void eigen(double rho, double p, double u, double v, 
     double q2 = u * u + v * v + w * w;
     double K  = gamma - 1.0;
 #if !NS_FAST
-    double c2   = gamma / (p * rho);
+    double c2   = dv(gamma, p * rho);
     double c    = sqrt(c2);
-    double H    = c2 / (gamma - 1.0) + 0.5 * q2;
-    double K_pc = K / (2.0 * c2);
-    double qn2c = qn / (2.0 * c);
-    double i2c  = 1.0 / (2.0 * c);  // n_a / (2c) for the normal component
+    double H    = dv(c2, gamma - 1.0) + 0.5 * q2;
+    double K_pc = dv(K, 2.0 * c2);
+    Rcp    r2c  = rcp(2.0 * c);
+    double qn2c = dv(qn, r2c);
+    double i2c  = dv(1.0, r2c);  // n_a / (2c) for the normal component
+    Rcp    rc2  = rcp(c2);
 #pragma unroll
-    for (int a = 0; a < 3; a++) E.L2[1 + a] = K * E.vel[a] / c2;
-    E.L2[4] = -K / c2;
+    for (int a = 0; a < 3; a++) E.L2[1 + a] = dv(K * E.vel[a], rc2);
+    E.L2[4] = dv(-K, rc2);
 #else
-    double ic2  = p * rho / gamma;  // 1 / c^2
-    double c2   = gamma / (p * rho);
+    double c2   = dv(gamma, p * rho);
+    double ic2  = dv(p * rho, gamma);  // 1 / c^2
     double c    = sqrt(c2);
-    double H    = c2 / K + 0.5 * q2;
+    double H    = dv(c2, K) + 0.5 * q2;
     double K_pc = 0.5 * K * ic2;
     double i2c  = 0.5 * c * ic2;    // 1 / (2c)
     double qn2c = qn * i2c;
@@ -292,6 +337,9 @@ __device__ __forceinline__ void Rmul(const Eig& E, const double ft[5], double ou
 template <int DIR, bool BORGES, int Q, class S>
 __device__ __forceinline__ double char_component(const Eig& E, const S& st) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
+#if NS_ROW_FENCE
+    __asm__ __volatile__("" ::: "memory");  // re-read the stencil per row: fewer live registers
+#endif
     double a[5], b[5];
 #pragma unroll
     for (int ii = 0; ii < 5; ii++) {
@@ -351,11 +399,14 @@ struct LdsSplit {  // split values in LDS, layout [cell][v][lane]
 // ------------------------------------------------------------------------------------------
 constexpr int TILE_S = 8;    // x/y tiles: faces along the sweep per block (one wave each)
 constexpr int TILE_L = 64;   // lanes along k
+#ifndef NS_TILE_WAVES
+#define NS_TILE_WAVES 1      // minimum waves per SIMD requested for the tile kernels
+#endif
 
 // x or y sweep: lanes along k (unit stride), 8 waves at 8 consecutive sweep positions.  The split
 // values of the 13 cells the tile needs are computed once into LDS.
 template <int DIR, int CONV, bool BORGES>
-__global__ __launch_bounds__(512) void k_flux_tile_xy(const double* __restrict__ U,
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
                                                       const double* __restrict__ W,
                                                       double* __restrict__ f, Geo g,
                                                       double gamma) {
@@ -399,7 +450,7 @@ __global__ __launch_bounds__(512) void k_flux_tile_xy(const double* __restrict__
 // z sweep: lanes along k = the sweep (64 faces per row segment), 8 rows (j) per block; each row
 // stages the split values of its 69 cells in LDS.
 template <int CONV, bool BORGES>
-__global__ __launch_bounds__(512) void k_flux_tile_z(const double* __restrict__ U,
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      double* __restrict__ f, Geo g,
                                                      double gamma) {
@@ -477,9 +528,10 @@ __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U
 // ------------------------------------------------------------------------------------------
 __device__ __forceinline__ void c2p(const double u[5], double gamma, double w[5]) {
     w[0]       = u[0];
-    w[2]       = u[2] / u[0];
-    w[3]       = u[3] / u[0];
-    w[4]       = u[4] / u[0];
+    Rcp r0     = rcp(u[0]);
+    w[2]       = dv(u[2], r0);
+    w[3]       = dv(u[3], r0);
+    w[4]       = dv(u[4], r0);
     double kin = 0.5 * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
     w[1]       = (gamma - 1.0) * (u[1] - u[0] * kin);
 }
@@ -490,18 +542,18 @@ __device__ __forceinline__ void p2c(const double w[5], double gamma, double u[5]
     u[3]       = w[0] * w[3];
     u[4]       = w[0] * w[4];
     double kin = 0.5 * w[0] * (w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
-    u[1]       = w[1] / (gamma - 1.0) + kin;
+    u[1]       = dv(w[1], gamma - 1.0) + kin;
 }
 
 __device__ __forceinline__ double local_dt(const double w[5], double gamma, double dx, double dy,
                                            double dz) {
-    double c  = sqrt(gamma * w[1] / w[0]);
+    double c  = sqrt(dv(gamma * w[1], w[0]));
     double ax = fabs(w[2] + c), bx = fabs(w[2] - c);
     double ay = fabs(w[3] + c), by = fabs(w[3] - c);
     double az = fabs(w[4] + c), bz = fabs(w[4] - c);
-    double ex  = dx / ((ax < bx) ? bx : ax);  // std::max
-    double ey  = dy / ((ay < by) ? by : ay);
-    double ez  = dz / ((az < bz) ? bz : az);
+    double ex  = dv(dx, (ax < bx) ? bx : ax);  // std::max
+    double ey  = dv(dy, (ay < by) ? by : ay);
+    double ez  = dv(dz, (az < bz) ? bz : az);
     double tmp = (ey < ex) ? ey : ex;  // std::min
     return (ez < tmp) ? ez : tmp;
 }
@@ -562,8 +614,8 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
         c2p(u, gamma, w2);
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            Ua[c + q * g.P]  = u[q];
-            Un[c + q * g.P]  = u[q];
+            if (Ua) Ua[c + q * g.P] = u[q];
+            Un[c + q * g.P] = u[q];
             Wst[c + q * g.P] = w2[q];
         }
     }
@@ -577,21 +629,29 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
 // copy applied to the primitive planes gives c2p(U) on the halo bit for bit (c2p is per cell).
 // ------------------------------------------------------------------------------------------
 __global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
-    int t2   = threadIdx.x + blockIdx.x * blockDim.x;  // fastest tangential index
-    int t1   = threadIdx.y + blockIdx.y * blockDim.y;
-    int face = blockIdx.z / NGC;
-    int h    = blockIdx.z % NGC;
+    int face = blockIdx.z;
     int axis = face >> 1;
-    int n1, n2;  // tangential extents
-    if (axis == 0) n1 = g.ny, n2 = g.nz;
-    else if (axis == 1) n1 = g.nx, n2 = g.nz;
-    else n1 = g.nx, n2 = g.ny;
-    if (t1 >= n1 || t2 >= n2) return;
-    int  n   = (axis == 0) ? g.nx : (axis == 1) ? g.ny : g.nz;
     // face order W,E,N,S,F,B: low side for W (0), N (2), B (5); high side for E, S, F
     bool low = (face == 0 || face == 2 || face == 5);
-    int  dst = low ? h : NGC + n + h;
-    int  src;
+    int  n   = (axis == 0) ? g.nx : (axis == 1) ? g.ny : g.nz;
+    int  t1, t2, h;
+    if (axis < 2) {
+        // x / y faces: lanes along k (unit stride), one layer per blockIdx.y slice
+        t2 = threadIdx.x + blockIdx.x * blockDim.x;
+        t1 = threadIdx.y + (blockIdx.y / NGC) * blockDim.y;
+        h  = blockIdx.y % NGC;
+        int n1 = (axis == 0) ? g.ny : g.nx;
+        if (t1 >= n1 || t2 >= g.nz) return;
+    } else {
+        // z faces: the 3 halo cells of a row are contiguous; lanes over (row j, layer h)
+        int lin = threadIdx.x + blockIdx.x * blockDim.x;  // j * NGC + h
+        t2      = lin / NGC;
+        h       = lin % NGC;
+        t1      = threadIdx.y + (blockIdx.y / NGC) * blockDim.y;
+        if ((blockIdx.y % NGC) != 0 || t1 >= g.nx || t2 >= g.ny) return;
+    }
+    int dst = low ? h : NGC + n + h;
+    int src;
     if (b.periodic[axis]) src = low ? n + h : NGC + h;
     else src = low ? NGC : NGC + n - 1;
     long ct, cd, cs;
@@ -632,26 +692,52 @@ __global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
 }
 
 // ------------------------------------------------------------------------------------------
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8 XCDs of an MI355X and
+// each XCD has its own L2; renumbering so that XCD x runs a contiguous range of tiles (planes i
+// slowest) keeps the x-neighbour planes of a stencil in the L2 of the XCD that reads them.
+// Returns (b2, b1, b0) tile coordinates for grid (n2, n1, n0), b0 slowest.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void xcd_tile(int& b2, int& b1, int& b0) {
+#if NS_XCD_MAP
+    const unsigned n2 = gridDim.x, n1 = gridDim.y, n0 = gridDim.z;
+    const unsigned N  = n2 * n1 * n0;
+    const unsigned b  = blockIdx.x + n2 * (blockIdx.y + n1 * blockIdx.z);
+    const unsigned q = N / 8, r = N % 8, x = b % 8, l = b / 8;
+    const unsigned t = x * q + (x < r ? x : r) + l;  // bijection of [0, N)
+    b2 = (int)(t % n2);
+    b1 = (int)((t / n2) % n1);
+    b0 = (int)(t / (n2 * n1));
+#else
+    b2 = blockIdx.x, b1 = blockIdx.y, b0 = blockIdx.z;
+#endif
+}
+
+// ------------------------------------------------------------------------------------------
 // viscous terms (centered_diffusion.cpp:3-182, cd-n.cpp:91-165, diffusion_scheme.cpp:3-58)
 // ------------------------------------------------------------------------------------------
 
-// one stencil sum along `stride` of the first- (SECOND=0) or second-derivative coefficients:
-// EXACT: sum of f(. + s + k) * c_k / d in k order, the zero centre term of the first derivative
-// skipped (an exact +-0 addend); FAST: sum of f * (c_k / d) with the reciprocal of d hoisted.
-template <int ORDER, bool SECOND>
-__device__ __forceinline__ double fd_sum(const double* f, long stride, double d, double id) {
+// one stencil sum of the first- (SECOND=0) or second-derivative coefficients; f(t) returns the
+// value at offset t - H, t = 0 .. 2H.  EXACT: sum of f * c_k / d in k order, the zero centre term
+// of the first derivative skipped (an exact +-0 addend); FAST: sum of f * (c_k / d) with the
+// reciprocal of d hoisted.
+template <int ORDER, bool SECOND, class F>
+__device__ __forceinline__ double fd_sum(const F& f, double d, double id) {
     constexpr int H     = ORDER / 2;
     constexpr int OI    = ORDER / 2 - 1;
     double        s     = 0.;
     bool          first = true;
+#if !NS_FAST
+    const Rcp rd = rcp(d);
+#endif
 #pragma unroll
     for (int t = 0; t <= 2 * H; t++) {
         if (!SECOND && t == H) continue;
         double co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
 #if !NS_FAST
-        double v = f[(long)(t - H) * stride] * co / d;
+        double v = dv(f(t) * co, rd);
+        (void)id;
 #else
-        double v = f[(long)(t - H) * stride] * (co * id);
+        double v = f(t) * (co * id);
         (void)d;
 #endif
         if (first) {
@@ -660,27 +746,46 @@ __device__ __forceinline__ double fd_sum(const double* f, long stride, double d,
             s += v;
         }
     }
-#if !NS_FAST
-    (void)id;
-#endif
     return s;
 }
+
+// Kernels with x-direction stencils march along i: each thread owns one (j, k) column of a
+// MARCH-plane chunk and keeps the x-window of the values it needs in registers, so every plane is
+// fetched from HBM once instead of once per stencil point.
+constexpr int MARCH = 16;
 
 // first derivatives of rho, p, u, v along x and y on the interior
 template <int ORDER>
 __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
                                                 Geo g) {
-    int k = threadIdx.x + blockIdx.x * blockDim.x;
-    int j = threadIdx.y + blockIdx.y * blockDim.y;
-    int i = blockIdx.z;
+    constexpr int H  = ORDER / 2;
+    const int     k  = threadIdx.x + blockIdx.x * blockDim.x;
+    const int     j  = threadIdx.y + blockIdx.y * blockDim.y;
+    const int     i0 = blockIdx.z * MARCH;
     if (k >= g.nz || j >= g.ny) return;
-    const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+    const int  i1 = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const long c0 = (long)(j + NGC) * g.sy + (k + NGC);
+    double     win[4][2 * H + 1];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const double* f = W + q * g.P + c;
-        Dx[c + q * g.P] = fd_sum<ORDER, false>(f, g.sx, g.dx, g.idx);
-        Dy[c + q * g.P] = fd_sum<ORDER, false>(f, g.sy, g.dy, g.idy);
+    for (int t = 0; t < 2 * H; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) win[q][t] = W[q * g.P + c0 + (long)(i0 + NGC - H + t) * g.sx];
+    for (int i = i0; i < i1; i++) {
+        const long c = c0 + (long)(i + NGC) * g.sx;
+#pragma unroll
+        for (int q = 0; q < 4; q++) win[q][2 * H] = W[q * g.P + c + (long)H * g.sx];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double* fy = W + q * g.P + c;
+            Dx[c + q * g.P] = fd_sum<ORDER, false>([&](int t) { return win[q][t]; }, g.dx, g.idx);
+            Dy[c + q * g.P] = fd_sum<ORDER, false>(
+                [&](int t) { return fy[(long)(t - H) * g.sy]; }, g.dy, g.idy);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int t = 0; t < 2 * H; t++) win[q][t] = win[q][t + 1];
     }
 }
 
@@ -713,38 +818,46 @@ __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
 
 // CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
 // constant-viscosity zero terms (dmu = dlamb = 0) dropped: each is an exact +-0 addend.
+// wx[q][t]: primitives rho, p, u, v at x offsets t - H; dyx[a][t]: dW/dy of u (a = 0), v (a = 1)
+// at x offsets t - H; Dx, Dy: derivative planes at the cell.
 template <int ORDER, bool SQ>
-__device__ __forceinline__ void viscous(const double* __restrict__ W,
+__device__ __forceinline__ void viscous(const double (&wx)[4][ORDER + 1],
+                                        const double (&dyx)[2][ORDER + 1],
                                         const double* __restrict__ Dx,
                                         const double* __restrict__ Dy, long c, const Geo& g,
                                         const Phys& ph, double& ene, double& x_mom,
                                         double& y_mom) {
-    const long P   = g.P;
-    const int  rho = 0, p = 1, u = 2, v = 3;
+    constexpr int H   = ORDER / 2;
+    const long    P   = g.P;
+    const int     rho = 0, p = 1, u = 2, v = 3;
     // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79); when
     // dx*dx == dy*dy the two sums are the same operations on the same operands
-    double du_dxdx   = fd_sum<ORDER, true>(W + u * P + c, g.sx, g.dxx, g.idxx);
-    double dv_dxdx   = fd_sum<ORDER, true>(W + v * P + c, g.sx, g.dxx, g.idxx);
-    double drho_dxdx = fd_sum<ORDER, true>(W + rho * P + c, g.sx, g.dxx, g.idxx);
-    double dp_dxdx   = fd_sum<ORDER, true>(W + p * P + c, g.sx, g.dxx, g.idxx);
+    double du_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[u][t]; }, g.dxx, g.idxx);
+    double dv_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[v][t]; }, g.dxx, g.idxx);
+    double drho_dxdx = fd_sum<ORDER, true>([&](int t) { return wx[rho][t]; }, g.dxx, g.idxx);
+    double dp_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[p][t]; }, g.dxx, g.idxx);
     double du_dydy, dv_dydy, drho_dydy, dp_dydy;
     if constexpr (SQ) {
         du_dydy = du_dxdx, dv_dydy = dv_dxdx, drho_dydy = drho_dxdx, dp_dydy = dp_dxdx;
     } else {
-        du_dydy   = fd_sum<ORDER, true>(W + u * P + c, g.sx, g.dyy, g.idyy);
-        dv_dydy   = fd_sum<ORDER, true>(W + v * P + c, g.sx, g.dyy, g.idyy);
-        drho_dydy = fd_sum<ORDER, true>(W + rho * P + c, g.sx, g.dyy, g.idyy);
-        dp_dydy   = fd_sum<ORDER, true>(W + p * P + c, g.sx, g.dyy, g.idyy);
+        du_dydy   = fd_sum<ORDER, true>([&](int t) { return wx[u][t]; }, g.dyy, g.idyy);
+        dv_dydy   = fd_sum<ORDER, true>([&](int t) { return wx[v][t]; }, g.dyy, g.idyy);
+        drho_dydy = fd_sum<ORDER, true>([&](int t) { return wx[rho][t]; }, g.dyy, g.idyy);
+        dp_dydy   = fd_sum<ORDER, true>([&](int t) { return wx[p][t]; }, g.dyy, g.idyy);
     }
     // cross derivatives: dW_dxdy = df_dy(dW_dx), dW_dydx = df_dx(dW_dy)
-    double du_dxdy = fd_sum<ORDER, false>(Dx + u * P + c, g.sy, g.dy, g.idy);
-    double dv_dxdy = fd_sum<ORDER, false>(Dx + v * P + c, g.sy, g.dy, g.idy);
-    double du_dydx = fd_sum<ORDER, false>(Dy + u * P + c, g.sx, g.dx, g.idx);
-    double dv_dydx = fd_sum<ORDER, false>(Dy + v * P + c, g.sx, g.dx, g.idx);
+    const double* Dxu     = Dx + u * P + c;
+    const double* Dxv     = Dx + v * P + c;
+    double        du_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxu[(long)(t - H) * g.sy]; },
+                                          g.dy, g.idy);
+    double dv_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxv[(long)(t - H) * g.sy]; }, g.dy,
+                                          g.idy);
+    double du_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[0][t]; }, g.dx, g.idx);
+    double dv_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[1][t]; }, g.dx, g.idx);
 
-    double du_dx = Dx[u * P + c], dv_dx = Dx[v * P + c], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
-    double du_dy = Dy[u * P + c], dv_dy = Dy[v * P + c], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
-    double U = W[u * P + c], V = W[v * P + c], Pp = W[p * P + c], RHO = W[rho * P + c];
+    double du_dx = Dxu[0], dv_dx = Dxv[0], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
+    double du_dy = dyx[0][H], dv_dy = dyx[1][H], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
+    double U = wx[u][H], V = wx[v][H], Pp = wx[p][H], RHO = wx[rho][H];
 
     double mu        = ph.mu;
     double lamb      = ph.lamb;
@@ -759,11 +872,11 @@ __device__ __forceinline__ void viscous(const double* __restrict__ W,
     double Dtauvx_dx = U * Dtauxx_dx + V * Dtauxy_dx + Tauxx * du_dx + Tauxy * dv_dy;
     double Dtauvy_dy = U * Dtauyx_dy + V * Dtauyy_dy + Tauyy * dv_dy + Tauyx * du_dy;
 #if !NS_FAST
-    double one_rho  = 1. / RHO;
-    double one_rho2 = 1. / (RHO * RHO);
-    double one_rho3 = 1. / (RHO * RHO * RHO);
+    double one_rho  = dv(1., RHO);
+    double one_rho2 = dv(1., RHO * RHO);
+    double one_rho3 = dv(1., RHO * RHO * RHO);
 #else
-    double one_rho  = 1. / RHO;
+    double one_rho  = dv(1., RHO);
     double one_rho2 = one_rho * one_rho;
     double one_rho3 = one_rho2 * one_rho;
 #endif
@@ -779,92 +892,131 @@ __device__ __forceinline__ void viscous(const double* __restrict__ W,
     ene           = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
 }
 
-// residual + stage update at one interior cell:
+// residual + stage update, marching along i:
 // R = R_in + (fx_i - fx_{i-1})/dx + (fy_j - fy_{j-1})/dy + (fz_k - fz_{k-1})/dz
 // (Weno5::compute_residual, weno-5.cpp:39-73), then the viscous terms (centered_diffusion.cpp:
 // 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96)
-// and the primitives of the new state (eq_of_state.cpp:61-87).
+// and the primitives of the new state (eq_of_state.cpp:61-87).  Without a stage (st.Uout null)
+// R is stored instead (operator-level calls).
 template <int ORDER, bool SQ>
 __global__ __launch_bounds__(256) void k_residual(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
     const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
     const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
-    int k = threadIdx.x + blockIdx.x * blockDim.x;
-    int j = threadIdx.y + blockIdx.y * blockDim.y;
-    int i = blockIdx.z;
+    constexpr int O  = (ORDER > 0) ? ORDER : 2;  // window size - 1 (unused without diffusion)
+    constexpr int H  = O / 2;
+    const int     k  = threadIdx.x + blockIdx.x * blockDim.x;
+    const int     j  = threadIdx.y + blockIdx.y * blockDim.y;
+    const int     i0 = blockIdx.z * MARCH;
     if (k >= g.nz || j >= g.ny) return;
-    const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
-    const long P = g.P;
-    double     R[5];
-    bool       have[5];
+    const int  i1   = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const long P    = g.P;
+    const long c0   = (long)(j + NGC) * g.sy + (k + NGC);
+    const bool conv = (which & 1) && ph.conv;
+    const bool visc = (ORDER > 0) && (which & 2);
+    double     wx[4][O + 1], dyx[2][O + 1], fxm[5];
+    if (visc) {
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        have[q] = (Rin != nullptr);
-        R[q]    = Rin ? Rin[c + q * P] : 0.0;
+        for (int t = 0; t < 2 * H; t++) {
+            const long ct = c0 + (long)(i0 + NGC - H + t) * g.sx;
+#pragma unroll
+            for (int q = 0; q < 4; q++) wx[q][t] = W[q * P + ct];
+            dyx[0][t] = Dy[2 * P + ct];
+            dyx[1][t] = Dy[3 * P + ct];
+        }
     }
-    if ((which & 1) && ph.conv) {
+    if (conv) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) fxm[q] = fx[q * P + c0 + (long)(i0 + NGC - 1) * g.sx];
+    }
+    double dt = 0, m1 = 0, m2 = 0, m3 = 0;
+    if (st.Uout) {
+        // SSP-RK3 stage (time_scheme.cpp:37-63): dt = CFL * min local dt (time_scheme.cpp:129)
+        dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
+        if (st.stage == 0) {
+            m1 = 1.0, m2 = 0.0, m3 = -dt;
+        } else if (st.stage == 1) {
+            m1 = 3.0 / 4.0, m2 = 1.0 / 4.0, m3 = -(1. / 4.) * dt;
+        } else {
+            m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
+        }
+    }
+    for (int i = i0; i < i1; i++) {
+        const long c = c0 + (long)(i + NGC) * g.sx;
+        double     R[5];
+        bool       have[5];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
+            have[q] = (Rin != nullptr);
+            R[q]    = Rin ? Rin[c + q * P] : 0.0;
+        }
+        if (conv) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                double fc = fx[c + q * P];
 #if !NS_FAST
-            double a = (fx[c + q * P] - fx[c - g.sx + q * P]) / g.dx;
-            double b = (fy[c + q * P] - fy[c - g.sy + q * P]) / g.dy;
+                double a = dv(fc - fxm[q], g.dx);
+                double b = dv(fy[c + q * P] - fy[c - g.sy + q * P], g.dy);
 #else
-            double a = (fx[c + q * P] - fx[c - g.sx + q * P]) * g.idx;
-            double b = (fy[c + q * P] - fy[c - g.sy + q * P]) * g.idy;
+                double a = (fc - fxm[q]) * g.idx;
+                double b = (fy[c + q * P] - fy[c - g.sy + q * P]) * g.idy;
 #endif
-            R[q] = have[q] ? R[q] + a : a;
-            R[q] += b;
-            if (g.dim == 3) {
+                fxm[q] = fc;
+                R[q]   = have[q] ? R[q] + a : a;
+                R[q] += b;
+                if (g.dim == 3) {
 #if !NS_FAST
-                R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) / g.dz;
+                    R[q] += dv(fz[c + q * P] - fz[c - 1 + q * P], g.dz);
 #else
-                R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) * g.idz;
+                    R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) * g.idz;
 #endif
+                }
+                have[q] = true;
             }
-            have[q] = true;
         }
-    }
-    if constexpr (ORDER > 0) {
-        if (which & 2) {
-            double ene, xm, ym;
-            viscous<ORDER, SQ>(W, Dx, Dy, c, g, ph, ene, xm, ym);
-            R[1]    = have[1] ? R[1] + ene : ene;
-            R[2]    = have[2] ? R[2] + xm : xm;
-            R[3]    = have[3] ? R[3] + ym : ym;
-            have[1] = have[2] = have[3] = true;
+        if constexpr (ORDER > 0) {
+            if (visc) {
+                const long cn = c + (long)H * g.sx;  // plane entering the window
+#pragma unroll
+                for (int q = 0; q < 4; q++) wx[q][2 * H] = W[q * P + cn];
+                dyx[0][2 * H] = Dy[2 * P + cn];
+                dyx[1][2 * H] = Dy[3 * P + cn];
+                double ene, xm, ym;
+                viscous<ORDER, SQ>(wx, dyx, Dx, Dy, c, g, ph, ene, xm, ym);
+                R[1]    = have[1] ? R[1] + ene : ene;
+                R[2]    = have[2] ? R[2] + xm : xm;
+                R[3]    = have[3] ? R[3] + ym : ym;
+                have[1] = have[2] = have[3] = true;
+#pragma unroll
+                for (int t = 0; t < 2 * H; t++) {
+#pragma unroll
+                    for (int q = 0; q < 4; q++) wx[q][t] = wx[q][t + 1];
+                    dyx[0][t] = dyx[0][t + 1];
+                    dyx[1][t] = dyx[1][t + 1];
+                }
+            }
         }
-    }
-    if (st.Uout == nullptr) {
+        if (st.Uout == nullptr) {
 #pragma unroll
-        for (int q = 0; q < 5; q++) Rout[c + q * P] = have[q] ? R[q] : 0.0;
-        return;
-    }
-    // SSP-RK3 stage (time_scheme.cpp:37-63): dt = CFL * min local dt (time_scheme.cpp:129)
-    double dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
-    double m1, m2, m3;
-    if (st.stage == 0) {
-        m1 = 1.0, m2 = 0.0, m3 = -dt;
-    } else if (st.stage == 1) {
-        m1 = 3.0 / 4.0, m2 = 1.0 / 4.0, m3 = -(1. / 4.) * dt;
-    } else {
-        m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
-    }
-    double un[5], w[5];
+            for (int q = 0; q < 5; q++) Rout[c + q * P] = have[q] ? R[q] : 0.0;
+            continue;
+        }
+        double un[5], w[5];
 #pragma unroll
-    for (int q = 0; q < 5; q++) {
-        double r = have[q] ? R[q] : 0.0;
+        for (int q = 0; q < 5; q++) {
+            double r = have[q] ? R[q] : 0.0;
 #if !NS_FAST
-        un[q] = (st.Un[c + q * P] * m1) + (st.Ucur[c + q * P] * m2) + (r * m3);
+            un[q] = (st.Un[c + q * P] * m1) + (st.Ucur[c + q * P] * m2) + (r * m3);
 #else
-        // explicit order-preserving form; FMA contraction is allowed in this mode
-        un[q] = st.Un[c + q * P] * m1 + st.Ucur[c + q * P] * m2 + r * m3;
+            un[q] = st.Un[c + q * P] * m1 + st.Ucur[c + q * P] * m2 + r * m3;
 #endif
-        st.Uout[c + q * P] = un[q];
-    }
-    c2p(un, ph.gamma, w);
+            st.Uout[c + q * P] = un[q];
+        }
+        c2p(un, ph.gamma, w);
 #pragma unroll
-    for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
+        for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
+    }
 }
 
 __global__ void k_time_accum(double* t, const unsigned long long* dtmin, double cfl) {
@@ -957,6 +1109,38 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __re
 }
 
 // ------------------------------------------------------------------------------------------
+// self-test of dv() against the IEEE division operator on random operands: sign, exponent in
+// [-300, 300] and mantissa drawn from a counter-based generator (splitmix64 of seed + index)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ double random_double(unsigned long long r) {
+    unsigned long long mant = r & ((1ull << 52) - 1);
+    unsigned long long e    = 1023ull - 300ull + ((r >> 52) % 601ull);
+    unsigned long long sign = (r >> 63) & 1ull;
+    return __longlong_as_double((long long)((sign << 63) | (e << 52) | mant));
+}
+
+__global__ void k_selftest_div(long n, unsigned long long seed, unsigned long long* count) {
+    unsigned long long bad = 0;
+    for (long t = threadIdx.x + (long)blockIdx.x * blockDim.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        double a = random_double(splitmix64(seed + 2 * (unsigned long long)t));
+        double b = random_double(splitmix64(seed + 2 * (unsigned long long)t + 1));
+        if (t % 97 == 0) a = 0.0;                     // exact-zero numerators
+        if (t % 89 == 0) b = (double)((t % 13) + 1);  // small integers (grid-like divisors)
+        double x = dv(a, b), y = a / b;
+        if (!(x == y)) bad++;
+    }
+    if (bad) atomicAdd(count, bad);
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 static dim3 cell_grid(int n2, int n1, int n0, dim3 blk) {
@@ -978,10 +1162,11 @@ static void l_step_begin(const double* W, double* Ua, double* Un, double* Wst,
 }
 
 static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s) {
+    // x, y faces: (k, tangential row, layer); z faces: ((j, layer), i)
     int  m1 = g.ny > g.nx ? g.ny : g.nx;
-    int  m2 = g.nz > g.ny ? g.nz : g.ny;
+    int  mx = g.nz > NGC * g.ny ? g.nz : NGC * g.ny;
     dim3 blk(64, 4, 1);
-    dim3 grd((m2 + 63) / 64, (m1 + 3) / 4, 6 * NGC);
+    dim3 grd((mx + 63) / 64, NGC * ((m1 + 3) / 4), 6);
     k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b);
 }
 
@@ -1031,7 +1216,7 @@ static void l_face_flux(int dir, const double* U, const double* W, double* f, Ge
 }
 
 static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {
-    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
+    dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
     if (ph.diff_order == 2) k_deriv1<2><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
     else if (ph.diff_order == 4) k_deriv1<4><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
     else k_deriv1<6><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
@@ -1049,7 +1234,7 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
                            const double* Dx, const double* Dy, const double* Rin, double* Rout,
                            StageArgs a, const unsigned long long* dtmin, int which, Geo g,
                            Phys ph, hipStream_t s) {
-    dim3 grd = cell_grid(g.nz, g.ny, g.nx, BLK);
+    dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
     if (g.square_xy)
         k_residual<ORDER, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
     else
@@ -1095,11 +1280,16 @@ static void l_time_accum(double* t, const unsigned long long* dtmin, double cfl,
     k_time_accum<<<1, 1, 0, s>>>(t, dtmin, cfl);
 }
 
+static void l_selftest_div(long n, unsigned long long seed, unsigned long long* count,
+                           hipStream_t s) {
+    k_selftest_div<<<1024, 256, 0, s>>>(n, seed, count);
+}
+
 const Launchers& launchers() {
     static const Launchers L = {l_reset_min,    l_step_begin, l_halo,       l_c2p_faces,
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
-                                l_time_accum,   l_zero_flags};
+                                l_time_accum,   l_zero_flags, l_selftest_div};
     return L;
 }
 

@@ -366,10 +366,11 @@ int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
         {
             Scope sc(c, T_STEP_BEGIN);
             L.reset_min(&c->dtmin[0], c->stream);
-            L.step_begin(c->W, c->Ua, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
+            L.step_begin(c->W, nullptr, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
         }
-        double* cur  = c->Ua;
-        double* nxt  = c->Ub;
+        // stage inputs Un -> Ua -> Ub: Un (Uprev of time_scheme.cpp:27) is never overwritten
+        double* cur  = c->Un;
+        double* nxt  = c->Ua;
         double* wcur = c->Ws[0];
         double* wnxt = c->Ws[1];
         for (int st = 0; st < 3; st++) {
@@ -390,7 +391,7 @@ int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
             }
             double* t = cur;
             cur       = nxt;
-            nxt       = t;
+            nxt       = (st == 0) ? c->Ub : t;
             t         = wcur;
             wcur      = wnxt;
             wnxt      = t;
@@ -449,6 +450,24 @@ int nsgpu_reduce(nsgpu_ctx* c, double sums[5], double* dt_min, unsigned* flags) 
     if (fl & 2u) return NSGPU_ERR_POSITIVITY;
     if (fl & 4u) return NSGPU_ERR_ALPHA;
     return NSGPU_OK;
+}
+
+int nsgpu_selftest(int device, long n, unsigned long long seed, long* mismatches) {
+    if (n < 0 || !mismatches) return NSGPU_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NSGPU_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return NSGPU_ERR_NODEVICE;
+    HIPCHK(hipSetDevice(device));
+    unsigned long long* d = nullptr;
+    HIPCHK(hipMalloc((void**)&d, sizeof(*d)));
+    unsigned long long h = 0;
+    int                rc = NSGPU_OK;
+    if (hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc) exact::launchers().selftest_div(n, seed, d, nullptr);
+    if (!rc && hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) rc = NSGPU_ERR_HIP;
+    hipFree(d);
+    *mismatches = (long)h;
+    return rc;
 }
 
 int nsgpu_timing(nsgpu_ctx* c, int enable) {

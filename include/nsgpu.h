@@ -116,6 +116,12 @@ int nsgpu_get_dt(nsgpu_ctx* ctx, double* dt);
  * (bit 0 non-finite, bit 1 rho<=0 or p<=0, bit 2 max|u|+c > 3). */
 int nsgpu_reduce(nsgpu_ctx* ctx, double sums[5], double* dt_min, unsigned* flags);
 
+/* No reference counterpart: checks on GPU `device` the device arithmetic the EXACT path relies on.
+ * Divides n pairs of random operands (exponents in [-300, 300]) with the kernels' division
+ * routine and with the IEEE operator and stores the number of differing quotients.  For callers
+ * that validate a new GPU, driver or compiler before trusting bitwise results. */
+int nsgpu_selftest(int device, long n, unsigned long long seed, long* mismatches);
+
 /* The context's HIP stream (a hipStream_t), for callers that time or order work around it. */
 void* nsgpu_stream(nsgpu_ctx* ctx);
 

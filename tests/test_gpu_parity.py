@@ -238,3 +238,9 @@ def test_full_size_256_rhs_windows(ns, oracle):
     case, W = C.config2(256)
     print("[gpu] 256^3 window parity (exact)", flush=True)
     _window_check(ns, oracle, case, W, "exact", 0)
+
+
+def test_division_routine_matches_ieee(ns):
+    # the kernels' division (rcp + Newton + remainder correction, no range scaling) must return
+    # the IEEE quotient on every tested operand pair (exponents in [-300, 300])
+    assert ns.selftest(n=1 << 26, seed=12345) == 0
