@@ -25,8 +25,10 @@ LIB_PATH = os.path.join(HERE, "lib", "libnsgpu.so")
 BC = {"periodic": 0, "neumann": 1}
 CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
 WEIGHTS = {"shu": 0, "borges": 1}
-MATH = {"exact": 0, "fast": 1}
+MATH = {"exact": 0, "fast": 1, "fp32": 2}
 STATE = {"conservative": 0, "primitive": 1}
+VISCOSITY = {"constant": 0, "sutherland": 1}
+SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 
 EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len", "nsgpu_set_state",
            "nsgpu_get_state", "nsgpu_apply_boundary", "nsgpu_convection_solve",
@@ -44,6 +46,7 @@ class Config(ctypes.Structure):
         ("cfl", ctypes.c_double),
         ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
         ("bc", ctypes.c_int * 6), ("math", ctypes.c_int),
+        ("viscosity", ctypes.c_int), ("suth_C", ctypes.c_double),
     ]
 
 
@@ -112,6 +115,8 @@ def make_config(case, math="exact"):
     for f, name in enumerate(case["bc"]):
         c.bc[f] = BC[name]
     c.math = MATH[math]
+    c.viscosity = VISCOSITY[case.get("viscosity", "constant")]
+    c.suth_C = case.get("suth_C", SUTH_C)
     return c
 
 

@@ -52,7 +52,8 @@ inline nsgpu_config make_config(const Params& p, const Array4d& like, int math) 
     for (int f = 0; f < 6; f++)
         c.bc[f] = (p.boundary_conditions.at(names[f]) == "Periodic") ? NSGPU_BC_PERIODIC
                                                                       : NSGPU_BC_NEUMANN;
-    c.math = math;
+    c.math      = math;
+    c.viscosity = NSGPU_VISC_CONSTANT;  // the reference's CenteredDiffusion uses a constant mu
     return c;
 }
 

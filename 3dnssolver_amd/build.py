@@ -1,7 +1,7 @@
 """Builds the HIP C-ABI library lib/libnsgpu.so for gfx950 (MI355X) with hipcc.
 
-ns_kernels.hip is compiled twice: once with -ffp-contract=off (EXACT: the reference's operation
-order, bitwise) and once with -ffp-contract=fast (FAST).  Every HIP compile disables packed FP32
+ns_kernels.hip is compiled three times: with -ffp-contract=off (EXACT: the reference's operation
+order, bitwise), with -ffp-contract=fast (FAST) and with single-precision reconstruction (FP32).  Every HIP compile disables packed FP32
 (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
 import os
@@ -34,6 +34,8 @@ def build(verbose=False, defines=(), out=None):
     units = [
         ("ns_kernels.hip", "kernels_exact.o", ["-DNS_FAST=0", "-ffp-contract=off", *defines]),
         ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=fast", *defines]),
+        ("ns_kernels.hip", "kernels_fp32.o", ["-DNS_FAST=1", "-DNS_FP32=1", "-ffp-contract=fast",
+                                              *defines]),
         ("nsgpu_capi.hip", "capi.o", ["-ffp-contract=off"]),
     ]
     for src, obj, extra in units:

@@ -124,3 +124,56 @@ def config2(n=256):
     case = dict(base_case(n, n, n), L=(2 * math.pi,) * 3, conv="weno5cw", weights="shu",
                 diff_order=4, mu=1.0 * M * c0 * 1.0 / Re)
     return case, tgv_W(case, M=M, seed=2)
+
+
+def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None):
+    """Decaying isotropic turbulence on [0, 2pi]^3: seeded random-phase solenoidal velocity with
+    energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), uniform density and pressure, rms velocity
+    Mt * c0.  Returns (W, u_prime, taylor_lambda) with u' the rms of one component and
+    lambda = u' / rms(du/dx) (spectral derivative)."""
+    n = case["nx"]
+    assert case["ny"] == n and case["nz"] == n
+    rng = np.random.default_rng(seed)
+    k1 = np.fft.fftfreq(n, 1.0 / n)
+    kr = np.fft.rfftfreq(n, 1.0 / n)
+    KX, KY, KZ = np.meshgrid(k1, k1, kr, indexing="ij")
+    K2 = KX ** 2 + KY ** 2 + KZ ** 2
+    K = np.sqrt(K2)
+    K2[0, 0, 0] = 1.0
+    amp = np.sqrt(K ** 4 * np.exp(-2.0 * (K / k0) ** 2) / (4.0 * np.pi * K2))
+    amp[0, 0, 0] = 0.0
+    comps = []
+    for _ in range(3):
+        ph = rng.random(K.shape) * 2.0 * np.pi
+        comps.append(amp * np.exp(1j * ph) * rng.standard_normal(K.shape))
+    # solenoidal projection: u_hat -= k (k . u_hat) / k^2
+    dot = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / K2
+    comps = [comps[0] - KX * dot, comps[1] - KY * dot, comps[2] - KZ * dot]
+    vel = [np.fft.irfftn(cc, s=(n, n, n)) for cc in comps]
+    urms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
+    if c0 is None:
+        # c0 with max |u_i| + c0 = alpha = 3 (the LF split constant)
+        vmax = max(np.abs(v).max() for v in vel) / urms
+        c0 = ALPHA / (1.0 + Mt * vmax)
+    scale = Mt * c0 / urms
+    vel = [v * scale for v in vel]
+    dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(vel[0]), s=(n, n, n))
+    uprime = np.sqrt(np.mean(vel[0] ** 2))
+    lam = uprime / np.sqrt(np.mean(dudx ** 2))
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    W[I + (0,)] = 1.0
+    W[I + (1,)] = c0 * c0 / case["gamma"]
+    for a in range(3):
+        W[I + (2 + a,)] = vel[a]
+    return W, uprime, lam
+
+
+def config3(n=512):
+    """Precision form: decaying isotropic turbulence, n^3 periodic on [0, 2pi]^3, turbulent Mach
+    0.5, Re_lambda = 100, characteristic WENO-5 (Shu) + CD4, CFL 0.5; mu = rho u' lambda / 100."""
+    case = dict(base_case(n, n, n), L=(2 * math.pi,) * 3, conv="weno5cw", weights="shu",
+                diff_order=4)
+    W, uprime, lam = isotropic_W(case, Mt=0.5, k0=4.0, seed=3)
+    case["mu"] = 1.0 * uprime * lam / 100.0
+    return case, W

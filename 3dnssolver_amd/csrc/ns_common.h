@@ -41,6 +41,9 @@ struct Phys {
     int    conv;        // 0 none, 1 component-wise, 2 characteristic
     int    borges;      // 0 Shu, 1 Borges
     int    diff_order;  // 0, 2, 4, 6
+    int    suth;        // Sutherland viscosity
+    double suth_C;      // Sutherland constant (T0 = 1)
+    double suth_k;      // mu * (1.0 + C)
 };
 
 // SSP-RK3 stage (time_scheme.cpp:37-63); the coefficients are formed on the device from dt
@@ -87,6 +90,9 @@ namespace exact {
 const Launchers& launchers();
 }
 namespace fast {
+const Launchers& launchers();
+}
+namespace fp32 {
 const Launchers& launchers();
 }
 

@@ -10,6 +10,10 @@
 //   NS_FAST=1 with -ffp-contract=fast  -> namespace nsgpu::fast
 //       fused multiply-add, reciprocals hoisted out of the stencils, no square root in the flux
 //       split, one division per WENO reconstruction.
+//   NS_FAST=1 NS_FP32=1                 -> namespace nsgpu::fp32
+//       the FAST arithmetic with the reconstruction (flux split, eigen system, projections,
+//       WENO) and the viscous fluxes in single precision; the state, the stage accumulation,
+//       the equation of state and the reductions stay FP64.
 //
 // Citations are paths relative to the reference's src/ directory.
 #include "ns_common.h"
@@ -20,10 +24,18 @@
 #ifndef NS_ROW_FENCE
 #define NS_ROW_FENCE 1
 #endif
+// arithmetic type of the reconstruction and the viscous fluxes: double for the EXACT and FAST
+// builds; float for the FP32 build (state, stage accumulation and reductions stay double)
+#ifndef NS_REAL
+#define NS_REAL double
+#endif
 #ifndef NS_XCD_MAP
 #define NS_XCD_MAP 0
 #endif
-#if NS_FAST
+#if NS_FP32
+#define NS_MODE fp32
+#define NS_REAL float
+#elif NS_FAST
 #define NS_MODE fast
 #else
 #define NS_MODE exact
@@ -32,15 +44,18 @@
 namespace nsgpu {
 namespace NS_MODE {
 
+typedef NS_REAL real;
+#define RL(x) ((real)(x))
+
 // ------------------------------------------------------------------------------------------
 // constants (convection_constants.hpp:4-12, cd_constants.hpp:9-27)
 // ------------------------------------------------------------------------------------------
-#define EPS_SHU 1E-6
-#define EPS_BORGES 1E-14
-#define ONEPERSIX (1. / 6.)
-#define THIRTEENPERTWELVE (13. / 12.)
-#define ONEPERFOUR (1. / 4.)
-#define LF_ALPHA 3.0  // ConvectionFlux::max_eigenvalue stores 3.0 (convection_flux.cpp:241)
+#define EPS_SHU RL(1E-6)
+#define EPS_BORGES RL(1E-14)
+#define ONEPERSIX RL(1. / 6.)
+#define THIRTEENPERTWELVE RL(13. / 12.)
+#define ONEPERFOUR RL(1. / 4.)
+#define LF_ALPHA RL(3.0)  // ConvectionFlux::max_eigenvalue stores 3.0 (convection_flux.cpp:241)
 
 __constant__ double c_fd1[3][7] = {
     {-1. / 2, 0., 1. / 2.},
@@ -86,54 +101,61 @@ __device__ __forceinline__ double dv(double n, const Rcp& R) {
     return __builtin_fma(rem, R.r, q);
 }
 __device__ __forceinline__ double dv(double n, double d) { return dv(n, rcp(d)); }
+// single precision (FP32 build): hardware reciprocal times the numerator, ~1 ulp
+struct RcpF {
+    float d, r;
+};
+__device__ __forceinline__ RcpF  rcp(float d) { return {d, __builtin_amdgcn_rcpf(d)}; }
+__device__ __forceinline__ float dv(float n, const RcpF& R) { return n * R.r; }
+__device__ __forceinline__ float dv(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 
 // ------------------------------------------------------------------------------------------
 // WENO-5 (weno-5.cpp:222-322)
 // ------------------------------------------------------------------------------------------
 template <bool BORGES>
-__device__ __forceinline__ double weno_combine(double a, double b, double c, double d, double e,
-                                               double d0, double d1, double d2, double f0,
-                                               double f1, double f2) {
-    double t1, t2;
-    t1        = a - 2. * b + c;
-    t2        = a - 4. * b + 3. * c;
-    double b0 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
-    t1        = b - 2. * c + d;
+__device__ __forceinline__ real weno_combine(real a, real b, real c, real d, real e,
+                                               real d0, real d1, real d2, real f0,
+                                               real f1, real f2) {
+    real t1, t2;
+    t1        = a - RL(2.) * b + c;
+    t2        = a - RL(4.) * b + RL(3.) * c;
+    real b0 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = b - RL(2.) * c + d;
     t2        = b - d;
-    double b1 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
-    t1        = c - 2. * d + e;
-    t2        = 3. * c - 4. * d + e;
-    double b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    real b1 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    t1        = c - RL(2.) * d + e;
+    t2        = RL(3.) * c - RL(4.) * d + e;
+    real b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
 #if !NS_FAST
-    double a0, a1, a2;
+    real a0, a1, a2;
     if constexpr (BORGES) {
-        double tau5 = fabs(b0 - b2);
-        a0          = d0 * (1.0 + dv(tau5, b0 + EPS_BORGES));
-        a1          = d1 * (1.0 + dv(tau5, b1 + EPS_BORGES));
-        a2          = d2 * (1.0 + dv(tau5, b2 + EPS_BORGES));
+        real tau5 = fabs(b0 - b2);
+        a0          = d0 * (RL(1.0) + dv(tau5, b0 + EPS_BORGES));
+        a1          = d1 * (RL(1.0) + dv(tau5, b1 + EPS_BORGES));
+        a2          = d2 * (RL(1.0) + dv(tau5, b2 + EPS_BORGES));
     } else {
         a0 = dv(d0, (EPS_SHU + b0) * (EPS_SHU + b0));
         a1 = dv(d1, (EPS_SHU + b1) * (EPS_SHU + b1));
         a2 = dv(d2, (EPS_SHU + b2) * (EPS_SHU + b2));
     }
-    double al = a0 + a1 + a2;
-    Rcp    ra = rcp(al);
+    real al = a0 + a1 + a2;
+    auto   ra = rcp(al);
     return dv(a0, ra) * f0 + dv(a1, ra) * f1 + dv(a2, ra) * f2;
 #else
     // one division: with s_k = (eps + b_k)^2 the Shu weights are
     // w_k = d_k prod_{j!=k} s_j / sum_m d_m prod_{j!=m} s_j; Borges a_k = d_k (1 + tau/e_k)
     // = d_k (e_k + tau) / e_k with e_k = b_k + eps, same clearing of denominators.
-    double p0, p1, p2;
+    real p0, p1, p2;
     if constexpr (BORGES) {
-        double tau5 = fabs(b0 - b2);
-        double e0 = b0 + EPS_BORGES, e1 = b1 + EPS_BORGES, e2 = b2 + EPS_BORGES;
+        real tau5 = fabs(b0 - b2);
+        real e0 = b0 + EPS_BORGES, e1 = b1 + EPS_BORGES, e2 = b2 + EPS_BORGES;
         p0 = d0 * (e0 + tau5) * (e1 * e2);
         p1 = d1 * (e1 + tau5) * (e0 * e2);
         p2 = d2 * (e2 + tau5) * (e0 * e1);
     } else {
-        double s0 = (EPS_SHU + b0) * (EPS_SHU + b0);
-        double s1 = (EPS_SHU + b1) * (EPS_SHU + b1);
-        double s2 = (EPS_SHU + b2) * (EPS_SHU + b2);
+        real s0 = (EPS_SHU + b0) * (EPS_SHU + b0);
+        real s1 = (EPS_SHU + b1) * (EPS_SHU + b1);
+        real s2 = (EPS_SHU + b2) * (EPS_SHU + b2);
         p0        = d0 * (s1 * s2);
         p1        = d1 * (s0 * s2);
         p2        = d2 * (s0 * s1);
@@ -144,20 +166,20 @@ __device__ __forceinline__ double weno_combine(double a, double b, double c, dou
 
 // Weno5::WENO5_left (weno-5.cpp:305-322), ideal weights d_left = {0.1, 0.6, 0.3}
 template <bool BORGES>
-__device__ __forceinline__ double weno_left(double a, double b, double c, double d, double e) {
-    double f0 = (2. * a - 7. * b + 11. * c) * ONEPERSIX;
-    double f1 = (-b + 5. * c + 2. * d) * ONEPERSIX;
-    double f2 = (2. * c + 5. * d - e) * ONEPERSIX;
-    return weno_combine<BORGES>(a, b, c, d, e, 0.1, 0.6, 0.3, f0, f1, f2);
+__device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e) {
+    real f0 = (RL(2.) * a - RL(7.) * b + RL(11.) * c) * ONEPERSIX;
+    real f1 = (-b + RL(5.) * c + RL(2.) * d) * ONEPERSIX;
+    real f2 = (RL(2.) * c + RL(5.) * d - e) * ONEPERSIX;
+    return weno_combine<BORGES>(a, b, c, d, e, RL(0.1), RL(0.6), RL(0.3), f0, f1, f2);
 }
 
 // Weno5::WENO5_right (weno-5.cpp:285-303), ideal weights d_right = {0.3, 0.6, 0.1}
 template <bool BORGES>
-__device__ __forceinline__ double weno_right(double a, double b, double c, double d, double e) {
-    double f0 = (-a + 5. * b + 2. * c) * ONEPERSIX;
-    double f1 = (2. * b + 5. * c - d) * ONEPERSIX;
-    double f2 = (11. * c - 7. * d + 2. * e) * ONEPERSIX;
-    return weno_combine<BORGES>(a, b, c, d, e, 0.3, 0.6, 0.1, f0, f1, f2);
+__device__ __forceinline__ real weno_right(real a, real b, real c, real d, real e) {
+    real f0 = (-a + RL(5.) * b + RL(2.) * c) * ONEPERSIX;
+    real f1 = (RL(2.) * b + RL(5.) * c - d) * ONEPERSIX;
+    real f2 = (RL(11.) * c - RL(7.) * d + RL(2.) * e) * ONEPERSIX;
+    return weno_combine<BORGES>(a, b, c, d, e, RL(0.3), RL(0.6), RL(0.1), f0, f1, f2);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -166,17 +188,17 @@ __device__ __forceinline__ double weno_right(double a, double b, double c, doubl
 // ------------------------------------------------------------------------------------------
 template <int DIR>
 __device__ __forceinline__ void split_cell(const double* __restrict__ U, const double* __restrict__ W,
-                                           long c, long P, double gamma, double s[10]) {
-    double rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
-    double qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
+                                           long c, long P, real gamma, real s[10]) {
+    real rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
+    real qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
 #if !NS_FAST
-    double cs = sqrt(dv(gamma * p, rho));
-    double H  = dv(cs * cs, gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+    real cs = sqrt(dv(gamma * p, rho));
+    real H  = dv(cs * cs, gamma - RL(1.0)) + RL(0.5) * (u * u + v * v + w * w);
 #else
-    double H = dv(gamma, gamma - 1.0) * dv(p, rho) + 0.5 * (u * u + v * v + w * w);
+    real H = dv(gamma, gamma - RL(1.0)) * dv(p, rho) + RL(0.5) * (u * u + v * v + w * w);
 #endif
-    double F[5];
-    double m = rho * qn;
+    real F[5];
+    real m = rho * qn;
     F[0]     = m;
     F[1]     = m * H;
     F[2]     = (DIR == 0) ? m * u + p : m * u;
@@ -184,9 +206,9 @@ __device__ __forceinline__ void split_cell(const double* __restrict__ U, const d
     F[4]     = (DIR == 2) ? m * w + p : m * w;
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        double uq = U[c + q * P];
-        s[q]      = (F[q] + LF_ALPHA * uq) * 0.5;
-        s[5 + q]  = (F[q] - LF_ALPHA * uq) * 0.5;
+        real uq = U[c + q * P];
+        s[q]      = (F[q] + LF_ALPHA * uq) * RL(0.5);
+        s[5 + q]  = (F[q] - LF_ALPHA * uq) * RL(0.5);
     }
 }
 
@@ -208,7 +230,7 @@ template <int A> struct Basis<2, A> {
 };
 
 // acc + K*x for a structural constant K in {-1, 0, 1}
-template <int K> __device__ __forceinline__ double cadd(double acc, double x) {
+template <int K> __device__ __forceinline__ real cadd(real acc, real x) {
     if constexpr (K == 1) return acc + x;
     else if constexpr (K == -1) return acc - x;
     else return acc;
@@ -217,54 +239,54 @@ template <int K> __device__ __forceinline__ double cadd(double acc, double x) {
 // the eigenvector entries that depend on the face state (ConvectionFlux::eigenvectors,
 // convection_flux.cpp:117-175, c^2 = gamma / (p rho) as written there)
 struct Eig {
-    double vel[3];
-    double c, H, qn, ql, qm, q2h;
-    double L0[5], L2[5], L3[5];  // rows 0, 2, 3 of L (all entries depend on the state)
+    real vel[3];
+    real c, H, qn, ql, qm, q2h;
+    real L0[5], L2[5], L3[5];  // rows 0, 2, 3 of L (all entries depend on the state)
 };
 
 template <int DIR>
-__device__ __forceinline__ void eigen(double rho, double p, double u, double v, double w,
-                                      double gamma, Eig& E) {
+__device__ __forceinline__ void eigen(real rho, real p, real u, real v, real w,
+                                      real gamma, Eig& E) {
     E.vel[0]  = u;
     E.vel[1]  = v;
     E.vel[2]  = w;
-    double qn = E.vel[DIR];
+    real qn = E.vel[DIR];
     // ql = u*lx + v*ly + w*lz, qm = u*mx + v*my + w*mz with one nonzero entry each
-    double ql = (DIR == 2) ? -v : w;
-    double qm = (DIR == 0) ? -v : u;
-    double q2 = u * u + v * v + w * w;
-    double K  = gamma - 1.0;
+    real ql = (DIR == 2) ? -v : w;
+    real qm = (DIR == 0) ? -v : u;
+    real q2 = u * u + v * v + w * w;
+    real K  = gamma - RL(1.0);
 #if !NS_FAST
-    double c2   = dv(gamma, p * rho);
-    double c    = sqrt(c2);
-    double H    = dv(c2, gamma - 1.0) + 0.5 * q2;
-    double K_pc = dv(K, 2.0 * c2);
-    Rcp    r2c  = rcp(2.0 * c);
-    double qn2c = dv(qn, r2c);
-    double i2c  = dv(1.0, r2c);  // n_a / (2c) for the normal component
-    Rcp    rc2  = rcp(c2);
+    real c2   = dv(gamma, p * rho);
+    real c    = sqrt(c2);
+    real H    = dv(c2, gamma - RL(1.0)) + RL(0.5) * q2;
+    real K_pc = dv(K, RL(2.0) * c2);
+    auto   r2c  = rcp(RL(2.0) * c);
+    real qn2c = dv(qn, r2c);
+    real i2c  = dv(RL(1.0), r2c);  // n_a / (2c) for the normal component
+    auto   rc2  = rcp(c2);
 #pragma unroll
     for (int a = 0; a < 3; a++) E.L2[1 + a] = dv(K * E.vel[a], rc2);
     E.L2[4] = dv(-K, rc2);
 #else
-    double c2   = dv(gamma, p * rho);
-    double ic2  = dv(p * rho, gamma);  // 1 / c^2
-    double c    = sqrt(c2);
-    double H    = dv(c2, K) + 0.5 * q2;
-    double K_pc = 0.5 * K * ic2;
-    double i2c  = 0.5 * c * ic2;    // 1 / (2c)
-    double qn2c = qn * i2c;
+    real c2   = dv(gamma, p * rho);
+    real ic2  = dv(p * rho, gamma);  // 1 / c^2
+    real c    = sqrt(c2);
+    real H    = dv(c2, K) + RL(0.5) * q2;
+    real K_pc = RL(0.5) * K * ic2;
+    real i2c  = RL(0.5) * c * ic2;    // 1 / (2c)
+    real qn2c = qn * i2c;
 #pragma unroll
     for (int a = 0; a < 3; a++) E.L2[1 + a] = K * E.vel[a] * ic2;
     E.L2[4] = -K * ic2;
 #endif
-    double t = 0.5 * K_pc * q2;
-    E.c = c, E.H = H, E.qn = qn, E.ql = ql, E.qm = qm, E.q2h = 0.5 * q2;
+    real t = RL(0.5) * K_pc * q2;
+    E.c = c, E.H = H, E.qn = qn, E.ql = ql, E.qm = qm, E.q2h = RL(0.5) * q2;
     E.L0[0] = t + qn2c;
     E.L3[0] = t - qn2c;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-        double kv = K_pc * E.vel[a];
+        real kv = K_pc * E.vel[a];
         if (a == DIR) {
             E.L0[1 + a] = -(kv + i2c);
             E.L3[1 + a] = -(kv - i2c);
@@ -275,13 +297,13 @@ __device__ __forceinline__ void eigen(double rho, double p, double u, double v, 
     }
     E.L0[4] = K_pc;
     E.L3[4] = K_pc;
-    E.L2[0] = 1.0 - K_pc * q2;
+    E.L2[0] = RL(1.0) - K_pc * q2;
 }
 
 // row Q of L times the split stencil values s[p] (p in conservative order), summed in p order
 // from 0.0 (matrix::square_matrix_matrix_product, matrix.cpp:12-20)
 template <int DIR, int Q>
-__device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
+__device__ __forceinline__ real Lrow(const Eig& E, const real s[5]) {
     if constexpr (Q == 0) {
         return E.L0[0] * s[0] + E.L0[1] * s[1] + E.L0[2] * s[2] + E.L0[3] * s[3] + E.L0[4] * s[4];
     } else if constexpr (Q == 2) {
@@ -289,13 +311,13 @@ __device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
     } else if constexpr (Q == 3) {
         return E.L3[0] * s[0] + E.L3[1] * s[1] + E.L3[2] * s[2] + E.L3[3] * s[3] + E.L3[4] * s[4];
     } else if constexpr (Q == 1) {  // {-qm, mx, my, mz, 0}
-        double acc = (-E.qm) * s[0];
+        real acc = (-E.qm) * s[0];
         acc        = cadd<Basis<DIR, 0>::m>(acc, s[1]);
         acc        = cadd<Basis<DIR, 1>::m>(acc, s[2]);
         acc        = cadd<Basis<DIR, 2>::m>(acc, s[3]);
         return acc;
     } else {  // Q == 4: {-ql, lx, ly, lz, 0}
-        double acc = (-E.ql) * s[0];
+        real acc = (-E.ql) * s[0];
         acc        = cadd<Basis<DIR, 0>::l>(acc, s[1]);
         acc        = cadd<Basis<DIR, 1>::l>(acc, s[2]);
         acc        = cadd<Basis<DIR, 2>::l>(acc, s[3]);
@@ -305,13 +327,13 @@ __device__ __forceinline__ double Lrow(const Eig& E, const double s[5]) {
 
 // R times the characteristic fluxes ft (matrix::square_matrix_vector_product, matrix.cpp:4-10)
 template <int DIR, int A>
-__device__ __forceinline__ double Rvel(const Eig& E, const double ft[5]) {
+__device__ __forceinline__ real Rvel(const Eig& E, const real ft[5]) {
     // row 1+A: {vel_A - c n_A, m_A, vel_A, vel_A + c n_A, l_A}
     constexpr int n   = Basis<DIR, A>::n;
-    double        va  = E.vel[A];
-    double        e0  = n ? va - E.c : va;
-    double        e3  = n ? va + E.c : va;
-    double        acc = e0 * ft[0];
+    real        va  = E.vel[A];
+    real        e0  = n ? va - E.c : va;
+    real        e3  = n ? va + E.c : va;
+    real        acc = e0 * ft[0];
     acc               = cadd<Basis<DIR, A>::m>(acc, ft[1]);
     acc               = acc + va * ft[2];
     acc               = acc + e3 * ft[3];
@@ -320,30 +342,30 @@ __device__ __forceinline__ double Rvel(const Eig& E, const double ft[5]) {
 }
 
 template <int DIR>
-__device__ __forceinline__ void Rmul(const Eig& E, const double ft[5], double out[5]) {
+__device__ __forceinline__ void Rmul(const Eig& E, const real ft[5], real out[5]) {
     out[0]    = ft[0] + ft[2] + ft[3];  // {1, 0, 1, 1, 0}
     out[1]    = Rvel<DIR, 0>(E, ft);
     out[2]    = Rvel<DIR, 1>(E, ft);
     out[3]    = Rvel<DIR, 2>(E, ft);
-    double qc = E.qn * E.c;             // {H - qn c, qm, q2/2, H + qn c, ql}
+    real qc = E.qn * E.c;             // {H - qn c, qm, q2/2, H + qn c, ql}
     out[4] = (E.H - qc) * ft[0] + E.qm * ft[1] + E.q2h * ft[2] + (E.H + qc) * ft[3] + E.ql * ft[4];
 }
 
 // ------------------------------------------------------------------------------------------
 // one face flux from the split values of its six stencil cells.  S(ii, v) returns split value v
-// (0..4 fL, 5..9 fR) of cell i-2+ii, ii = 0..5; Wl, Wr are the primitives of cells i and i+1.
+// (0..4 fL, 5..9 fR) of cell i-2+ii, ii = 0..5; Wl, Wr are the primitives of cells i and i+1
 // (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
 // ------------------------------------------------------------------------------------------
 template <int DIR, bool BORGES, int Q, class S>
-__device__ __forceinline__ double char_component(const Eig& E, const S& st) {
+__device__ __forceinline__ real char_component(const Eig& E, const S& st) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
 #if NS_ROW_FENCE
     __asm__ __volatile__("" ::: "memory");  // re-read the stencil per row: fewer live registers
 #endif
-    double a[5], b[5];
+    real a[5], b[5];
 #pragma unroll
     for (int ii = 0; ii < 5; ii++) {
-        double sl[5], sr[5];
+        real sl[5], sr[5];
 #pragma unroll
         for (int p = 0; p < 5; p++) {
             sl[p] = st(ii, p);
@@ -357,8 +379,8 @@ __device__ __forceinline__ double char_component(const Eig& E, const S& st) {
 }
 
 template <int DIR, int CONV, bool BORGES, class S>
-__device__ __forceinline__ void face_flux(const S& st, const double Wl[5], const double Wr[5],
-                                          double gamma, double out[5]) {
+__device__ __forceinline__ void face_flux(const S& st, const real Wl[5], const real Wr[5],
+                                          real gamma, real out[5]) {
     if constexpr (CONV == 1) {
 #pragma unroll
         for (int q = 0; q < 5; q++)
@@ -366,12 +388,12 @@ __device__ __forceinline__ void face_flux(const S& st, const double Wl[5], const
                      weno_right<BORGES>(st(1, 5 + q), st(2, 5 + q), st(3, 5 + q), st(4, 5 + q),
                                         st(5, 5 + q));
     } else {
-        double Wa[5];
+        real Wa[5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) Wa[q] = 0.5 * (Wl[q] + Wr[q]);
+        for (int q = 0; q < 5; q++) Wa[q] = RL(0.5) * (Wl[q] + Wr[q]);
         Eig E;
         eigen<DIR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
-        double ft[5];
+        real ft[5];
         ft[0] = char_component<DIR, BORGES, 0>(E, st);
         ft[1] = char_component<DIR, BORGES, 1>(E, st);
         ft[2] = char_component<DIR, BORGES, 2>(E, st);
@@ -382,14 +404,14 @@ __device__ __forceinline__ void face_flux(const S& st, const double Wl[5], const
 }
 
 struct RegSplit {  // split values held in registers: s[ii][v]
-    const double (*s)[10];
-    __device__ double operator()(int ii, int v) const { return s[ii][v]; }
+    const real (*s)[10];
+    __device__ real operator()(int ii, int v) const { return s[ii][v]; }
 };
 
 template <int LANES>
 struct LdsSplit {  // split values in LDS, layout [cell][v][lane]
-    const double* base;  // &tile[first cell][0][lane]
-    __device__ double operator()(int ii, int v) const { return base[(ii * 10 + v) * LANES]; }
+    const real* base;  // &tile[first cell][0][lane]
+    __device__ real operator()(int ii, int v) const { return base[(ii * 10 + v) * LANES]; }
 };
 
 // ------------------------------------------------------------------------------------------
@@ -409,9 +431,9 @@ template <int DIR, int CONV, bool BORGES>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
                                                       const double* __restrict__ W,
                                                       double* __restrict__ f, Geo g,
-                                                      double gamma) {
+                                                      real gamma) {
     constexpr int NC = TILE_S + 5;
-    __shared__ double tile[NC * 10 * TILE_L];
+    __shared__ real tile[NC * 10 * TILE_L];
     const int lane = threadIdx.x, w = threadIdx.y;
     const int k    = NGC + blockIdx.x * TILE_L + lane;
     const int o    = NGC + blockIdx.y;              // j for the x sweep, i for the y sweep
@@ -424,7 +446,7 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const doubl
     for (int t = w; t < NC; t += TILE_S) {
         int cs = s0 + t;
         if (kin && cs <= n + 2 * NGC - 1) {
-            double sv[10];
+            real sv[10];
             split_cell<DIR>(U, W, (long)cs * ss + (long)o * so + k, g.P, gamma, sv);
 #pragma unroll
             for (int v = 0; v < 10; v++) tile[(t * 10 + v) * TILE_L + lane] = sv[v];
@@ -435,13 +457,13 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const doubl
     if (!kin || fidx >= n) return;
     const int  i = fidx + NGC - 1;  // face i+1/2
     const long c = (long)i * ss + (long)o * so + k;
-    double     Wl[5], Wr[5];
+    real     Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
         for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + ss + q * g.P];
     }
     LdsSplit<TILE_L> st{&tile[(w * 10) * TILE_L + lane]};
-    double           out[5];
+    real           out[5];
     face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -453,20 +475,20 @@ template <int CONV, bool BORGES>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      double* __restrict__ f, Geo g,
-                                                     double gamma) {
+                                                     real gamma) {
     constexpr int NC = TILE_L + 5;
-    __shared__ double tile[8][NC * 10];
+    __shared__ real tile[8][NC * 10];
     const int lane = threadIdx.x, r = threadIdx.y;
     const int j    = NGC + blockIdx.y * 8 + r;
     const int i    = NGC + blockIdx.z;
     const int f0   = blockIdx.x * TILE_L;  // first face of the segment (0-based)
     const bool rowok = (j < NGC + g.ny);   // every wave reaches the barrier below
     const long row = (long)i * g.sx + (long)j * g.sy;
-    double*    T   = tile[r];
+    real*      T   = tile[r];
     for (int t = lane; t < NC; t += TILE_L) {
         int cs = f0 + t;  // padded k of the cell
         if (rowok && cs <= g.nz + 2 * NGC - 1) {
-            double sv[10];
+            real sv[10];
             split_cell<2>(U, W, row + cs, g.P, gamma, sv);
 #pragma unroll
             for (int v = 0; v < 10; v++) T[v * NC + t] = sv[v];
@@ -477,16 +499,16 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
     if (!rowok || fidx >= g.nz) return;
     const int  kf = fidx + NGC - 1;
     const long c  = row + kf;
-    double     Wl[5], Wr[5];
+    real     Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
         for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + 1 + q * g.P];
     }
     struct {
-        const double* b;
-        __device__ double operator()(int ii, int v) const { return b[v * NC + ii]; }
+        const real* b;
+        __device__ real operator()(int ii, int v) const { return b[v * NC + ii]; }
     } st{&T[lane]};
-    double out[5];
+    real out[5];
     face_flux<2, CONV, BORGES>(st, Wl, Wr, gamma, out);
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -496,7 +518,7 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
 template <int DIR, int CONV, bool BORGES>
 __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
                                                     const double* __restrict__ W,
-                                                    double* __restrict__ f, Geo g, double gamma,
+                                                    double* __restrict__ f, Geo g, real gamma,
                                                     int i) {
     // tangential indices (t1 slow, t2 fast) over the interior
     int t2 = threadIdx.x + blockIdx.x * blockDim.x;
@@ -508,16 +530,16 @@ __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U
     if (DIR == 0) c = (long)i * g.sx + (long)(t1 + NGC) * g.sy + (t2 + NGC), s = g.sx;
     else if (DIR == 1) c = (long)(t1 + NGC) * g.sx + (long)i * g.sy + (t2 + NGC), s = g.sy;
     else c = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy + i, s = 1;
-    double sv[6][10];
+    real sv[6][10];
 #pragma unroll
     for (int ii = 0; ii < 6; ii++) split_cell<DIR>(U, W, c + (ii - 2) * s, g.P, gamma, sv[ii]);
-    double Wl[5], Wr[5];
+    real Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
         for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + s + q * g.P];
     }
     RegSplit st{sv};
-    double   out[5];
+    real   out[5];
     face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -721,10 +743,10 @@ __device__ __forceinline__ void xcd_tile(int& b2, int& b1, int& b0) {
 // of the first derivative skipped (an exact +-0 addend); FAST: sum of f * (c_k / d) with the
 // reciprocal of d hoisted.
 template <int ORDER, bool SECOND, class F>
-__device__ __forceinline__ double fd_sum(const F& f, double d, double id) {
+__device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
     constexpr int H     = ORDER / 2;
     constexpr int OI    = ORDER / 2 - 1;
-    double        s     = 0.;
+    real        s     = RL(0.);
     bool          first = true;
 #if !NS_FAST
     const Rcp rd = rcp(d);
@@ -732,12 +754,12 @@ __device__ __forceinline__ double fd_sum(const F& f, double d, double id) {
 #pragma unroll
     for (int t = 0; t <= 2 * H; t++) {
         if (!SECOND && t == H) continue;
-        double co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
+        real co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
 #if !NS_FAST
-        double v = dv(f(t) * co, rd);
+        real v = dv(f(t) * co, rd);
         (void)id;
 #else
-        double v = f(t) * (co * id);
+        real v = f(t) * (co * id);
         (void)d;
 #endif
         if (first) {
@@ -766,7 +788,7 @@ __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
     if (k >= g.nz || j >= g.ny) return;
     const int  i1 = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
     const long c0 = (long)(j + NGC) * g.sy + (k + NGC);
-    double     win[4][2 * H + 1];
+    real     win[4][2 * H + 1];
 #pragma unroll
     for (int t = 0; t < 2 * H; t++)
 #pragma unroll
@@ -820,23 +842,30 @@ __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
 // constant-viscosity zero terms (dmu = dlamb = 0) dropped: each is an exact +-0 addend.
 // wx[q][t]: primitives rho, p, u, v at x offsets t - H; dyx[a][t]: dW/dy of u (a = 0), v (a = 1)
 // at x offsets t - H; Dx, Dy: derivative planes at the cell.
-template <int ORDER, bool SQ>
-__device__ __forceinline__ void viscous(const double (&wx)[4][ORDER + 1],
-                                        const double (&dyx)[2][ORDER + 1],
+// Sutherland law (extension, see nsgpu.h): ((mu (1 + C)) / (T + C)) (T sqrt(T)), T = gamma p / rho
+__device__ __forceinline__ real mu_suth(real rho, real p, const Phys& ph) {
+    real T = dv((real)ph.gamma * p, rho);
+    return dv((real)ph.suth_k, T + (real)ph.suth_C) * (T * sqrt(T));
+}
+
+template <int ORDER, bool SQ, bool SUTH>
+__device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
+                                        const real (&dyx)[2][ORDER + 1],
+                                        const double* __restrict__ W,
                                         const double* __restrict__ Dx,
                                         const double* __restrict__ Dy, long c, const Geo& g,
-                                        const Phys& ph, double& ene, double& x_mom,
-                                        double& y_mom) {
+                                        const Phys& ph, real& ene, real& x_mom,
+                                        real& y_mom) {
     constexpr int H   = ORDER / 2;
     const long    P   = g.P;
     const int     rho = 0, p = 1, u = 2, v = 3;
     // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79); when
     // dx*dx == dy*dy the two sums are the same operations on the same operands
-    double du_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[u][t]; }, g.dxx, g.idxx);
-    double dv_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[v][t]; }, g.dxx, g.idxx);
-    double drho_dxdx = fd_sum<ORDER, true>([&](int t) { return wx[rho][t]; }, g.dxx, g.idxx);
-    double dp_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[p][t]; }, g.dxx, g.idxx);
-    double du_dydy, dv_dydy, drho_dydy, dp_dydy;
+    real du_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[u][t]; }, g.dxx, g.idxx);
+    real dv_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[v][t]; }, g.dxx, g.idxx);
+    real drho_dxdx = fd_sum<ORDER, true>([&](int t) { return wx[rho][t]; }, g.dxx, g.idxx);
+    real dp_dxdx   = fd_sum<ORDER, true>([&](int t) { return wx[p][t]; }, g.dxx, g.idxx);
+    real du_dydy, dv_dydy, drho_dydy, dp_dydy;
     if constexpr (SQ) {
         du_dydy = du_dxdx, dv_dydy = dv_dxdx, drho_dydy = drho_dxdx, dp_dydy = dp_dxdx;
     } else {
@@ -848,48 +877,82 @@ __device__ __forceinline__ void viscous(const double (&wx)[4][ORDER + 1],
     // cross derivatives: dW_dxdy = df_dy(dW_dx), dW_dydx = df_dx(dW_dy)
     const double* Dxu     = Dx + u * P + c;
     const double* Dxv     = Dx + v * P + c;
-    double        du_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxu[(long)(t - H) * g.sy]; },
+    real        du_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxu[(long)(t - H) * g.sy]; },
                                           g.dy, g.idy);
-    double dv_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxv[(long)(t - H) * g.sy]; }, g.dy,
+    real dv_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxv[(long)(t - H) * g.sy]; }, g.dy,
                                           g.idy);
-    double du_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[0][t]; }, g.dx, g.idx);
-    double dv_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[1][t]; }, g.dx, g.idx);
+    real du_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[0][t]; }, g.dx, g.idx);
+    real dv_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[1][t]; }, g.dx, g.idx);
 
-    double du_dx = Dxu[0], dv_dx = Dxv[0], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
-    double du_dy = dyx[0][H], dv_dy = dyx[1][H], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
-    double U = wx[u][H], V = wx[v][H], Pp = wx[p][H], RHO = wx[rho][H];
+    real du_dx = Dxu[0], dv_dx = Dxv[0], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
+    real du_dy = dyx[0][H], dv_dy = dyx[1][H], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
+    real U = wx[u][H], V = wx[v][H], Pp = wx[p][H], RHO = wx[rho][H];
 
-    double mu        = ph.mu;
-    double lamb      = ph.lamb;
-    double Dtauxx_dx = ph.two_mu * du_dxdx + lamb * (du_dxdx + dv_dydx);
-    double Dtauyy_dy = ph.two_mu * dv_dydy + lamb * (du_dxdy + dv_dydy);
-    double Dtauxy_dx = mu * (du_dydx + dv_dxdx);
-    double Dtauyx_dy = mu * (du_dydy + dv_dxdy);
-    double Tauxx     = ph.two_mu * du_dx + lamb * (du_dx + dv_dy);
-    double Tauyy     = Tauxx;  // tauxx(...) as written at centered_diffusion.cpp:143
-    double Tauxy     = mu * (du_dy + dv_dx);
-    double Tauyx     = Tauxy;
-    double Dtauvx_dx = U * Dtauxx_dx + V * Dtauxy_dx + Tauxx * du_dx + Tauxy * dv_dy;
-    double Dtauvy_dy = U * Dtauyx_dy + V * Dtauyy_dy + Tauyy * dv_dy + Tauyx * du_dy;
+    real Dtauxx_dx, Dtauyy_dy, Dtauxy_dx, Dtauyx_dy, Tauxx, Tauxy, mu, dmu_dx = 0, dmu_dy = 0;
+    if constexpr (!SUTH) {
+        mu          = (real)ph.mu;
+        real lamb   = (real)ph.lamb;
+        real two_mu = (real)ph.two_mu;
+        Dtauxx_dx   = two_mu * du_dxdx + lamb * (du_dxdx + dv_dydx);
+        Dtauyy_dy   = two_mu * dv_dydy + lamb * (du_dxdy + dv_dydy);
+        Dtauxy_dx = mu * (du_dydx + dv_dxdx);
+        Dtauyx_dy = mu * (du_dydy + dv_dxdy);
+        Tauxx     = two_mu * du_dx + lamb * (du_dx + dv_dy);
+        Tauxy     = mu * (du_dy + dv_dx);
+    } else {
+        // viscosity and its first derivatives from the primitive stencils; the full expressions
+        // of diffusion_scheme.cpp:3-58 with the dmu / dlamb terms (centered_diffusion.cpp:96-145)
+        const double* Wr = W + rho * P + c;
+        const double* Wp = W + p * P + c;
+        mu     = mu_suth(RHO, Pp, ph);
+        dmu_dx = fd_sum<ORDER, false>([&](int t) { return mu_suth(wx[rho][t], wx[p][t], ph); },
+                                      g.dx, g.idx);
+        dmu_dy = fd_sum<ORDER, false>(
+            [&](int t) { return mu_suth(Wr[(long)(t - H) * g.sy], Wp[(long)(t - H) * g.sy], ph); },
+            g.dy, g.idy);
+        real lamb     = (-RL(2.) / RL(3.)) * mu;
+        real dlamb_dx = -(RL(2.) / RL(3.)) * dmu_dx;
+        real dlamb_dy = -(RL(2.) / RL(3.)) * dmu_dy;
+        Dtauxx_dx = RL(2.0) * dmu_dx * du_dx + RL(2.0) * mu * du_dxdx + dlamb_dx * (du_dx + dv_dy);
+        Dtauxx_dx += lamb * (du_dxdx + dv_dydx);
+        Dtauyy_dy = RL(2.0) * dmu_dy * dv_dy + RL(2.0) * mu * dv_dydy + dlamb_dy * (du_dx + dv_dy);
+        Dtauyy_dy += lamb * (du_dxdy + dv_dydy);
+        Dtauxy_dx = dmu_dx * (du_dy + dv_dx) + mu * (du_dydx + dv_dxdx);
+        Dtauyx_dy = dmu_dy * (du_dy + dv_dx) + mu * (du_dydy + dv_dxdy);
+        Tauxx     = RL(2.0) * mu * du_dx + lamb * (du_dx + dv_dy);
+        Tauxy     = mu * (du_dy + dv_dx);
+    }
+    real Tauyy     = Tauxx;  // tauxx(...) as written at centered_diffusion.cpp:143
+    real Tauyx     = Tauxy;
+    real Dtauvx_dx = U * Dtauxx_dx + V * Dtauxy_dx + Tauxx * du_dx + Tauxy * dv_dy;
+    real Dtauvy_dy = U * Dtauyx_dy + V * Dtauyy_dy + Tauyy * dv_dy + Tauyx * du_dy;
 #if !NS_FAST
-    double one_rho  = dv(1., RHO);
-    double one_rho2 = dv(1., RHO * RHO);
-    double one_rho3 = dv(1., RHO * RHO * RHO);
+    real one_rho  = dv(RL(1.), RHO);
+    real one_rho2 = dv(RL(1.), RHO * RHO);
+    real one_rho3 = dv(RL(1.), RHO * RHO * RHO);
 #else
-    double one_rho  = dv(1., RHO);
-    double one_rho2 = one_rho * one_rho;
-    double one_rho3 = one_rho2 * one_rho;
+    real one_rho  = dv(RL(1.), RHO);
+    real one_rho2 = one_rho * one_rho;
+    real one_rho3 = one_rho2 * one_rho;
 #endif
-    double temp2x = mu * (2.0 * one_rho2 * drho_dx * dp_dx + one_rho * dp_dxdx +
+    real temp2x = mu * (RL(2.0) * one_rho2 * drho_dx * dp_dx + one_rho * dp_dxdx +
                           one_rho3 * Pp * drho_dx * drho_dx - one_rho2 * Pp * drho_dxdx);
-    double Dqx_dx = -ph.heat * temp2x;
     // the y heat-flux term multiplies by the enum value p = 1 (centered_diffusion.cpp:166)
-    double temp2y = mu * (2.0 * one_rho2 * drho_dy * dp_dy + one_rho * dp_dydy +
+    real temp2y = mu * (RL(2.0) * one_rho2 * drho_dy * dp_dy + one_rho * dp_dydy +
                           one_rho3 * drho_dy * drho_dy - one_rho2 * Pp * drho_dydy);
-    double Dqy_dy = -ph.heat * temp2y;
-    x_mom         = -Dtauxx_dx - Dtauyx_dy;
-    y_mom         = -Dtauyy_dy - Dtauxy_dx;
-    ene           = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
+    real Dqx_dx, Dqy_dy;
+    if constexpr (!SUTH) {
+        Dqx_dx = -(real)ph.heat * temp2x;
+        Dqy_dy = -(real)ph.heat * temp2y;
+    } else {
+        real temp1x = dmu_dx * (one_rho * dp_dx - Pp * one_rho2 * drho_dx);
+        real temp1y = dmu_dy * (one_rho * dp_dy - Pp * one_rho2 * drho_dy);
+        Dqx_dx        = -(real)ph.heat * (temp1x + temp2x);
+        Dqy_dy        = -(real)ph.heat * (temp1y + temp2y);
+    }
+    x_mom = -Dtauxx_dx - Dtauyx_dy;
+    y_mom = -Dtauyy_dy - Dtauxy_dx;
+    ene   = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
 }
 
 // residual + stage update, marching along i:
@@ -898,7 +961,7 @@ __device__ __forceinline__ void viscous(const double (&wx)[4][ORDER + 1],
 // 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96)
 // and the primitives of the new state (eq_of_state.cpp:61-87).  Without a stage (st.Uout null)
 // R is stored instead (operator-level calls).
-template <int ORDER, bool SQ>
+template <int ORDER, bool SQ, bool SUTH>
 __global__ __launch_bounds__(256) void k_residual(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
@@ -915,7 +978,8 @@ __global__ __launch_bounds__(256) void k_residual(
     const long c0   = (long)(j + NGC) * g.sy + (k + NGC);
     const bool conv = (which & 1) && ph.conv;
     const bool visc = (ORDER > 0) && (which & 2);
-    double     wx[4][O + 1], dyx[2][O + 1], fxm[5];
+    real       wx[4][O + 1], dyx[2][O + 1];
+    double     fxm[5];
     if (visc) {
 #pragma unroll
         for (int t = 0; t < 2 * H; t++) {
@@ -982,8 +1046,8 @@ __global__ __launch_bounds__(256) void k_residual(
                 for (int q = 0; q < 4; q++) wx[q][2 * H] = W[q * P + cn];
                 dyx[0][2 * H] = Dy[2 * P + cn];
                 dyx[1][2 * H] = Dy[3 * P + cn];
-                double ene, xm, ym;
-                viscous<ORDER, SQ>(wx, dyx, Dx, Dy, c, g, ph, ene, xm, ym);
+                real ene, xm, ym;
+                viscous<ORDER, SQ, SUTH>(wx, dyx, W, Dx, Dy, c, g, ph, ene, xm, ym);
                 R[1]    = have[1] ? R[1] + ene : ene;
                 R[2]    = have[2] ? R[2] + xm : xm;
                 R[3]    = have[3] ? R[3] + ym : ym;
@@ -1235,10 +1299,17 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
                            StageArgs a, const unsigned long long* dtmin, int which, Geo g,
                            Phys ph, hipStream_t s) {
     dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
-    if (g.square_xy)
-        k_residual<ORDER, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
-    else
-        k_residual<ORDER, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    if (ph.suth) {
+        if (g.square_xy)
+            k_residual<ORDER, true, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+        else
+            k_residual<ORDER, false, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    } else {
+        if (g.square_xy)
+            k_residual<ORDER, true, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+        else
+            k_residual<ORDER, false, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    }
 }
 
 static void l_residual(const double* fx, const double* fy, const double* fz, const double* W,

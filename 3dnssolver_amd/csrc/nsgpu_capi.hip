@@ -100,7 +100,11 @@ int validate(const nsgpu_config* c) {
     if (c->weights != NSGPU_WEIGHTS_SHU && c->weights != NSGPU_WEIGHTS_BORGES) return NSGPU_ERR_INVALID;
     if (c->diff_order != 0 && c->diff_order != 2 && c->diff_order != 4 && c->diff_order != 6)
         return NSGPU_ERR_INVALID;
-    if (c->math != NSGPU_MATH_EXACT && c->math != NSGPU_MATH_FAST) return NSGPU_ERR_INVALID;
+    if (c->math != NSGPU_MATH_EXACT && c->math != NSGPU_MATH_FAST && c->math != NSGPU_MATH_FP32)
+        return NSGPU_ERR_INVALID;
+    if (c->viscosity != NSGPU_VISC_CONSTANT && c->viscosity != NSGPU_VISC_SUTHERLAND)
+        return NSGPU_ERR_INVALID;
+    if (c->viscosity == NSGPU_VISC_SUTHERLAND && !(c->suth_C > 0)) return NSGPU_ERR_INVALID;
     // periodicity per axis from West / South / Front (decomposition.cpp:145-161); every face of a
     // non-periodic axis must be Neumann (domain.cpp:87-137 leaves other kinds without a boundary)
     static const int lead[3] = {0, 3, 4};
@@ -213,7 +217,12 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     c->ph.lamb       = (-2. / 3.) * cfg->mu;                           // centered_diffusion.cpp:101
     c->ph.two_mu     = 2.0 * cfg->mu;
     c->ph.heat       = cfg->gamma / (cfg->Pr * (cfg->gamma - 1.0));    // centered_diffusion.cpp:161
-    c->L = (cfg->math == NSGPU_MATH_FAST) ? &fast::launchers() : &exact::launchers();
+    c->ph.suth       = (cfg->viscosity == NSGPU_VISC_SUTHERLAND);
+    c->ph.suth_C     = cfg->suth_C;
+    c->ph.suth_k     = cfg->mu * (1.0 + cfg->suth_C);
+    c->L = (cfg->math == NSGPU_MATH_FAST)   ? &fast::launchers()
+           : (cfg->math == NSGPU_MATH_FP32) ? &fp32::launchers()
+                                            : &exact::launchers();
     c->plane_bytes = (size_t)g.P * sizeof(double);
     size_t five = 5 * c->plane_bytes, four = 4 * c->plane_bytes;
     rc = NSGPU_OK;

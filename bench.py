@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--math", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--math", choices=["exact", "fast", "fp32"], default="exact")
     ap.add_argument("--conv", choices=["weno5cw", "weno5"], default="weno5cw")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)

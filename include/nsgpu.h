@@ -58,12 +58,19 @@ enum { NSGPU_BC_PERIODIC = 0, NSGPU_BC_NEUMANN = 1 };
 enum { NSGPU_CONV_NONE = 0, NSGPU_CONV_WENO5 = 1, NSGPU_CONV_WENO5CW = 2 };
 enum { NSGPU_WEIGHTS_SHU = 0, NSGPU_WEIGHTS_BORGES = 1 };
 /* arithmetic of the kernels:
- *   EXACT  the reference's FP64 operation order, no contraction, IEEE division and sqrt:
- *          results are bitwise those of a strict (-ffp-contract=off) build of the reference
+ *   EXACT  the reference's FP64 operation order, no contraction, correctly rounded division and
+ *          sqrt: results are bitwise those of a strict (-ffp-contract=off) build of the reference
  *   FAST   FP64 with fused multiply-add, hoisted reciprocals and one division per WENO
- *          reconstruction; agrees with EXACT to round-off (tolerances in the tests) */
-enum { NSGPU_MATH_EXACT = 0, NSGPU_MATH_FAST = 1 };
+ *          reconstruction; agrees with EXACT to round-off (tolerances in the tests)
+ *   FP32   FAST with the reconstruction and the viscous fluxes in single precision; state, stage
+ *          accumulation, equation of state and reductions in FP64 */
+enum { NSGPU_MATH_EXACT = 0, NSGPU_MATH_FAST = 1, NSGPU_MATH_FP32 = 2 };
 enum { NSGPU_STATE_CONSERVATIVE = 0, NSGPU_STATE_PRIMITIVE = 1 };
+/* viscosity law: CONSTANT mu (CenteredDiffusion as the reference runs it, dmu = 0,
+ * centered_diffusion.cpp:96-103), or SUTHERLAND mu(T) = mu (1 + C)/(T + C) T^1.5 with
+ * T = gamma p / rho (Viscosity::mu_sutherland, viscosity.cpp:5-11, which the reference defines
+ * but never calls) feeding the dmu slots of the same residual */
+enum { NSGPU_VISC_CONSTANT = 0, NSGPU_VISC_SUTHERLAND = 1 };
 enum { NSGPU_RHS_CONVECTION = 1, NSGPU_RHS_DIFFUSION = 2, NSGPU_RHS_BOTH = 3 };
 
 typedef struct {
@@ -77,6 +84,8 @@ typedef struct {
     int    diff_order;  /* 0: no diffusion; 2, 4, 6: CenteredDiffusion order */
     int    bc[6];       /* NSGPU_BC_*, faces W,E,N,S,F,B */
     int    math;        /* NSGPU_MATH_* */
+    int    viscosity;   /* NSGPU_VISC_*: CONSTANT (the reference path) or SUTHERLAND */
+    double suth_C;      /* Sutherland constant over the reference temperature (T0 = 1) */
 } nsgpu_config;
 
 /* Create a context on GPU `device` (HIP ordinal).  Allocates the block's device arrays. */

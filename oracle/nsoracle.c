@@ -608,7 +608,8 @@ static double dtauyx_dy(double mu, double dmu_dy, double du_dy, double dv_dx, do
  * Tauyy = tauxx(...), Tauxy*dv_dy, the heat-flux y term uses the enum value p = 1) */
 static void diff_residual(const nso_cfg* c, const double* Wp, const double* Dx, const double* Dy,
                           const double* Dxx, const double* Dyy, const double* Dxy,
-                          const double* Dyx, double* Rs) {
+                          const double* Dyx, const double* MU, const double* DMUX,
+                          const double* DMUY, double* Rs) {
     pad_t        P = pad_of(c);
     const int    rho = 0, p = 1, u = 2, v = 3, w = 4;
     const double m_mu = c->mu, m_gamma = c->gamma, m_Pr = c->Pr;
@@ -617,9 +618,9 @@ static void diff_residual(const nso_cfg* c, const double* Wp, const double* Dx, 
         for (long j = NGC; j < NGC + c->ny; j++)
             for (long k = NGC; k < NGC + c->nz; k++) {
                 size_t b        = ((((size_t)i) * P.NY + j) * P.NZ + k) * NQ;
-                double mu       = m_mu;
-                double dmu_dx   = 0.0;
-                double dmu_dy   = 0.0;
+                double mu       = MU ? MU[b] : m_mu;
+                double dmu_dx   = MU ? DMUX[b] : 0.0;
+                double dmu_dy   = MU ? DMUY[b] : 0.0;
                 double lamb     = (-2. / 3.) * mu;
                 double dlamb_dx = -(2. / 3.) * dmu_dx;
                 double dlamb_dy = -(2. / 3.) * dmu_dy;
@@ -678,6 +679,16 @@ static void diff_residual(const nso_cfg* c, const double* Wp, const double* Dx, 
  * ddf_dydy runs the x stencil scaled by dy*dy (cd-n.cpp:74-79).  The first-derivative arrays are
  * fresh (zero halos) and get the periodic face exchange (centered_diffusion.cpp:22-24), so their
  * halos are the periodic wrap, or zero beyond a physical wall. */
+/* Sutherland law, Viscosity::mu_sutherland (viscosity.cpp:5-11): mu0 (T0 + C)/(T + C) (T/T0)^1.5
+ * with T = gamma p / rho and T0 = 1.  The reference defines the law but never calls it
+ * (viscosity.cpp:3 has an empty body); here T^1.5 is written T*sqrt(T) (two IEEE operations, so
+ * the GPU reproduces it bit for bit) and the viscosity derivatives feed the dmu slots that
+ * compute_residual already has (centered_diffusion.cpp:96-103). */
+double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double p) {
+    double T = gamma * p / rho;
+    return mu0 * (1.0 + C) / (T + C) * (T * sqrt(T));
+}
+
 static void diffusion(const nso_cfg* c, const double* W, double* R) {
     double  dx = c->Lx / c->nx, dy = c->Ly / c->ny;
     double* Dxx = alloc_zero(c);
@@ -694,8 +705,29 @@ static void diffusion(const nso_cfg* c, const double* W, double* R) {
     communicate(c, Dy);
     nso_fd(c, Dx, Dxy, 1, 0, dy); /* dW_dxdy = df_dy(dW_dx) */
     nso_fd(c, Dy, Dyx, 0, 0, dx); /* dW_dydx = df_dx(dW_dy) */
-    diff_residual(c, W, Dx, Dy, Dxx, Dyy, Dxy, Dyx, R);
+    double *MU = NULL, *DMUX = NULL, *DMUY = NULL;
+    if (c->viscosity == 1) {
+        /* mu on the interior and the face halos (from W there), its derivatives on the interior
+         * with the first-derivative stencil; stored in slot 0 of 5-variable arrays */
+        pad_t P = pad_of(c);
+        MU      = alloc_zero(c);
+        DMUX    = alloc_zero(c);
+        DMUY    = alloc_zero(c);
+        for (long i = 0; i < P.NX; i++)
+            for (long j = 0; j < P.NY; j++)
+                for (long k = 0; k < P.NZ; k++) {
+                    int out = (i < NGC || i >= NGC + c->nx) + (j < NGC || j >= NGC + c->ny) +
+                              (k < NGC || k >= NGC + c->nz);
+                    if (out > 1) continue;
+                    const double* w  = &AT(W, P, i, j, k, 0);
+                    AT(MU, P, i, j, k, 0) = nso_mu_sutherland(c->mu, c->suth_C, c->gamma, w[0], w[1]);
+                }
+        nso_fd(c, MU, DMUX, 0, 0, dx);
+        nso_fd(c, MU, DMUY, 1, 0, dy);
+    }
+    diff_residual(c, W, Dx, Dy, Dxx, Dyy, Dxy, Dyx, MU, DMUX, DMUY, R);
     free(Dxx), free(Dyy), free(Dx), free(Dy), free(Dxy), free(Dyx);
+    free(MU), free(DMUX), free(DMUY);
 }
 
 /* ------------------------------------------------------------------------------------------ */

@@ -37,6 +37,8 @@ typedef struct {
     int    weights;    /* NSO_W_* */
     int    diff_order; /* 0 = no diffusion, else 2, 4 or 6 (cd_constants.hpp) */
     int    bc[6];      /* NSO_BC_*, W,E,N,S,F,B */
+    int    viscosity;  /* 0: constant mu (the reference path); 1: Sutherland law (extension) */
+    double suth_C;     /* Sutherland constant over the reference temperature (T0 = 1) */
 } nso_cfg;
 
 /* number of doubles of one padded 5-variable array */
@@ -70,6 +72,7 @@ void   nso_eigenvectors(const double Wavg[5], double gamma, int dir, double L[25
  * second selects the second-derivative coefficients; order is c->diff_order. */
 void   nso_fd(const nso_cfg* c, const double* f, double* df, int dir, int second, double d);
 int    nso_num_threads(void);
+double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double p);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,8 @@ NGC = 3
 BC = {"periodic": 0, "neumann": 1}
 CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
 WEIGHTS = {"shu": 0, "borges": 1}
+VISCOSITY = {"constant": 0, "sutherland": 1}
+SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 
 
 class NsoCfg(ctypes.Structure):
@@ -28,6 +30,7 @@ class NsoCfg(ctypes.Structure):
         ("cfl", ctypes.c_double),
         ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
         ("bc", ctypes.c_int * 6),
+        ("viscosity", ctypes.c_int), ("suth_C", ctypes.c_double),
     ]
 
 
@@ -44,6 +47,8 @@ def make_cfg(case):
     c.diff_order = case["diff_order"]
     for f, name in enumerate(case["bc"]):
         c.bc[f] = BC[name]
+    c.viscosity = VISCOSITY[case.get("viscosity", "constant")]
+    c.suth_C = case.get("suth_C", SUTH_C)
     return c
 
 
@@ -105,11 +110,16 @@ class Oracle(_Lib):
         L.nso_dt.restype = ctypes.c_double
         L.nso_weno5_left.restype = None
         L.nso_weno5_right.restype = None
+        L.nso_mu_sutherland.restype = ctypes.c_double
+        L.nso_mu_sutherland.argtypes = [ctypes.c_double] * 5
 
     def dt(self, case, W):
         cfg = make_cfg(case)
         W = np.ascontiguousarray(W, dtype=np.float64)
         return self.lib.nso_dt(ctypes.byref(cfg), _ptr(W))
+
+    def mu_sutherland(self, mu0, C, gamma, rho, p):
+        return self.lib.nso_mu_sutherland(mu0, C, gamma, rho, p)
 
     def p2c(self, case, W):
         cfg = make_cfg(case)

@@ -272,3 +272,64 @@ def test_shear_and_tgv_stable(oracle):
     W2, _, dts = oracle.step(case, W, 20)
     assert np.isfinite(po.interior(W2)).all()
     assert dts.max() / dts.min() < 1.05
+
+
+# ---------------------------------------------------------------------------------------------
+# Sutherland viscosity (extension: the reference defines the law, viscosity.cpp:5-11, but never
+# calls it; these checks pin the oracle's use of it)
+# ---------------------------------------------------------------------------------------------
+
+def test_sutherland_law_closed_form(oracle):
+    mu0, C, g = 2e-3, 110.4 / 300.0, 1.4
+    # T = gamma p / rho = 1 (the reference temperature) -> mu0
+    assert oracle.mu_sutherland(mu0, C, g, 1.4, 1.0) == pytest.approx(mu0, rel=1e-15)
+    for rho, p in [(1.0, 2.0), (0.7, 0.3), (2.0, 5.0)]:
+        T = g * p / rho
+        assert oracle.mu_sutherland(mu0, C, g, rho, p) == pytest.approx(
+            mu0 * (1 + C) / (T + C) * T ** 1.5, rel=1e-14)
+
+
+def test_sutherland_uniform_temperature_equals_constant_mu(oracle):
+    # rho varies but p / rho does not: mu(T) is uniform, dmu = 0, so the residual equals the
+    # constant-viscosity residual with mu = mu(T)
+    C_ = cases_module()
+    case = _case(conv="none", diff_order=4, mu=1e-2)
+    W = C_.random_smooth_W(case, 21)
+    W[I3 + (1,)] = 0.9 * W[I3 + (0,)]
+    U = oracle.p2c(case, W)
+    _, _, Rs = oracle.rhs(dict(case, viscosity="sutherland"), U, which=2)
+    mu_T = oracle.mu_sutherland(case["mu"], po.SUTH_C, case["gamma"], 1.0, 0.9)
+    _, _, Rc = oracle.rhs(dict(case, mu=mu_T), U, which=2)
+    np.testing.assert_allclose(po.interior(Rs), po.interior(Rc), rtol=0,
+                               atol=1e-12 * np.abs(po.interior(Rc)).max())
+
+
+def test_sutherland_viscosity_gradient_term_converges(oracle):
+    # u = v = 0, uniform p, rho(x): only the heat-flux terms act on energy.  With T = gamma p/rho,
+    # the residual as written is -k [ mu' (-p rho'/rho^2) + mu (p rho'^2/rho^3 - p rho''/rho^2) ]
+    # (k = gamma/(Pr (gamma-1))) plus the y term; compare with exact derivatives: the error must
+    # fall at 4th order
+    errs = []
+    for n in (16, 32):
+        case = dict(_case(conv="none", diff_order=4, mu=1e-2, L=(1.0, 1.0, 1.0)),
+                    nx=n, ny=4, nz=4, viscosity="sutherland")
+        x = (np.arange(n) + 0.5) / n
+        rho = 1.0 + 0.3 * np.sin(2 * np.pi * x)
+        r1 = 0.3 * 2 * np.pi * np.cos(2 * np.pi * x)
+        r2 = -0.3 * (2 * np.pi) ** 2 * np.sin(2 * np.pi * x)
+        p, g, C, mu0 = 1.0, case["gamma"], po.SUTH_C, case["mu"]
+        W = np.zeros(po.padded_shape(case))
+        W[I3 + (0,)] = rho[:, None, None]
+        W[I3 + (1,)] = p
+        _, _, R = oracle.rhs(case, oracle.p2c(case, W), which=2)
+        T = g * p / rho
+        mu = mu0 * (1 + C) / (T + C) * T ** 1.5
+        dmu_dT = mu * (1.5 / T - 1.0 / (T + C))
+        mu_x = dmu_dT * (-g * p * r1 / rho ** 2)
+        k = g / (case["Pr"] * (g - 1))
+        exact = -k * (mu_x * (-p * r1 / rho ** 2) + mu * (p * r1 ** 2 / rho ** 3 - p * r2 / rho ** 2))
+        # quirk: "d2rho/dy2" is the x stencil divided by dy^2, i.e. rho'' (dx/dy)^2
+        ratio = (case["L"][0] / n) ** 2 / (case["L"][1] / case["ny"]) ** 2
+        exact += -k * mu * (-p * r2 * ratio / rho ** 2)
+        errs.append(np.abs(po.interior(R)[:, 1, 1, 1] - exact).max())
+    assert errs[0] / errs[1] > 12, errs
