@@ -11,6 +11,10 @@ reference's operator interface (paths relative to the reference's src/):
   Solver.step(n)                    TimeScheme::step (time_scheme.cpp:3-70), state on the GPU
   Solver.dt                         TimeScheme::get_dt (time_scheme.hpp:31)
 
+  Ensemble(solvers)                 many blocks advanced together: batched launches, one hipGraph
+                                    per SSP-RK3 step, tile shapes from an autotuner (no reference
+                                    counterpart; each member's result is its own Solver.step)
+
 Arrays are numpy float64 in the reference Array4d layout (NX, NY, NZ, 5), NX = nx + 6.
 There is no CPU fallback: without the HIP library or a GPU every call raises.
 """
@@ -34,7 +38,9 @@ EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len",
            "nsgpu_get_state", "nsgpu_apply_boundary", "nsgpu_convection_solve",
            "nsgpu_diffusion_solve", "nsgpu_rhs", "nsgpu_step", "nsgpu_get_dt", "nsgpu_reduce",
            "nsgpu_stream", "nsgpu_selftest", "nsgpu_timing", "nsgpu_timer_count", "nsgpu_timer_name",
-           "nsgpu_timer_read"]
+           "nsgpu_timer_read", "nsgpu_ensemble_create", "nsgpu_ensemble_destroy",
+           "nsgpu_ensemble_set_tiles", "nsgpu_ensemble_get_tiles", "nsgpu_ensemble_step",
+           "nsgpu_ensemble_tune"]
 
 
 class Config(ctypes.Structure):
@@ -93,6 +99,14 @@ def load_library(path=None):
     L.nsgpu_timing.argtypes = [P, ctypes.c_int]
     L.nsgpu_timer_name.restype = ctypes.c_char_p
     L.nsgpu_timer_read.argtypes = [P, ctypes.c_int, D, ctypes.POINTER(ctypes.c_long)]
+    I = ctypes.POINTER(ctypes.c_int)
+    L.nsgpu_ensemble_create.argtypes = [ctypes.POINTER(P), ctypes.c_int, ctypes.POINTER(P)]
+    L.nsgpu_ensemble_destroy.argtypes = [P]
+    L.nsgpu_ensemble_destroy.restype = None
+    L.nsgpu_ensemble_set_tiles.argtypes = [P, I]
+    L.nsgpu_ensemble_get_tiles.argtypes = [P, I]
+    L.nsgpu_ensemble_step.argtypes = [P, ctypes.c_int, ctypes.c_int]
+    L.nsgpu_ensemble_tune.argtypes = [P, ctypes.c_int, D, D]
     _lib = L
     return L
 
@@ -232,3 +246,55 @@ class Solver:
             self._check(self.L.nsgpu_timer_read(self.h, i, ctypes.byref(ms), ctypes.byref(n)))
             out[self.L.nsgpu_timer_name(i).decode()] = (ms.value, n.value)
         return out
+
+
+class Ensemble:
+    """Independent blocks (Solver objects on one device, same math mode and scheme options)
+    advanced together.  Members stay usable as Solvers between ensemble calls."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        self.L = load_library()
+        arr = (ctypes.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = ctypes.c_void_p()
+        rc = self.L.nsgpu_ensemble_create(arr, len(self.members), ctypes.byref(h))
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nsgpu_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def tiles(self):
+        """(n, 3) int array: kw_flux, kw_cell, march per member."""
+        t = np.zeros((len(self.members), 3), dtype=np.int32)
+        self._check(self.L.nsgpu_ensemble_get_tiles(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return t
+
+    @tiles.setter
+    def tiles(self, t):
+        t = np.ascontiguousarray(t, dtype=np.int32).reshape(len(self.members), 3)
+        self._check(self.L.nsgpu_ensemble_set_tiles(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def step(self, nsteps=1, graph=True):
+        self._check(self.L.nsgpu_ensemble_step(self.h, nsteps, 1 if graph else 0))
+
+    def tune(self, steps=3):
+        """Runs the autotuner; returns (ms per step with default tiles, with the chosen tiles)."""
+        a = np.zeros(1)
+        b = np.zeros(1)
+        self._check(self.L.nsgpu_ensemble_tune(self.h, steps, _ptr(a), _ptr(b)))
+        return float(a[0]), float(b[0])

@@ -1,7 +1,9 @@
 """Builds the HIP C-ABI library lib/libnsgpu.so for gfx950 (MI355X) with hipcc.
 
 ns_kernels.hip is compiled three times: with -ffp-contract=off (EXACT: the reference's operation
-order, bitwise), with -ffp-contract=fast (FAST) and with single-precision reconstruction (FP32).  Every HIP compile disables packed FP32
+order, bitwise), with -ffp-contract=on (FAST: fused multiply-add within each source expression, so
+every instantiation of a kernel body fuses the same operations and the single-block and ensemble
+kernels agree bitwise) and with single-precision reconstruction (FP32, same contraction).  Every HIP compile disables packed FP32
 (-Xclang -target-feature -Xclang -packed-fp32-ops).
 """
 import os
@@ -33,8 +35,8 @@ def build(verbose=False, defines=(), out=None):
     objs = []
     units = [
         ("ns_kernels.hip", "kernels_exact.o", ["-DNS_FAST=0", "-ffp-contract=off", *defines]),
-        ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=fast", *defines]),
-        ("ns_kernels.hip", "kernels_fp32.o", ["-DNS_FAST=1", "-DNS_FP32=1", "-ffp-contract=fast",
+        ("ns_kernels.hip", "kernels_fast.o", ["-DNS_FAST=1", "-ffp-contract=on", *defines]),
+        ("ns_kernels.hip", "kernels_fp32.o", ["-DNS_FAST=1", "-DNS_FP32=1", "-ffp-contract=on",
                                               *defines]),
         ("nsgpu_capi.hip", "capi.o", ["-ffp-contract=off"]),
     ]

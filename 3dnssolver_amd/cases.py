@@ -177,3 +177,26 @@ def config3(n=512):
     W, uprime, lam = isotropic_W(case, Mt=0.5, k0=4.0, seed=3)
     case["mu"] = 1.0 * uprime * lam / 100.0
     return case, W
+
+
+ENSEMBLE_EDGES = (32, 48, 64, 96, 128)
+
+
+def config5(n_blocks=64, seed=5, edges=ENSEMBLE_EDGES):
+    """Batched form: n_blocks independent periodic double shear layers on [0,1]^3, each edge
+    length drawn uniformly from `edges` (independently per axis), Mach uniform in [0.2, 0.6],
+    Re uniform in [500, 5000], perturbation amplitude uniform in [0.01, 0.1]; component-wise
+    WENO-5 (Shu) + CD4, CFL 0.5.  Returns a list of (case, W, params)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for b in range(n_blocks):
+        nx, ny, nz = (int(v) for v in rng.choice(edges, size=3))
+        M = float(rng.uniform(0.2, 0.6))
+        Re = float(rng.uniform(500.0, 5000.0))
+        amp = float(rng.uniform(0.01, 0.1))
+        c0 = sound_speed(M)
+        case = dict(base_case(nx, ny, nz), conv="weno5", weights="shu", diff_order=4,
+                    mu=1.0 * M * c0 * 1.0 / Re)
+        W = shear_W(case, M=M, amp=amp, seed=int(rng.integers(1 << 30)))
+        out.append((case, W, dict(M=M, Re=Re, amp=amp)))
+    return out

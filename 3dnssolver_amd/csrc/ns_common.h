@@ -55,6 +55,75 @@ struct StageArgs {
     int           stage;  // 0, 1, 2
 };
 
+// ------------------------------------------------------------------------------------------
+// ensembles: independent blocks of different sizes advanced by batched launches.  Each kernel
+// family runs once for all blocks; workgroup w of a launch belongs to the block b with
+// start[fam][b] <= w < start[fam][b + 1] and is tile (w - start) of that block's local grid.
+// ------------------------------------------------------------------------------------------
+enum EnsFam {
+    EF_STEP_BEGIN, EF_HALO, EF_FLUX_X, EF_FLUX_Y, EF_FLUX_Z, EF_PLANE_X, EF_PLANE_Y, EF_PLANE_Z,
+    EF_DERIV1, EF_DERIV_HALO, EF_RESIDUAL, EF_COUNT
+};
+
+// tile shape of one block: lanes along k of the face-flux waves (64 / kw_flux rows per wave),
+// lanes along k of the 256-thread cell kernels (256 / kw_cell rows), planes per marching chunk
+struct EnsTiles {
+    int kw_flux, kw_cell, march;
+};
+
+struct EnsBlock {
+    Geo   g;
+    Bnd   b;
+    Phys  ph;
+    double* W;       // primitive state: read by the step prologue, written by the last stage
+    double* Un;      // step-start conservative state
+    double* Ua;      // stage states
+    double* Ub;
+    double* Ws[2];   // stage primitives
+    double* F[3];    // face fluxes
+    double* Dx;      // first derivatives
+    double* Dy;
+    unsigned long long* dtmin;
+    double*  dtime;
+    EnsTiles t;
+    int      grid[EF_COUNT][3];  // local grid of each family (x fastest); 0 = no work
+};
+
+struct EnsTable {
+    const EnsBlock* blk;    // device array [n]
+    const int*      start;  // device array [EF_COUNT][n + 1] of workgroup prefix sums
+    int             n;
+};
+
+inline int ens_cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// local grid of every family for one block (must match the index decode of the kernel bodies)
+inline void ens_grids(EnsBlock& e) {
+    const Geo& g  = e.g;
+    const int  kf = e.t.kw_flux, kc = e.t.kw_cell;
+    auto set = [&](int f, int x, int y, int z) {
+        e.grid[f][0] = x, e.grid[f][1] = y, e.grid[f][2] = z;
+    };
+    long cells = (long)g.nx * g.ny * g.nz;
+    set(EF_STEP_BEGIN, (int)((cells + 2047) / 2048), 1, 1);
+    int m1 = g.ny > g.nx ? g.ny : g.nx;
+    int mx = g.nz > 3 * g.ny ? g.nz : 3 * g.ny;
+    set(EF_HALO, ens_cdiv(mx, 64), 3 * ens_cdiv(m1, 4), 6);
+    const bool conv = e.ph.conv != 0;
+    set(EF_FLUX_X, conv ? ens_cdiv(g.nz, kf) : 0, ens_cdiv(g.ny, 64 / kf), ens_cdiv(g.nx, 8));
+    set(EF_FLUX_Y, conv ? ens_cdiv(g.nz, kf) : 0, ens_cdiv(g.nx, 64 / kf), ens_cdiv(g.ny, 8));
+    set(EF_FLUX_Z, (conv && g.dim == 3) ? ens_cdiv(g.nz, kf) : 0, ens_cdiv(g.ny, 8 * (64 / kf)),
+        g.nx);
+    set(EF_PLANE_X, conv ? ens_cdiv(g.nz, 64) : 0, ens_cdiv(g.ny, 4), 1);
+    set(EF_PLANE_Y, conv ? ens_cdiv(g.nz, 64) : 0, ens_cdiv(g.nx, 4), 1);
+    set(EF_PLANE_Z, (conv && g.dim == 3) ? ens_cdiv(g.ny, 64) : 0, ens_cdiv(g.nx, 4), 1);
+    const bool diff = e.ph.diff_order != 0;
+    set(EF_DERIV1, diff ? ens_cdiv(g.nz, kc) : 0, ens_cdiv(g.ny, 256 / kc),
+        ens_cdiv(g.nx, e.t.march));
+    set(EF_DERIV_HALO, diff ? ens_cdiv(g.nz, 64) : 0, ens_cdiv(m1, 4), 4 * NGC);
+    set(EF_RESIDUAL, ens_cdiv(g.nz, kc), ens_cdiv(g.ny, 256 / kc), ens_cdiv(g.nx, e.t.march));
+}
+
 // kernel launchers of one math mode (EXACT or FAST); all enqueue on `s`
 struct Launchers {
     void (*reset_min)(unsigned long long* bits, hipStream_t s);
@@ -84,6 +153,9 @@ struct Launchers {
     void (*zero_flags)(unsigned* flags, hipStream_t s);
     // counts the operand pairs where the division routine of the kernels differs from `/`
     void (*selftest_div)(long n, unsigned long long seed, unsigned long long* count, hipStream_t s);
+    // one SSP-RK3 step of every block of an ensemble; total[f] = workgroups of family f; the
+    // template choices (convection, weights, diffusion order, viscosity law) come from `ph`
+    void (*ens_step)(const EnsTable& t, const int* total, const Phys& ph, hipStream_t s);
 };
 
 namespace exact {

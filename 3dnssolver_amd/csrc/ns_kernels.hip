@@ -7,9 +7,10 @@
 //       reference.  Multiplications by the structural zeros / ones of the axis-aligned
 //       eigenvector bases are skipped: x + (+-0) == x and 1*x == x exactly, so only the sign of
 //       an exact zero can differ.
-//   NS_FAST=1 with -ffp-contract=fast  -> namespace nsgpu::fast
-//       fused multiply-add, reciprocals hoisted out of the stencils, no square root in the flux
-//       split, one division per WENO reconstruction.
+//   NS_FAST=1 with -ffp-contract=on    -> namespace nsgpu::fast
+//       fused multiply-add (within each source expression, so every instantiation of a body
+//       fuses alike), reciprocals hoisted out of the stencils, no square root in the flux split,
+//       one division per WENO reconstruction.
 //   NS_FAST=1 NS_FP32=1                 -> namespace nsgpu::fp32
 //       the FAST arithmetic with the reconstruction (flux split, eigen system, projections,
 //       WENO) and the viscous fluxes in single precision; the state, the stage accumulation,
@@ -28,9 +29,6 @@
 // builds; float for the FP32 build (state, stage accumulation and reductions stay double)
 #ifndef NS_REAL
 #define NS_REAL double
-#endif
-#ifndef NS_XCD_MAP
-#define NS_XCD_MAP 0
 #endif
 #if NS_FP32
 #define NS_MODE fp32
@@ -425,23 +423,27 @@ constexpr int TILE_L = 64;   // lanes along k
 #define NS_TILE_WAVES 1      // minimum waves per SIMD requested for the tile kernels
 #endif
 
-// x or y sweep: lanes along k (unit stride), 8 waves at 8 consecutive sweep positions.  The split
-// values of the 13 cells the tile needs are computed once into LDS.
+// x or y sweep: 8 waves at 8 consecutive sweep positions; a wave covers kw consecutive k (unit
+// stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
+// values of the 13 cells the tile needs are computed once into LDS.  (bx, by, bz) = (k tile,
+// row tile, sweep tile).
 template <int DIR, int CONV, bool BORGES>
-__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
-                                                      const double* __restrict__ W,
-                                                      double* __restrict__ f, Geo g,
-                                                      real gamma) {
+__device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
+                                                  const double* __restrict__ W,
+                                                  double* __restrict__ f, const Geo& g,
+                                                  real gamma, int bx, int by, int bz, int kw,
+                                                  real* __restrict__ tile) {
     constexpr int NC = TILE_S + 5;
-    __shared__ real tile[NC * 10 * TILE_L];
     const int lane = threadIdx.x, w = threadIdx.y;
-    const int k    = NGC + blockIdx.x * TILE_L + lane;
-    const int o    = NGC + blockIdx.y;              // j for the x sweep, i for the y sweep
-    const int s0   = blockIdx.z * TILE_S;           // first face of the tile (0-based)
+    const int rows = TILE_L / kw;
+    const int k    = NGC + bx * kw + (lane & (kw - 1));
+    const int o    = NGC + by * rows + lane / kw;  // j for the x sweep, i for the y sweep
+    const int s0   = bz * TILE_S;                   // first face of the tile (0-based)
     const int n    = (DIR == 0) ? g.nx : g.ny;
+    const int no   = (DIR == 0) ? g.ny : g.nx;
     const long ss  = (DIR == 0) ? g.sx : g.sy;      // stride along the sweep
     const long so  = (DIR == 0) ? g.sy : g.sx;      // stride of the other index
-    const bool kin = (k < NGC + g.nz);
+    const bool kin = (k < NGC + g.nz) && (o < NGC + no);
     // cells s0 .. s0+12 (padded sweep index) feed faces s0+2 .. s0+9
     for (int t = w; t < NC; t += TILE_S) {
         int cs = s0 + t;
@@ -469,23 +471,38 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const doubl
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-// z sweep: lanes along k = the sweep (64 faces per row segment), 8 rows (j) per block; each row
-// stages the split values of its 69 cells in LDS.
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
+                                                      const double* __restrict__ W,
+                                                      double* __restrict__ f, Geo g,
+                                                      real gamma) {
+    __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
+    flux_tile_xy_body<DIR, CONV, BORGES>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
+                                         TILE_L, tile);
+}
+
+// z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
+// each, a block 8 waves; each row stages the split values of its kw + 5 cells in LDS.
+// (bx, by, bz) = (segment, row tile, i).
+constexpr int ZLDS_WAVE_MAX = 4 * (16 + 5) * 10;  // the largest per-wave need (kw = 16)
 template <int CONV, bool BORGES>
-__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
-                                                     const double* __restrict__ W,
-                                                     double* __restrict__ f, Geo g,
-                                                     real gamma) {
-    constexpr int NC = TILE_L + 5;
-    __shared__ real tile[8][NC * 10];
+__device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
+                                                 const double* __restrict__ W,
+                                                 double* __restrict__ f, const Geo& g,
+                                                 real gamma, int bx, int by, int bz, int kw,
+                                                 real* __restrict__ lds) {
+    const int NC   = kw + 5;
+    const int sub  = TILE_L / kw;                   // rows per wave
     const int lane = threadIdx.x, r = threadIdx.y;
-    const int j    = NGC + blockIdx.y * 8 + r;
-    const int i    = NGC + blockIdx.z;
-    const int f0   = blockIdx.x * TILE_L;  // first face of the segment (0-based)
-    const bool rowok = (j < NGC + g.ny);   // every wave reaches the barrier below
+    const int kk   = lane & (kw - 1);
+    const int sr   = lane / kw;
+    const int j    = NGC + (by * 8 + r) * sub + sr;
+    const int i    = NGC + bz;
+    const int f0   = bx * kw;                       // first face of the segment (0-based)
+    const bool rowok = (j < NGC + g.ny);            // every wave reaches the barrier below
     const long row = (long)i * g.sx + (long)j * g.sy;
-    real*      T   = tile[r];
-    for (int t = lane; t < NC; t += TILE_L) {
+    real*      T   = lds + (r * sub + sr) * (NC * 10);
+    for (int t = kk; t < NC; t += kw) {
         int cs = f0 + t;  // padded k of the cell
         if (rowok && cs <= g.nz + 2 * NGC - 1) {
             real sv[10];
@@ -494,8 +511,8 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
             for (int v = 0; v < 10; v++) T[v * NC + t] = sv[v];
         }
     }
-    __syncthreads();  // rows are independent; each wave only reads its own row
-    const int fidx = f0 + lane;
+    __syncthreads();  // rows are independent; each wave only reads its own rows
+    const int fidx = f0 + kk;
     if (!rowok || fidx >= g.nz) return;
     const int  kf = fidx + NGC - 1;
     const long c  = row + kf;
@@ -506,23 +523,35 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
     }
     struct {
         const real* b;
-        __device__ real operator()(int ii, int v) const { return b[v * NC + ii]; }
-    } st{&T[lane]};
+        int         nc;
+        __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
+    } st{&T[kk], NC};
     real out[5];
     face_flux<2, CONV, BORGES>(st, Wl, Wr, gamma, out);
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-// one plane of faces (sweep index `i`), one thread per face, split values in registers
+template <int CONV, bool BORGES>
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
+                                                     const double* __restrict__ W,
+                                                     double* __restrict__ f, Geo g,
+                                                     real gamma) {
+    __shared__ real tile[8 * (TILE_L + 5) * 10];
+    flux_tile_z_body<CONV, BORGES>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z, TILE_L,
+                                   tile);
+}
+
+// one plane of faces (sweep index `i`), one thread per face, split values in registers;
+// block (64, 4) over the tangential indices, (bx, by) = (fast, slow) tile
 template <int DIR, int CONV, bool BORGES>
-__global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
-                                                    const double* __restrict__ W,
-                                                    double* __restrict__ f, Geo g, real gamma,
-                                                    int i) {
+__device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
+                                                const double* __restrict__ W,
+                                                double* __restrict__ f, const Geo& g, real gamma,
+                                                int i, int bx, int by) {
     // tangential indices (t1 slow, t2 fast) over the interior
-    int t2 = threadIdx.x + blockIdx.x * blockDim.x;
-    int t1 = threadIdx.y + blockIdx.y * blockDim.y;
+    int t2 = threadIdx.x + bx * 64;
+    int t1 = threadIdx.y + by * 4;
     int n1 = (DIR == 0) ? g.ny : g.nx;
     int n2 = (DIR == 2) ? g.ny : g.nz;
     if (t1 >= n1 || t2 >= n2) return;
@@ -543,6 +572,14 @@ __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U
     face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
+}
+
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
+                                                    const double* __restrict__ W,
+                                                    double* __restrict__ f, Geo g, real gamma,
+                                                    int i) {
+    flux_plane_body<DIR, CONV, BORGES>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -612,16 +649,15 @@ __global__ void k_reset_min(unsigned long long* bits) {
 __global__ void k_zero_flags(unsigned* flags) { *flags = 0u; }
 
 // step prologue: dt from W (time_scheme.cpp:16, :98-136), U = p2c(W) (:25), Uprev = U (:27),
-// and the stage-1 primitives c2p(U) (:36).  Grid-stride over the interior.
-__global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W,
-                                                    double* __restrict__ Ua,
-                                                    double* __restrict__ Un,
-                                                    double* __restrict__ Wst,
-                                                    unsigned long long* dtmin, Geo g, double gamma) {
+// and the stage-1 primitives c2p(U) (:36).  Workgroup `wg` of `nwg` strides over the interior.
+__device__ __forceinline__ void step_begin_body(const double* __restrict__ W,
+                                                double* __restrict__ Ua, double* __restrict__ Un,
+                                                double* __restrict__ Wst,
+                                                unsigned long long* dtmin, const Geo& g,
+                                                double gamma, int wg, int nwg) {
     long   n  = (long)g.nx * g.ny * g.nz;
     double dt = 1E300;
-    for (long t = threadIdx.x + (long)blockIdx.x * blockDim.x; t < n;
-         t += (long)gridDim.x * blockDim.x) {
+    for (long t = threadIdx.x + (long)wg * blockDim.x; t < n; t += (long)nwg * blockDim.x) {
         int    k = (int)(t % g.nz);
         long   r = t / g.nz;
         int    j = (int)(r % g.ny);
@@ -644,33 +680,43 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
     block_min_atomic(dt, dtmin);
 }
 
+__global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W,
+                                                    double* __restrict__ Ua,
+                                                    double* __restrict__ Un,
+                                                    double* __restrict__ Wst,
+                                                    unsigned long long* dtmin, Geo g, double gamma) {
+    step_begin_body(W, Ua, Un, Wst, dtmin, g, gamma, blockIdx.x, gridDim.x);
+}
+
 // ------------------------------------------------------------------------------------------
 // halo fill: Neumann faces (boundary_neumann.cpp:22-132) and the periodic face exchange of one
 // rank (communicator.cpp:16-144 with the slicers of array_4d.cpp:39-164).  The six face regions
 // are disjoint and read interior cells only, so one launch covers them in any order.  The same
 // copy applied to the primitive planes gives c2p(U) on the halo bit for bit (c2p is per cell).
 // ------------------------------------------------------------------------------------------
-__global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
-    int face = blockIdx.z;
+// block (64, 4); (bx, by, bz): bz = face, by = (row tile, layer), bx = lane tile
+__device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, const Geo& g,
+                                          const Bnd& b, int bx, int by, int bz) {
+    int face = bz;
     int axis = face >> 1;
     // face order W,E,N,S,F,B: low side for W (0), N (2), B (5); high side for E, S, F
     bool low = (face == 0 || face == 2 || face == 5);
     int  n   = (axis == 0) ? g.nx : (axis == 1) ? g.ny : g.nz;
     int  t1, t2, h;
     if (axis < 2) {
-        // x / y faces: lanes along k (unit stride), one layer per blockIdx.y slice
-        t2 = threadIdx.x + blockIdx.x * blockDim.x;
-        t1 = threadIdx.y + (blockIdx.y / NGC) * blockDim.y;
-        h  = blockIdx.y % NGC;
+        // x / y faces: lanes along k (unit stride), one layer per `by` slice
+        t2 = threadIdx.x + bx * 64;
+        t1 = threadIdx.y + (by / NGC) * 4;
+        h  = by % NGC;
         int n1 = (axis == 0) ? g.ny : g.nx;
         if (t1 >= n1 || t2 >= g.nz) return;
     } else {
         // z faces: the 3 halo cells of a row are contiguous; lanes over (row j, layer h)
-        int lin = threadIdx.x + blockIdx.x * blockDim.x;  // j * NGC + h
+        int lin = threadIdx.x + bx * 64;  // j * NGC + h
         t2      = lin / NGC;
         h       = lin % NGC;
-        t1      = threadIdx.y + (blockIdx.y / NGC) * blockDim.y;
-        if ((blockIdx.y % NGC) != 0 || t1 >= g.nx || t2 >= g.ny) return;
+        t1      = threadIdx.y + (by / NGC) * 4;
+        if ((by % NGC) != 0 || t1 >= g.nx || t2 >= g.ny) return;
     }
     int dst = low ? h : NGC + n + h;
     int src;
@@ -694,6 +740,10 @@ __global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
     for (int q = 0; q < nB; q++) B[cd + q * g.P] = B[cs + q * g.P];
 }
 
+__global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
+    halo_body(A, nA, B, nB, g, b, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
 // W = c2p(U) on the interior and the face halos (edges and corners are never read)
 __global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
                                                    double* __restrict__ W, Geo g, double gamma) {
@@ -711,27 +761,6 @@ __global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
     c2p(u, gamma, w);
 #pragma unroll
     for (int q = 0; q < 5; q++) W[c + q * g.P] = w[q];
-}
-
-// ------------------------------------------------------------------------------------------
-// XCD-aware block order.  Workgroups are dispatched round-robin over the 8 XCDs of an MI355X and
-// each XCD has its own L2; renumbering so that XCD x runs a contiguous range of tiles (planes i
-// slowest) keeps the x-neighbour planes of a stencil in the L2 of the XCD that reads them.
-// Returns (b2, b1, b0) tile coordinates for grid (n2, n1, n0), b0 slowest.
-// ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void xcd_tile(int& b2, int& b1, int& b0) {
-#if NS_XCD_MAP
-    const unsigned n2 = gridDim.x, n1 = gridDim.y, n0 = gridDim.z;
-    const unsigned N  = n2 * n1 * n0;
-    const unsigned b  = blockIdx.x + n2 * (blockIdx.y + n1 * blockIdx.z);
-    const unsigned q = N / 8, r = N % 8, x = b % 8, l = b / 8;
-    const unsigned t = x * q + (x < r ? x : r) + l;  // bijection of [0, N)
-    b2 = (int)(t % n2);
-    b1 = (int)((t / n2) % n1);
-    b0 = (int)(t / (n2 * n1));
-#else
-    b2 = blockIdx.x, b1 = blockIdx.y, b0 = blockIdx.z;
-#endif
 }
 
 // ------------------------------------------------------------------------------------------
@@ -776,17 +805,19 @@ __device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
 // fetched from HBM once instead of once per stencil point.
 constexpr int MARCH = 16;
 
-// first derivatives of rho, p, u, v along x and y on the interior
+// first derivatives of rho, p, u, v along x and y on the interior.  256 threads: kw lanes along
+// k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i)
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
-                                                double* __restrict__ Dx, double* __restrict__ Dy,
-                                                Geo g) {
-    constexpr int H  = ORDER / 2;
-    const int     k  = threadIdx.x + blockIdx.x * blockDim.x;
-    const int     j  = threadIdx.y + blockIdx.y * blockDim.y;
-    const int     i0 = blockIdx.z * MARCH;
+__device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double* __restrict__ Dx,
+                                            double* __restrict__ Dy, const Geo& g, int bx, int by,
+                                            int bz, int kw, int march) {
+    constexpr int H   = ORDER / 2;
+    const int     tid = threadIdx.x + threadIdx.y * 64;
+    const int     k   = (tid & (kw - 1)) + bx * kw;
+    const int     j   = tid / kw + by * (256 / kw);
+    const int     i0  = bz * march;
     if (k >= g.nz || j >= g.ny) return;
-    const int  i1 = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const int  i1 = (i0 + march < g.nx) ? i0 + march : g.nx;
     const long c0 = (long)(j + NGC) * g.sy + (k + NGC);
     real     win[4][2 * H + 1];
 #pragma unroll
@@ -811,15 +842,24 @@ __global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
     }
 }
 
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
+                                                double* __restrict__ Dx, double* __restrict__ Dy,
+                                                Geo g) {
+    deriv1_body<ORDER>(W, Dx, Dy, g, blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
+}
+
 // periodic exchange of the derivative halos that the cross derivatives read: the j halos of
 // dW/dx and the i halos of dW/dy (centered_diffusion.cpp:22-24).  At a physical wall the halos
 // stay zero: the arrays are zeroed once at allocation and never written there.
-__global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
-    int  k    = threadIdx.x + blockIdx.x * blockDim.x;
-    int  t1   = threadIdx.y + blockIdx.y * blockDim.y;
-    int  side = blockIdx.z / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
-    int  h    = blockIdx.z % NGC;
-    bool low  = ((blockIdx.z / NGC) & 1) == 0;
+// block (64, 4); (bx, by, bz) = (k tile, row tile, (side, low/high, layer))
+__device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Geo& g,
+                                                const Bnd& b, int bx, int by, int bz) {
+    int  k    = threadIdx.x + bx * 64;
+    int  t1   = threadIdx.y + by * 4;
+    int  side = bz / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
+    int  h    = bz % NGC;
+    bool low  = ((bz / NGC) & 1) == 0;
     if (k >= g.nz) return;
     if (side == 0) {
         if (!b.periodic[1] || t1 >= g.nx) return;
@@ -836,6 +876,10 @@ __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
         for (int q = 0; q < 4; q++)
             Dy[base + (long)dst * g.sx + q * g.P] = Dy[base + (long)src * g.sx + q * g.P];
     }
+}
+
+__global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
+    deriv_halo_body(Dx, Dy, g, b, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
@@ -961,19 +1005,22 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
 // 173-176), then now = prev*m1 + now*m2 + R*m3 (TimeScheme::substep, time_scheme.cpp:72-96)
 // and the primitives of the new state (eq_of_state.cpp:61-87).  Without a stage (st.Uout null)
 // R is stored instead (operator-level calls).
+// 256 threads as in deriv1_body: (bx, by, bz) = (k tile, j tile, chunk of `march` planes i)
 template <int ORDER, bool SQ, bool SUTH>
-__global__ __launch_bounds__(256) void k_residual(
+__device__ __forceinline__ void residual_body(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
-    const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
-    const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
-    constexpr int O  = (ORDER > 0) ? ORDER : 2;  // window size - 1 (unused without diffusion)
-    constexpr int H  = O / 2;
-    const int     k  = threadIdx.x + blockIdx.x * blockDim.x;
-    const int     j  = threadIdx.y + blockIdx.y * blockDim.y;
-    const int     i0 = blockIdx.z * MARCH;
+    const double* __restrict__ Rin, double* __restrict__ Rout, const StageArgs& st,
+    const unsigned long long* __restrict__ dtmin, int which, const Geo& g, const Phys& ph, int bx,
+    int by, int bz, int kw, int march) {
+    constexpr int O   = (ORDER > 0) ? ORDER : 2;  // window size - 1 (unused without diffusion)
+    constexpr int H   = O / 2;
+    const int     tid = threadIdx.x + threadIdx.y * 64;
+    const int     k   = (tid & (kw - 1)) + bx * kw;
+    const int     j   = tid / kw + by * (256 / kw);
+    const int     i0  = bz * march;
     if (k >= g.nz || j >= g.ny) return;
-    const int  i1   = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const int  i1   = (i0 + march < g.nx) ? i0 + march : g.nx;
     const long P    = g.P;
     const long c0   = (long)(j + NGC) * g.sy + (k + NGC);
     const bool conv = (which & 1) && ph.conv;
@@ -1081,6 +1128,16 @@ __global__ __launch_bounds__(256) void k_residual(
 #pragma unroll
         for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
     }
+}
+
+template <int ORDER, bool SQ, bool SUTH>
+__global__ __launch_bounds__(256) void k_residual(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
+    const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
+    const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
+    residual_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Dy, Rin, Rout, st, dtmin, which, g, ph,
+                                   blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
 }
 
 __global__ void k_time_accum(double* t, const unsigned long long* dtmin, double cfl) {
@@ -1202,6 +1259,152 @@ __global__ void k_selftest_div(long n, unsigned long long seed, unsigned long lo
         if (!(x == y)) bad++;
     }
     if (bad) atomicAdd(count, bad);
+}
+
+// ------------------------------------------------------------------------------------------
+// ensemble (batched) kernels: one launch per family covers every block of an ensemble; each
+// workgroup finds its block by a binary search over the family's workgroup prefix sums and runs
+// the same body as the single-block kernel on its tile.  Stage buffers follow the single-block
+// step: stage s reads U_s = (Un, Ua, Ub)[s] with primitives (Ws0, Ws1, Ws0)[s] and writes
+// (Ua, Ub, Ua)[s] with primitives (Ws1, Ws0, W)[s].
+// ------------------------------------------------------------------------------------------
+// returns the block of this workgroup.  The kernels copy the descriptor fields they use into
+// locals before any store, so the compiler keeps them in scalar registers.
+__device__ __forceinline__ int ens_locate(const EnsTable& T, int fam, int& bx, int& by, int& bz) {
+    const int* st = T.start + fam * (T.n + 1);
+    const int  w  = blockIdx.x;
+    int        lo = 0, hi = T.n;  // st[0] = 0 <= w < st[n]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (st[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    const int l  = w - st[lo];
+    const int gx = T.blk[lo].grid[fam][0], gy = T.blk[lo].grid[fam][1];
+    bx = l % gx;
+    by = (l / gx) % gy;
+    bz = l / (gx * gy);
+    return lo;
+}
+
+__device__ __forceinline__ double* ens_u(const EnsBlock* E, int s) {
+    return s == 0 ? E->Un : (s == 1 ? E->Ua : E->Ub);
+}
+__device__ __forceinline__ double* ens_w(const EnsBlock* E, int s) {
+    return (s & 1) ? E->Ws[1] : E->Ws[0];
+}
+
+__global__ void kb_reset_min(EnsTable T) {
+    int b = threadIdx.x + blockIdx.x * blockDim.x;
+    if (b < T.n) *T.blk[b].dtmin = (unsigned long long)__double_as_longlong(1E10);
+}
+
+__global__ void kb_time_accum(EnsTable T) {
+    int b = threadIdx.x + blockIdx.x * blockDim.x;
+    if (b >= T.n) return;
+    const EnsBlock* E = T.blk + b;
+    *E->dtime = *E->dtime + __longlong_as_double((long long)*E->dtmin) * E->ph.cfl;
+}
+
+__global__ __launch_bounds__(256) void kb_step_begin(EnsTable T, double* Ua) {
+    int             bx, by, bz;
+    const EnsBlock* E   = T.blk + ens_locate(T, EF_STEP_BEGIN, bx, by, bz);
+    const Geo       g   = E->g;
+    const double    gam = E->ph.gamma;
+    const int       nwg = E->grid[EF_STEP_BEGIN][0];
+    double *W = E->W, *Un = E->Un, *Ws = E->Ws[0];
+    unsigned long long* dtmin = E->dtmin;
+    step_begin_body(W, Ua, Un, Ws, dtmin, g, gam, bx, nwg);
+}
+
+__global__ void kb_halo(EnsTable T, int s) {
+    int             bx, by, bz;
+    const EnsBlock* E  = T.blk + ens_locate(T, EF_HALO, bx, by, bz);
+    const Geo       g  = E->g;
+    const Bnd       bd = E->b;
+    double *U = ens_u(E, s), *Wp = ens_w(E, s);
+    halo_body(U, 5, Wp, 5, g, bd, bx, by, bz);
+}
+
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T, int s) {
+    __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
+    int             bx, by, bz;
+    const EnsBlock* E   = T.blk + ens_locate(T, DIR == 0 ? EF_FLUX_X : EF_FLUX_Y, bx, by, bz);
+    const Geo       g   = E->g;
+    const real      gam = (real)E->ph.gamma;
+    const int       kw  = E->t.kw_flux;
+    const double *U = ens_u(E, s), *Wp = ens_w(E, s);
+    double*       f = E->F[DIR];
+    flux_tile_xy_body<DIR, CONV, BORGES>(U, Wp, f, g, gam, bx, by, bz, kw, tile);
+}
+
+template <int CONV, bool BORGES>
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T, int s) {
+    __shared__ real tile[8 * ZLDS_WAVE_MAX];
+    int             bx, by, bz;
+    const EnsBlock* E   = T.blk + ens_locate(T, EF_FLUX_Z, bx, by, bz);
+    const Geo       g   = E->g;
+    const real      gam = (real)E->ph.gamma;
+    const int       kw  = E->t.kw_flux;
+    const double *U = ens_u(E, s), *Wp = ens_w(E, s);
+    double*       f = E->F[2];
+    flux_tile_z_body<CONV, BORGES>(U, Wp, f, g, gam, bx, by, bz, kw, tile);
+}
+
+template <int DIR, int CONV, bool BORGES>
+__global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
+    int             bx, by, bz;
+    const EnsBlock* E   = T.blk + ens_locate(T, EF_PLANE_X + DIR, bx, by, bz);
+    const Geo       g   = E->g;
+    const real      gam = (real)E->ph.gamma;
+    const int       n   = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
+    const double *U = ens_u(E, s), *Wp = ens_w(E, s);
+    double*       f = E->F[DIR];
+    flux_plane_body<DIR, CONV, BORGES>(U, Wp, f, g, gam, NGC + n - 1, bx, by);
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void kb_deriv1(EnsTable T, int s) {
+    int             bx, by, bz;
+    const EnsBlock* E  = T.blk + ens_locate(T, EF_DERIV1, bx, by, bz);
+    const Geo       g  = E->g;
+    const int       kw = E->t.kw_cell, march = E->t.march;
+    const double*   Wp = ens_w(E, s);
+    double *Dx = E->Dx, *Dy = E->Dy;
+    deriv1_body<ORDER>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
+}
+
+__global__ void kb_deriv_halo(EnsTable T) {
+    int             bx, by, bz;
+    const EnsBlock* E  = T.blk + ens_locate(T, EF_DERIV_HALO, bx, by, bz);
+    const Geo       g  = E->g;
+    const Bnd       bd = E->b;
+    double *Dx = E->Dx, *Dy = E->Dy;
+    deriv_halo_body(Dx, Dy, g, bd, bx, by, bz);
+}
+
+// which / Rin / Rout are launch arguments (3, null, null) rather than constants, so that the body
+// keeps the control flow of the single-block kernel
+template <int ORDER, bool SUTH>
+__global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
+                                                   const double* Rin, double* Rout) {
+    int             bx, by, bz;
+    const EnsBlock* E  = T.blk + ens_locate(T, EF_RESIDUAL, bx, by, bz);
+    const Geo       g  = E->g;
+    const Phys      ph = E->ph;
+    const int       kw = E->t.kw_cell, march = E->t.march;
+    const double *fx = E->F[0], *fy = E->F[1], *fz = E->F[2], *Dx = E->Dx, *Dy = E->Dy;
+    const double* Wp = ens_w(E, s);
+    StageArgs     a{E->Un, ens_u(E, s), s == 1 ? E->Ub : E->Ua,
+                    s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s};
+    const unsigned long long* dtmin = E->dtmin;
+    if (g.square_xy)
+        residual_body<ORDER, true, SUTH>(fx, fy, fz, Wp, Dx, Dy, Rin, Rout, a, dtmin, which, g,
+                                         ph, bx, by, bz, kw, march);
+    else
+        residual_body<ORDER, false, SUTH>(fx, fy, fz, Wp, Dx, Dy, Rin, Rout, a, dtmin, which, g,
+                                          ph, bx, by, bz, kw, march);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1356,11 +1559,58 @@ static void l_selftest_div(long n, unsigned long long seed, unsigned long long* 
     k_selftest_div<<<1024, 256, 0, s>>>(n, seed, count);
 }
 
+template <int CONV, bool B>
+static void ens_fluxes(const EnsTable& T, const int* total, int s, hipStream_t st) {
+    dim3 tb(TILE_L, 8, 1), pb(64, 4, 1);
+    if (total[EF_FLUX_X]) kb_flux_tile_xy<0, CONV, B><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_X]) kb_flux_plane<0, CONV, B><<<total[EF_PLANE_X], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Y]) kb_flux_tile_xy<1, CONV, B><<<total[EF_FLUX_Y], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Y]) kb_flux_plane<1, CONV, B><<<total[EF_PLANE_Y], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Z]) kb_flux_tile_z<CONV, B><<<total[EF_FLUX_Z], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Z]) kb_flux_plane<2, CONV, B><<<total[EF_PLANE_Z], pb, 0, st>>>(T, s);
+}
+
+template <int ORDER, bool SUTH>
+static void ens_stage(const EnsTable& T, const int* total, const Phys& ph, int s, hipStream_t st) {
+    kb_halo<<<total[EF_HALO], dim3(64, 4, 1), 0, st>>>(T, s);
+    if (ph.conv == 1) {
+        if (ph.borges) ens_fluxes<1, true>(T, total, s, st);
+        else ens_fluxes<1, false>(T, total, s, st);
+    } else if (ph.conv == 2) {
+        if (ph.borges) ens_fluxes<2, true>(T, total, s, st);
+        else ens_fluxes<2, false>(T, total, s, st);
+    }
+    if constexpr (ORDER > 0) {
+        kb_deriv1<ORDER><<<total[EF_DERIV1], 256, 0, st>>>(T, s);
+        kb_deriv_halo<<<total[EF_DERIV_HALO], dim3(64, 4, 1), 0, st>>>(T);
+    }
+    kb_residual<ORDER, SUTH><<<total[EF_RESIDUAL], 256, 0, st>>>(T, s, 3, nullptr, nullptr);
+}
+
+template <int ORDER, bool SUTH>
+static void ens_step_t(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {
+    const unsigned nb = (unsigned)((T.n + 63) / 64);
+    kb_reset_min<<<nb, 64, 0, st>>>(T);
+    kb_step_begin<<<total[EF_STEP_BEGIN], 256, 0, st>>>(T, nullptr);
+    for (int s = 0; s < 3; s++) ens_stage<ORDER, SUTH>(T, total, ph, s, st);
+    kb_time_accum<<<nb, 64, 0, st>>>(T);
+}
+
+static void l_ens_step(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {
+    const bool su = ph.suth != 0;
+    switch (ph.diff_order) {
+    case 2: su ? ens_step_t<2, true>(T, total, ph, st) : ens_step_t<2, false>(T, total, ph, st); break;
+    case 4: su ? ens_step_t<4, true>(T, total, ph, st) : ens_step_t<4, false>(T, total, ph, st); break;
+    case 6: su ? ens_step_t<6, true>(T, total, ph, st) : ens_step_t<6, false>(T, total, ph, st); break;
+    default: su ? ens_step_t<0, true>(T, total, ph, st) : ens_step_t<0, false>(T, total, ph, st); break;
+    }
+}
+
 const Launchers& launchers() {
     static const Launchers L = {l_reset_min,    l_step_begin, l_halo,       l_c2p_faces,
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
-                                l_time_accum,   l_zero_flags, l_selftest_div};
+                                l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step};
     return L;
 }
 

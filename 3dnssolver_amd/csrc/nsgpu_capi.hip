@@ -9,9 +9,12 @@
 //   (the last stage also writes W = c2p(U); time += dt on the device)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <new>
+#include <vector>
 
 #include "../../include/nsgpu.h"
 #include "ns_common.h"
@@ -119,9 +122,12 @@ int validate(const nsgpu_config* c) {
     return NSGPU_OK;
 }
 
-int alloc(double** p, size_t bytes) {
+// zero-filled device buffer.  The fill is ordered on the context's stream: that stream is
+// non-blocking, so a fill on the null stream could still be running when the first upload on
+// the context's stream writes the buffer (nsgpu_create synchronises the stream before returning).
+int alloc(double** p, size_t bytes, hipStream_t s) {
     HIPCHK(hipMalloc((void**)p, bytes));
-    HIPCHK(hipMemset(*p, 0, bytes));
+    HIPCHK(hipMemsetAsync(*p, 0, bytes, s));
     return NSGPU_OK;
 }
 
@@ -227,26 +233,27 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     size_t five = 5 * c->plane_bytes, four = 4 * c->plane_bytes;
     rc = NSGPU_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
-    if (!rc) rc = alloc(&c->W, five);
-    if (!rc) rc = alloc(&c->W2, five);
-    for (int t = 0; t < 2 && !rc; t++) rc = alloc(&c->Ws[t], five);
-    if (!rc) rc = alloc(&c->Un, five);
-    if (!rc) rc = alloc(&c->Ua, five);
-    if (!rc) rc = alloc(&c->Ub, five);
-    for (int d = 0; d < 3 && !rc; d++) rc = alloc(&c->F[d], five);
-    if (!rc) rc = alloc(&c->Dx, four);
-    if (!rc) rc = alloc(&c->Dy, four);
-    if (!rc) rc = alloc(&c->R, five);
-    if (!rc) rc = alloc(&c->stage_aos, five);
+    if (!rc) rc = alloc(&c->W, five, c->stream);
+    if (!rc) rc = alloc(&c->W2, five, c->stream);
+    for (int t = 0; t < 2 && !rc; t++) rc = alloc(&c->Ws[t], five, c->stream);
+    if (!rc) rc = alloc(&c->Un, five, c->stream);
+    if (!rc) rc = alloc(&c->Ua, five, c->stream);
+    if (!rc) rc = alloc(&c->Ub, five, c->stream);
+    for (int d = 0; d < 3 && !rc; d++) rc = alloc(&c->F[d], five, c->stream);
+    if (!rc) rc = alloc(&c->Dx, four, c->stream);
+    if (!rc) rc = alloc(&c->Dy, four, c->stream);
+    if (!rc) rc = alloc(&c->R, five, c->stream);
+    if (!rc) rc = alloc(&c->stage_aos, five, c->stream);
     c->partial_max = 4096;
-    if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max);
-    if (!rc) rc = alloc((double**)&c->dtmin, 2 * sizeof(unsigned long long));
-    if (!rc) rc = alloc((double**)&c->flags, sizeof(double));
-    if (!rc) rc = alloc(&c->dtime, sizeof(double));
-    if (!rc) rc = alloc(&c->sums, 5 * sizeof(double));
+    if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max, c->stream);
+    if (!rc) rc = alloc((double**)&c->dtmin, 2 * sizeof(unsigned long long), c->stream);
+    if (!rc) rc = alloc((double**)&c->flags, sizeof(double), c->stream);
+    if (!rc) rc = alloc(&c->dtime, sizeof(double), c->stream);
+    if (!rc) rc = alloc(&c->sums, 5 * sizeof(double), c->stream);
     for (int t = 0; t < 2 * T_COUNT && !rc; t++)
         if (hipEventCreate(&c->ev[t]) != hipSuccess) rc = NSGPU_ERR_HIP;
     c->Ulast = c->Ua;
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (rc) {
         nsgpu_destroy(c);
         return rc;
@@ -495,6 +502,306 @@ int nsgpu_timer_read(nsgpu_ctx* c, int i, double* ms, long* launches) {
     if (ms) *ms = c->ms[i];
     if (launches) *launches = c->launches[i];
     return NSGPU_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// ensembles (include/nsgpu.h): batched launches over the member blocks, one step per graph
+// ------------------------------------------------------------------------------------------
+struct nsgpu_ensemble {
+    std::vector<nsgpu_ctx*> m;
+    std::vector<EnsTiles>   tiles;
+    std::vector<EnsBlock>   hblk;
+    std::vector<int>        hstart;
+    int                     total[EF_COUNT] = {};
+    int                     device = 0;
+    hipStream_t             stream = nullptr;
+    EnsBlock*               dblk   = nullptr;
+    int*                    dstart = nullptr;
+    hipGraph_t              graph  = nullptr;
+    hipGraphExec_t          exec   = nullptr;
+    hipEvent_t              ev[2]  = {nullptr, nullptr};
+};
+
+namespace {
+
+void ens_drop_graph(nsgpu_ensemble* e) {
+    if (e->exec) hipGraphExecDestroy(e->exec);
+    if (e->graph) hipGraphDestroy(e->graph);
+    e->exec  = nullptr;
+    e->graph = nullptr;
+}
+
+// descriptors and workgroup prefix sums from the members' current buffers and the tiles
+int ens_build(nsgpu_ensemble* e) {
+    const int n = (int)e->m.size();
+    e->hblk.assign(n, EnsBlock{});
+    e->hstart.assign((size_t)EF_COUNT * (n + 1), 0);
+    for (int b = 0; b < n; b++) {
+        nsgpu_ctx* c = e->m[b];
+        EnsBlock&  k = e->hblk[b];
+        k.g = c->g, k.b = c->b, k.ph = c->ph;
+        k.W = c->W, k.Un = c->Un, k.Ua = c->Ua, k.Ub = c->Ub;
+        k.Ws[0] = c->Ws[0], k.Ws[1] = c->Ws[1];
+        for (int d = 0; d < 3; d++) k.F[d] = c->F[d];
+        k.Dx = c->Dx, k.Dy = c->Dy;
+        k.dtmin = &c->dtmin[0];
+        k.dtime = c->dtime;
+        k.t     = e->tiles[b];
+        ens_grids(k);
+    }
+    for (int f = 0; f < EF_COUNT; f++) {
+        long acc = 0;
+        for (int b = 0; b < n; b++) {
+            e->hstart[(size_t)f * (n + 1) + b] = (int)acc;
+            const int* gr = e->hblk[b].grid[f];
+            acc += (long)gr[0] * gr[1] * gr[2];
+            if (acc > 0x7fffffffL) return NSGPU_ERR_INVALID;
+        }
+        e->hstart[(size_t)f * (n + 1) + n] = (int)acc;
+        e->total[f]                        = (int)acc;
+    }
+    HIPCHK(hipMemcpyAsync(e->dblk, e->hblk.data(), sizeof(EnsBlock) * n, hipMemcpyHostToDevice,
+                          e->stream));
+    HIPCHK(hipMemcpyAsync(e->dstart, e->hstart.data(), sizeof(int) * e->hstart.size(),
+                          hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return NSGPU_OK;
+}
+
+EnsTable ens_table(const nsgpu_ensemble* e) { return EnsTable{e->dblk, e->dstart, (int)e->m.size()}; }
+
+void ens_enqueue(nsgpu_ensemble* e) {
+    const nsgpu_ctx* c0 = e->m[0];
+    c0->L->ens_step(ens_table(e), e->total, c0->ph, e->stream);
+}
+
+int ens_capture(nsgpu_ensemble* e) {
+    ens_drop_graph(e);
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    ens_enqueue(e);
+    hipGraph_t g  = nullptr;
+    hipError_t rc = hipStreamEndCapture(e->stream, &g);
+    if (rc != hipSuccess) {
+        fprintf(stderr, "nsgpu: ensemble capture failed: %s\n", hipGetErrorString(rc));
+        return NSGPU_ERR_HIP;
+    }
+    e->graph = g;
+    HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    return NSGPU_OK;
+}
+
+// members may have swapped their state buffers (nsgpu_step alternates W and W2): re-read them
+int ens_refresh(nsgpu_ensemble* e) {
+    for (nsgpu_ctx* c : e->m) HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t b = 0; b < e->m.size(); b++)
+        if (e->hblk[b].W != e->m[b]->W) return ens_build(e);
+    return NSGPU_OK;
+}
+
+int ens_run(nsgpu_ensemble* e, int nsteps, bool graph) {
+    if (graph && !e->exec) {
+        int rc = ens_capture(e);
+        if (rc) return rc;
+    }
+    for (int s = 0; s < nsteps; s++) {
+        if (graph) HIPCHK(hipGraphLaunch(e->exec, e->stream));
+        else ens_enqueue(e);
+    }
+    HIPCHK(hipGetLastError());
+    return NSGPU_OK;
+}
+
+int ens_set_tiles(nsgpu_ensemble* e, const std::vector<EnsTiles>& t) {
+    e->tiles = t;
+    ens_drop_graph(e);
+    return ens_build(e);
+}
+
+// state kept across autotuner trials: W in the member's operator-level array R, the stage state
+// Ua (the conservative state the member reports) in its staging array, and the time on the host
+int ens_save(nsgpu_ensemble* e, std::vector<double>& times) {
+    times.resize(e->m.size());
+    for (size_t b = 0; b < e->m.size(); b++) {
+        nsgpu_ctx* c = e->m[b];
+        HIPCHK(hipMemcpyAsync(c->R, c->W, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(c->stage_aos, c->Ua, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
+                              e->stream));
+        HIPCHK(hipMemcpyAsync(&times[b], c->dtime, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return NSGPU_OK;
+}
+
+int ens_restore(nsgpu_ensemble* e, const std::vector<double>& times) {
+    for (size_t b = 0; b < e->m.size(); b++) {
+        nsgpu_ctx* c = e->m[b];
+        HIPCHK(hipMemcpyAsync(c->W, c->R, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(c->Ua, c->stage_aos, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
+                              e->stream));
+        HIPCHK(hipMemcpyAsync(c->dtime, &times[b], sizeof(double), hipMemcpyHostToDevice, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return NSGPU_OK;
+}
+
+// ms per step of `steps` graph replays with tiles t (best of 2), state restored afterwards
+int ens_trial(nsgpu_ensemble* e, const std::vector<EnsTiles>& t, int steps,
+              const std::vector<double>& times, double* ms) {
+    int rc = ens_set_tiles(e, t);
+    if (!rc) rc = ens_run(e, 1, true);
+    double best = 1e300;
+    for (int r = 0; r < 2 && !rc; r++) {
+        HIPCHK(hipEventRecord(e->ev[0], e->stream));
+        rc = ens_run(e, steps, true);
+        HIPCHK(hipEventRecord(e->ev[1], e->stream));
+        HIPCHK(hipEventSynchronize(e->ev[1]));
+        float m = 0.f;
+        HIPCHK(hipEventElapsedTime(&m, e->ev[0], e->ev[1]));
+        best = std::min(best, (double)m / steps);
+    }
+    if (!rc) rc = ens_restore(e, times);
+    *ms = best;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out) {
+    if (!out) return NSGPU_ERR_INVALID;
+    *out = nullptr;
+    if (!members || n < 1 || n > 65536) return NSGPU_ERR_INVALID;
+    const nsgpu_ctx* c0 = members[0];
+    if (!c0) return NSGPU_ERR_INVALID;
+    for (int b = 0; b < n; b++) {
+        const nsgpu_ctx* c = members[b];
+        if (!c || c->device != c0->device || c->L != c0->L || c->ph.conv != c0->ph.conv ||
+            c->ph.borges != c0->ph.borges || c->ph.diff_order != c0->ph.diff_order ||
+            c->ph.suth != c0->ph.suth || c->g.dim != c0->g.dim)
+            return NSGPU_ERR_INVALID;
+        for (int a = 0; a < b; a++)
+            if (members[a] == c) return NSGPU_ERR_INVALID;
+    }
+    HIPCHK(hipSetDevice(c0->device));
+    nsgpu_ensemble* e = new (std::nothrow) nsgpu_ensemble();
+    if (!e) return NSGPU_ERR_INVALID;
+    e->m.assign(members, members + n);
+    e->device = c0->device;
+    e->tiles.assign(n, EnsTiles{64, 64, 16});
+    int rc = NSGPU_OK;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc && hipMalloc((void**)&e->dblk, sizeof(EnsBlock) * n) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc && hipMalloc((void**)&e->dstart, sizeof(int) * EF_COUNT * (n + 1)) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    for (int t = 0; t < 2 && !rc; t++)
+        if (hipEventCreate(&e->ev[t]) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc) rc = ens_build(e);
+    if (rc) {
+        nsgpu_ensemble_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return NSGPU_OK;
+}
+
+void nsgpu_ensemble_destroy(nsgpu_ensemble* e) {
+    if (!e) return;
+    hipSetDevice(e->device);
+    if (e->stream) hipStreamSynchronize(e->stream);
+    ens_drop_graph(e);
+    if (e->dblk) hipFree(e->dblk);
+    if (e->dstart) hipFree(e->dstart);
+    for (hipEvent_t v : e->ev)
+        if (v) hipEventDestroy(v);
+    if (e->stream) hipStreamDestroy(e->stream);
+    delete e;
+}
+
+int nsgpu_ensemble_set_tiles(nsgpu_ensemble* e, const int* tiles) {
+    if (!e || !tiles) return NSGPU_ERR_INVALID;
+    std::vector<EnsTiles> t(e->m.size());
+    for (size_t b = 0; b < t.size(); b++) {
+        t[b] = EnsTiles{tiles[3 * b], tiles[3 * b + 1], tiles[3 * b + 2]};
+        for (int kw : {t[b].kw_flux, t[b].kw_cell})
+            if (kw != 16 && kw != 32 && kw != 64) return NSGPU_ERR_INVALID;
+        if (t[b].march < 1 || t[b].march > 1024) return NSGPU_ERR_INVALID;
+    }
+    HIPCHK(hipSetDevice(e->device));
+    int rc = ens_refresh(e);
+    return rc ? rc : ens_set_tiles(e, t);
+}
+
+int nsgpu_ensemble_get_tiles(const nsgpu_ensemble* e, int* tiles) {
+    if (!e || !tiles) return NSGPU_ERR_INVALID;
+    for (size_t b = 0; b < e->m.size(); b++) {
+        tiles[3 * b]     = e->tiles[b].kw_flux;
+        tiles[3 * b + 1] = e->tiles[b].kw_cell;
+        tiles[3 * b + 2] = e->tiles[b].march;
+    }
+    return NSGPU_OK;
+}
+
+int nsgpu_ensemble_step(nsgpu_ensemble* e, int nsteps, int use_graph) {
+    if (!e || nsteps < 0) return NSGPU_ERR_INVALID;
+    HIPCHK(hipSetDevice(e->device));
+    int rc = ens_refresh(e);
+    if (!rc) rc = ens_run(e, nsteps, use_graph != 0);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    // the last stage leaves the conservative state in Ua and the primitive state in W
+    for (nsgpu_ctx* c : e->m) c->Ulast = c->Ua;
+    return rc;
+}
+
+int nsgpu_ensemble_tune(nsgpu_ensemble* e, int steps, double* ms_default, double* ms_tuned) {
+    if (!e || steps < 1) return NSGPU_ERR_INVALID;
+    HIPCHK(hipSetDevice(e->device));
+    int rc = ens_refresh(e);
+    if (rc) return rc;
+    std::vector<double> times;
+    rc = ens_save(e, times);
+    if (rc) return rc;
+    const size_t n = e->m.size();
+    std::vector<EnsTiles> best(n, EnsTiles{64, 64, 16});
+    double                tbest = 0;
+    rc = ens_trial(e, best, steps, times, &tbest);
+    if (rc) return rc;
+    if (ms_default) *ms_default = tbest;
+    // classes: members with equal nz share kw_flux / kw_cell, members with equal nx share march
+    std::map<int, std::vector<size_t>> by_nz, by_nx;
+    for (size_t b = 0; b < n; b++) {
+        by_nz[e->m[b]->g.nz].push_back(b);
+        by_nx[e->m[b]->g.nx].push_back(b);
+    }
+    const int kws[3] = {64, 32, 16}, marches[4] = {16, 8, 32, 4};
+    for (int pass = 0; pass < 2; pass++) {
+        bool improved = false;
+        auto try_field = [&](const std::vector<size_t>& cls, int EnsTiles::*field, const int* vals,
+                             int nv) -> int {
+            for (int v = 0; v < nv; v++) {
+                if (best[cls[0]].*field == vals[v]) continue;
+                std::vector<EnsTiles> cand = best;
+                for (size_t b : cls) cand[b].*field = vals[v];
+                double t  = 0;
+                int    r2 = ens_trial(e, cand, steps, times, &t);
+                if (r2) return r2;
+                if (t < tbest * 0.99) best = cand, tbest = t, improved = true;
+            }
+            return NSGPU_OK;
+        };
+        for (auto& kv : by_nz) {
+            if ((rc = try_field(kv.second, &EnsTiles::kw_flux, kws, 3))) return rc;
+            if ((rc = try_field(kv.second, &EnsTiles::kw_cell, kws, 3))) return rc;
+        }
+        for (auto& kv : by_nx)
+            if ((rc = try_field(kv.second, &EnsTiles::march, marches, 4))) return rc;
+        if (!improved) break;
+    }
+    rc = ens_set_tiles(e, best);
+    if (ms_tuned) *ms_tuned = tbest;
+    return rc;
 }
 
 }  // extern "C"

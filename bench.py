@@ -63,8 +63,148 @@ def run_cpu_baseline(n, steps, timeout):
     return None
 
 
+def precision_workload(args):
+    """Precision form (FP32 reconstruction + viscous fluxes, FP64 accumulation) on decaying
+    isotropic turbulence at n^3: the FP64 path and the FP32 path advance the same seeded state
+    the same number of steps; reports both rates and the L2 / Linf errors of density and total
+    energy of the FP32 run against the FP64 run."""
+    import numpy as np
+    ns = importlib.import_module("3dnssolver_amd")
+    C = importlib.import_module("3dnssolver_amd.cases")
+    n = args.n
+    print(f"[bench] precision form {n}^3: generating the isotropic field", file=sys.stderr, flush=True)
+    case, W0 = C.config3(n)
+    out = {}
+    states = {}
+    for math in ("exact", "fp32"):
+        s = ns.Solver(case, math=math)
+        s.set_state(W0, "primitive")
+        s.step(args.warmup)
+        print(f"[bench] {math}: {args.steps} steps", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        el = time.perf_counter() - t0
+        out[math] = n ** 3 * args.steps / el / 1e9
+        U = s.get_state("conservative")[3:-3, 3:-3, 3:-3]
+        states[math] = (U[..., 0].copy(), U[..., 1].copy())
+        _, _, flags, rc = s.reduce()
+        out[math + "_health"] = {"status": rc, "flags": flags}
+        s.close()
+        del U
+    errs = {}
+    for q, name in ((0, "rho"), (1, "rhoE")):
+        a, b = states["fp32"][q], states["exact"][q]
+        d = a - b
+        errs[name] = {"L2_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(b * b))),
+                      "Linf_rel": float(np.abs(d).max() / np.abs(b).max()),
+                      "L2_abs": float(np.sqrt(np.mean(d * d))), "Linf_abs": float(np.abs(d).max())}
+    print(json.dumps({
+        "metric": "precision form: Gcell-RK3-steps/s, FP32 reconstruction vs FP64",
+        "value": round(out["fp32"], 6), "unit": "Gcell-RK3-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"decaying isotropic turbulence {n}^3, [0,2pi]^3, E(k)~k^4 exp(-2(k/4)^2), "
+                               f"Mt 0.5, Re_lambda 100, WENO-5cw + CD4, SSP-RK3 CFL 0.5",
+                   "grid": [n, n, n]},
+        "fp64_value": round(out["exact"], 6), "speedup_vs_fp64": round(out["fp32"] / out["exact"], 4),
+        "errors_vs_fp64": errs,
+        "health": {"fp32": out["fp32_health"], "fp64": out["exact_health"]},
+    }))
+
+
+def ensemble_workload(args):
+    """Batched form: 64 independent periodic shear layers of uneven sizes advanced by one
+    hipGraph-captured SSP-RK3 step over all blocks (FP32 reconstruction, FP64 accumulation),
+    tiles from the autotuner.  Reports the aggregate rate, the rate of the same blocks stepped one
+    after another through the single-block path, and per-instance relative L2 errors of density
+    and total energy against the plain FP64 method run on each block alone."""
+    import numpy as np
+    ns = importlib.import_module("3dnssolver_amd")
+    C = importlib.import_module("3dnssolver_amd.cases")
+    members = C.config5(n_blocks=args.blocks)
+    cells = sum(c["nx"] * c["ny"] * c["nz"] for c, _, _ in members)
+    print(f"[bench] ensemble of {len(members)} blocks, {cells / 1e6:.2f} Mcells, math={args.math}",
+          file=sys.stderr, flush=True)
+    solvers = []
+    for case, W, _ in members:
+        s = ns.Solver(case, math=args.math)
+        s.set_state(W, "primitive")
+        solvers.append(s)
+    ens = ns.Ensemble(solvers)
+    ms_default = ms_tuned = None
+    if not args.no_tune:
+        ms_default, ms_tuned = ens.tune(steps=3)
+        print(f"[bench] autotuner: {ms_default:.3f} -> {ms_tuned:.3f} ms/step", file=sys.stderr,
+              flush=True)
+    tiles = ens.tiles
+    ens.step(args.warmup)
+    t0 = time.perf_counter()
+    ens.step(args.steps)
+    el = time.perf_counter() - t0
+    value = cells * args.steps / el / 1e9
+    nog = 3
+    t0 = time.perf_counter()
+    ens.step(nog, graph=False)
+    ms_nograph = (time.perf_counter() - t0) / nog * 1e3
+    done = args.warmup + args.steps + nog
+    # the same blocks one after another through the single-block path, same math (timed on
+    # copies of the initial states)
+    t_solo = 0.0
+    for case, W, _ in members:
+        s = ns.Solver(case, math=args.math)
+        s.set_state(W, "primitive")
+        s.step(1)
+        t0 = time.perf_counter()
+        s.step(nog)
+        t_solo += time.perf_counter() - t0
+        s.close()
+    ms_solo = t_solo / nog * 1e3
+    # per-instance error against the plain FP64 method (EXACT math) on the block alone
+    print(f"[bench] FP64 reference runs ({done} steps per block)", file=sys.stderr, flush=True)
+    l2 = {"rho": [], "rhoE": []}
+    health = []
+    for (case, W, _), s in zip(members, solvers):
+        U = s.get_state("conservative")[3:-3, 3:-3, 3:-3]
+        health.append(s.reduce()[3])
+        r = ns.Solver(case, math="exact")
+        r.set_state(W, "primitive")
+        r.step(done)
+        Ur = r.get_state("conservative")[3:-3, 3:-3, 3:-3]
+        r.close()
+        for q, name in ((0, "rho"), (1, "rhoE")):
+            d = U[..., q] - Ur[..., q]
+            l2[name].append(float(np.sqrt(np.sum(d * d) / np.sum(Ur[..., q] ** 2))))
+    ens.close()
+    for s in solvers:
+        s.close()
+    print(json.dumps({
+        "metric": "batched form: aggregate Gcell-RK3-steps/s over the ensemble",
+        "value": round(value, 6), "unit": "Gcell-RK3-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {"exact": "f64", "fast": "f64", "fp32": "f32+f64"}[args.math], "data": "synthetic",
+        "config": {"workload": f"{len(members)} periodic double shear layers on [0,1]^3, edges from "
+                               f"{{32,48,64,96,128}} per axis, Mach U[0.2,0.6], Re U[500,5000], "
+                               f"amplitude U[0.01,0.1], WENO-5 (Shu, LF) + CD4, SSP-RK3 CFL 0.5",
+                   "cells": cells, "math": args.math, "graph": True},
+        "autotuner": {"ms_default_tiles": ms_default, "ms_tuned": ms_tuned,
+                      "tiles_kw_flux_kw_cell_march": tiles.tolist()},
+        "ms_per_step_without_graph": round(ms_nograph, 4),
+        "ms_per_step_blocks_one_at_a_time": round(ms_solo, 4),
+        "l2_rel_vs_fp64_plain": {k: {"max": max(v), "median": float(np.median(v))}
+                                 for k, v in l2.items()},
+        "l2_rel_per_instance": l2, "steps_compared": done,
+        "health": {"nonzero": [h for h in health if h != 0]},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["tgv", "precision", "ensemble"], default="tgv",
+                    help="tgv: the headline run; precision: FP32-reconstruction form; "
+                         "ensemble: batched form")
+    ap.add_argument("--blocks", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
@@ -85,6 +225,12 @@ def main():
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               f"see DESIGN.md for how the block would shard.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "precision":
+        precision_workload(args)
+        return
+    if args.workload == "ensemble":
+        ensemble_workload(args)
+        return
 
     ns = importlib.import_module("3dnssolver_amd")
     C = importlib.import_module("3dnssolver_amd.cases")

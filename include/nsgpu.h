@@ -141,6 +141,38 @@ int nsgpu_timer_count(void);
 const char* nsgpu_timer_name(int i);
 int nsgpu_timer_read(nsgpu_ctx* ctx, int i, double* ms, long* launches);
 
+/* ---- Ensembles: many independent blocks of different sizes on one GPU ----------------------
+ * No reference counterpart: the reference advances one block per MPI rank, each with its own
+ * driver loop (driver.cpp:45-64 calling TimeScheme::step, time_scheme.cpp:3-70).  An ensemble
+ * advances n member contexts together: each kernel family of an SSP-RK3 step runs as ONE launch
+ * over every block, and the whole step (33 launches) is captured once into a hipGraph and
+ * replayed.  Each member keeps its own dt (its own CFL minimum), time and boundary conditions,
+ * and its results are those of nsgpu_step on that member alone.  Members must share the device,
+ * the math mode, the convection scheme and weights, the diffusion order, the viscosity law and
+ * dim; sizes, lengths, gamma, mu, Pr, CFL and boundary kinds may differ.  The member contexts
+ * stay usable (set_state, get_state, reduce, step) between ensemble calls. */
+typedef struct nsgpu_ensemble nsgpu_ensemble;
+
+int  nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out);
+void nsgpu_ensemble_destroy(nsgpu_ensemble* ens);
+
+/* Tile shape per member, 3 ints each: kw_flux (lanes along k of a face-flux wave: 16, 32 or 64),
+ * kw_cell (lanes along k of the 256-thread cell kernels: 16, 32 or 64), march (planes per
+ * marching chunk of the x-stencil kernels, 1..1024).  Defaults: 64, 64, 16. */
+int nsgpu_ensemble_set_tiles(nsgpu_ensemble* ens, const int* tiles);
+int nsgpu_ensemble_get_tiles(const nsgpu_ensemble* ens, int* tiles);
+
+/* nsteps SSP-RK3 steps of every member; use_graph != 0 replays the captured step graph,
+ * otherwise the launches are enqueued directly.  Synchronises once at the end. */
+int nsgpu_ensemble_step(nsgpu_ensemble* ens, int nsteps, int use_graph);
+
+/* Autotuner: coordinate descent over the tile shapes, kw_flux and kw_cell per class of members
+ * with equal nz and march per class of equal nx, timing `steps` graph-replayed steps per trial
+ * (best of 2).  The member states are restored after every trial, so tuning leaves them as they
+ * were.  Stores the step time of the default tiles and of the chosen tiles (ms, may be NULL)
+ * and leaves the chosen tiles set. */
+int nsgpu_ensemble_tune(nsgpu_ensemble* ens, int steps, double* ms_default, double* ms_tuned);
+
 #ifdef __cplusplus
 }
 #endif

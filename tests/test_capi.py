@@ -60,3 +60,16 @@ def test_strerror_and_null_args():
     assert L.nsgpu_strerror(-6).decode().startswith("max |u|+c")
     assert L.nsgpu_step(None, 1, None, None) == -1
     assert L.nsgpu_array_len(None) == 0
+
+
+def test_ensemble_rejects_bad_arguments_before_touching_a_device():
+    L = package().load_library()
+    h = ctypes.c_void_p()
+    none = (ctypes.c_void_p * 1)(None)
+    assert L.nsgpu_ensemble_create(None, 1, ctypes.byref(h)) == -1
+    assert L.nsgpu_ensemble_create(none, 0, ctypes.byref(h)) == -1
+    assert L.nsgpu_ensemble_create(none, 1, ctypes.byref(h)) == -1
+    assert L.nsgpu_ensemble_create(none, 1, None) == -1
+    assert L.nsgpu_ensemble_step(None, 1, 1) == -1
+    assert L.nsgpu_ensemble_tune(None, 1, None, None) == -1
+    assert L.nsgpu_ensemble_set_tiles(None, None) == -1
