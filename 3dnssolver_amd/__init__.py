@@ -28,7 +28,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libnsgpu.so")
 
 BC = {"periodic": 0, "neumann": 1}
 CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
-WEIGHTS = {"shu": 0, "borges": 1}
+WEIGHTS = {"shu": 0, "borges": 1, "learned": 2}
+MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3
 MATH = {"exact": 0, "fast": 1, "fp32": 2}
 STATE = {"conservative": 0, "primitive": 1}
 VISCOSITY = {"constant": 0, "sutherland": 1}
@@ -40,7 +41,8 @@ EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len",
            "nsgpu_stream", "nsgpu_selftest", "nsgpu_timing", "nsgpu_timer_count", "nsgpu_timer_name",
            "nsgpu_timer_read", "nsgpu_ensemble_create", "nsgpu_ensemble_destroy",
            "nsgpu_ensemble_set_tiles", "nsgpu_ensemble_get_tiles", "nsgpu_ensemble_step",
-           "nsgpu_ensemble_tune"]
+           "nsgpu_ensemble_tune", "nsgpu_set_mlp", "nsgpu_mlp_forward", "nsgpu_mlp_loss_grad",
+           "nsgpu_mlp_train", "nsgpu_mlp_recon"]
 
 
 class Config(ctypes.Structure):
@@ -107,6 +109,14 @@ def load_library(path=None):
     L.nsgpu_ensemble_get_tiles.argtypes = [P, I]
     L.nsgpu_ensemble_step.argtypes = [P, ctypes.c_int, ctypes.c_int]
     L.nsgpu_ensemble_tune.argtypes = [P, ctypes.c_int, D, D]
+    Fp = ctypes.POINTER(ctypes.c_float)
+    L.nsgpu_set_mlp.argtypes = [P, Fp]
+    L.nsgpu_mlp_forward.argtypes = [ctypes.c_int, Fp, ctypes.c_long, Fp, Fp]
+    L.nsgpu_mlp_loss_grad.argtypes = [ctypes.c_int, Fp, ctypes.c_long, Fp, Fp, Fp, D, Fp]
+    L.nsgpu_mlp_train.argtypes = [ctypes.c_int, Fp, ctypes.c_long, Fp, Fp, Fp, ctypes.c_int,
+                                  ctypes.c_long, ctypes.c_float, Fp]
+    L.nsgpu_mlp_recon.argtypes = [ctypes.c_int, ctypes.c_int, Fp, ctypes.c_long, D, ctypes.c_int, D,
+                                  I]
     _lib = L
     return L
 
@@ -132,6 +142,66 @@ def make_config(case, math="exact"):
     c.viscosity = VISCOSITY[case.get("viscosity", "constant")]
     c.suth_C = case.get("suth_C", SUTH_C)
     return c
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None and a.shape != shape:
+        raise ValueError(f"array shape {a.shape} != {shape}")
+    return a
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _call(rc):
+    if rc != 0:
+        raise NsgpuError(rc, load_library().nsgpu_strerror(rc).decode())
+
+
+def mlp_forward(params, X, device=0):
+    """Softmax weights (n, 3) of the learned-weights network for features X (n, 8), on the GPU."""
+    L = load_library()
+    p = _f32(params, (MLP_NPARAM,))
+    X = _f32(X)
+    W = np.zeros((len(X), 3), dtype=np.float32)
+    _call(L.nsgpu_mlp_forward(device, _fp(p), len(X), _fp(X), _fp(W)))
+    return W
+
+
+def mlp_loss_grad(params, X, Q, T, device=0):
+    """(mean squared reconstruction error, gradient) on the GPU."""
+    L = load_library()
+    p = _f32(params, (MLP_NPARAM,))
+    X, Q, T = _f32(X), _f32(Q), _f32(T)
+    g = np.zeros(MLP_NPARAM, dtype=np.float32)
+    loss = np.zeros(1)
+    _call(L.nsgpu_mlp_loss_grad(device, _fp(p), len(T), _fp(X), _fp(Q), _fp(T), _ptr(loss), _fp(g)))
+    return float(loss[0]), g
+
+
+def mlp_train(params, X, Q, T, steps, batch=65536, lr=1e-3, device=0):
+    """Adam training on the GPU; returns (new parameters, loss per step)."""
+    L = load_library()
+    p = _f32(params, (MLP_NPARAM,)).copy()
+    X, Q, T = _f32(X), _f32(Q), _f32(T)
+    hist = np.zeros(max(steps, 1), dtype=np.float32)
+    _call(L.nsgpu_mlp_train(device, _fp(p), len(T), _fp(X), _fp(Q), _fp(T), steps, batch, lr,
+                            _fp(hist)))
+    return p, hist[:steps]
+
+
+def mlp_recon(params, S, side, math="exact", device=0):
+    """The face kernels' learned reconstruction of stencils S (n, 5): (values, regimes)."""
+    L = load_library()
+    p = _f32(params, (MLP_NPARAM,))
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    out = np.zeros(len(S))
+    rg = np.zeros(len(S), dtype=np.int32)
+    _call(L.nsgpu_mlp_recon(device, MATH[math], _fp(p), len(S), _ptr(S), side, _ptr(out),
+                            rg.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return out, rg
 
 
 def selftest(n=1 << 26, seed=1, device=0):
@@ -185,6 +255,11 @@ class Solver:
         A = np.zeros(self.shape)
         self._check(self.L.nsgpu_get_state(self.h, _ptr(A), STATE[kind]))
         return A
+
+    def set_mlp(self, params):
+        """Network of the learned weights (contexts with weights = "learned")."""
+        p = _f32(params, (MLP_NPARAM,))
+        self._check(self.L.nsgpu_set_mlp(self.h, _fp(p)))
 
     def apply_boundary(self, A):
         A = self._arr(A).copy()

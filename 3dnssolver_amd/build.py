@@ -39,14 +39,20 @@ def build(verbose=False, defines=(), out=None):
         ("ns_kernels.hip", "kernels_fp32.o", ["-DNS_FAST=1", "-DNS_FP32=1", "-ffp-contract=on",
                                               *defines]),
         ("nsgpu_capi.hip", "capi.o", ["-ffp-contract=off"]),
+        ("ns_mlp.hip", "mlp.o", ["-ffp-contract=on"]),
     ]
+    cmds = []
     for src, obj, extra in units:
         o = os.path.join(OUT_, obj)
-        cmd = ["hipcc", "-c", *COMMON, *extra, os.path.join(CSRC, src), "-o", o]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append(["hipcc", "-c", *COMMON, *extra, os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    if verbose:
+        for cmd in cmds:
+            print(" ".join(cmd))
+    # the units are independent: compile them concurrently
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        list(ex.map(_run, cmds))
     lib = os.path.join(OUT_, "libnsgpu.so")
     tmp = lib + ".tmp"
     _run(["hipcc", "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])

@@ -33,17 +33,20 @@ struct Bnd {
     int face_kind[6];  // W,E,N,S,F,B: 0 periodic, 1 Neumann
 };
 
+struct MlpPacked;  // ns_mlp.h
+
 struct Phys {
     double gamma, mu, Pr, cfl;
     double lamb;        // (-2./3.) * mu            (centered_diffusion.cpp:101)
     double two_mu;      // 2.0 * mu                 (diffusion_scheme.cpp:5, :27)
     double heat;        // gamma / (Pr * (gamma - 1.0))  (centered_diffusion.cpp:161)
     int    conv;        // 0 none, 1 component-wise, 2 characteristic
-    int    borges;      // 0 Shu, 1 Borges
+    int    weights;     // 0 Shu, 1 Borges, 2 learned (network `mlp`)
     int    diff_order;  // 0, 2, 4, 6
     int    suth;        // Sutherland viscosity
     double suth_C;      // Sutherland constant (T0 = 1)
     double suth_k;      // mu * (1.0 + C)
+    const MlpPacked* mlp;  // packed network of the learned weights (device), or null
 };
 
 // SSP-RK3 stage (time_scheme.cpp:37-63); the coefficients are formed on the device from dt
@@ -156,7 +159,13 @@ struct Launchers {
     // one SSP-RK3 step of every block of an ensemble; total[f] = workgroups of family f; the
     // template choices (convection, weights, diffusion order, viscosity law) come from `ph`
     void (*ens_step)(const EnsTable& t, const int* total, const Phys& ph, hipStream_t s);
+    // the learned reconstruction of the face-flux kernels on n stencils S[n][5] (side 0 left-,
+    // 1 right-biased): out[n] and the regime of each stencil (0 smooth, 1 network, 2 flagged)
+    void (*mlp_recon)(const MlpPacked* P, long n, const double* S, int side, double* out,
+                      int* regime, hipStream_t s);
 };
+// packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
+void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);
 
 namespace exact {
 const Launchers& launchers();

@@ -18,6 +18,7 @@
 //
 // Citations are paths relative to the reference's src/ directory.
 #include "ns_common.h"
+#include "ns_mlp.h"
 
 #ifndef NS_FAST
 #error "compile with -DNS_FAST=0 or -DNS_FAST=1"
@@ -164,7 +165,7 @@ __device__ __forceinline__ real weno_combine(real a, real b, real c, real d, rea
 
 // Weno5::WENO5_left (weno-5.cpp:305-322), ideal weights d_left = {0.1, 0.6, 0.3}
 template <bool BORGES>
-__device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e) {
+__device__ __forceinline__ real weno_left_c(real a, real b, real c, real d, real e) {
     real f0 = (RL(2.) * a - RL(7.) * b + RL(11.) * c) * ONEPERSIX;
     real f1 = (-b + RL(5.) * c + RL(2.) * d) * ONEPERSIX;
     real f2 = (RL(2.) * c + RL(5.) * d - e) * ONEPERSIX;
@@ -173,11 +174,57 @@ __device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e
 
 // Weno5::WENO5_right (weno-5.cpp:285-303), ideal weights d_right = {0.3, 0.6, 0.1}
 template <bool BORGES>
-__device__ __forceinline__ real weno_right(real a, real b, real c, real d, real e) {
+__device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, real e) {
     real f0 = (-a + RL(5.) * b + RL(2.) * c) * ONEPERSIX;
     real f1 = (RL(2.) * b + RL(5.) * c - d) * ONEPERSIX;
     real f2 = (RL(11.) * c - RL(7.) * d + RL(2.) * e) * ONEPERSIX;
     return weno_combine<BORGES>(a, b, c, d, e, RL(0.3), RL(0.6), RL(0.1), f0, f1, f2);
+}
+
+
+// ------------------------------------------------------------------------------------------
+// learned weights (ns_mlp.h; no reference counterpart).  The network sees the stencil of the
+// left-biased reconstruction; WENO5_right(a..e) is fed mirrored, (e..a).  Smooth and flagged
+// stencils keep the classic Shu reconstruction, computed by the same code as the classic path.
+// Every lane of the wavefront must call this (the network uses matrix instructions); `active`
+// marks the lanes whose face is real.
+// ------------------------------------------------------------------------------------------
+struct Learn {
+    const MlpPacked* P;
+    bool             active;
+};
+
+template <int SIDE>
+__device__ __forceinline__ real weno_learned(real a, real b, real c, real d, real e,
+                                             const Learn& L, int* regime = nullptr) {
+    const real classic = SIDE ? weno_right_c<false>(a, b, c, d, e) : weno_left_c<false>(a, b, c, d, e);
+    const real v[5]    = {SIDE ? e : a, SIDE ? d : b, c, SIDE ? b : d, SIDE ? a : e};
+    float      x[8];
+    const int  rg   = mlp_features(v, x);
+    const bool need = L.active && rg == 1;
+    if (regime) *regime = rg;
+    if (__ballot(need) == 0) return classic;  // wavefront-uniform
+    float xs[8], w[3];
+#pragma unroll
+    for (int k = 0; k < 8; k++) xs[k] = need ? x[k] : 0.0f;
+    mlp_weights_lane(L.P, xs, w);
+    if (!need) return classic;
+    const real f0 = (RL(2.) * v[0] - RL(7.) * v[1] + RL(11.) * v[2]) * ONEPERSIX;
+    const real f1 = (-v[1] + RL(5.) * v[2] + RL(2.) * v[3]) * ONEPERSIX;
+    const real f2 = (RL(2.) * v[2] + RL(5.) * v[3] - v[4]) * ONEPERSIX;
+    return (real)w[0] * f0 + (real)w[1] * f1 + (real)w[2] * f2;
+}
+
+// weights WT: 0 Shu, 1 Borges, 2 learned
+template <int WT>
+__device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e, const Learn& L) {
+    if constexpr (WT == 2) return weno_learned<0>(a, b, c, d, e, L);
+    else return weno_left_c<WT == 1>(a, b, c, d, e);
+}
+template <int WT>
+__device__ __forceinline__ real weno_right(real a, real b, real c, real d, real e, const Learn& L) {
+    if constexpr (WT == 2) return weno_learned<1>(a, b, c, d, e, L);
+    else return weno_right_c<WT == 1>(a, b, c, d, e);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -354,8 +401,8 @@ __device__ __forceinline__ void Rmul(const Eig& E, const real ft[5], real out[5]
 // (0..4 fL, 5..9 fR) of cell i-2+ii, ii = 0..5; Wl, Wr are the primitives of cells i and i+1
 // (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
 // ------------------------------------------------------------------------------------------
-template <int DIR, bool BORGES, int Q, class S>
-__device__ __forceinline__ real char_component(const Eig& E, const S& st) {
+template <int DIR, int WT, int Q, class S>
+__device__ __forceinline__ real char_component(const Eig& E, const S& st, const Learn& L) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
 #if NS_ROW_FENCE
     __asm__ __volatile__("" ::: "memory");  // re-read the stencil per row: fewer live registers
@@ -372,19 +419,19 @@ __device__ __forceinline__ real char_component(const Eig& E, const S& st) {
         a[ii] = Lrow<DIR, Q>(E, sl);
         b[ii] = Lrow<DIR, Q>(E, sr);
     }
-    return weno_left<BORGES>(a[0], a[1], a[2], a[3], a[4]) +
-           weno_right<BORGES>(b[0], b[1], b[2], b[3], b[4]);
+    return weno_left<WT>(a[0], a[1], a[2], a[3], a[4], L) +
+           weno_right<WT>(b[0], b[1], b[2], b[3], b[4], L);
 }
 
-template <int DIR, int CONV, bool BORGES, class S>
+template <int DIR, int CONV, int WT, class S>
 __device__ __forceinline__ void face_flux(const S& st, const real Wl[5], const real Wr[5],
-                                          real gamma, real out[5]) {
+                                          real gamma, real out[5], const Learn& L) {
     if constexpr (CONV == 1) {
 #pragma unroll
         for (int q = 0; q < 5; q++)
-            out[q] = weno_left<BORGES>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q)) +
-                     weno_right<BORGES>(st(1, 5 + q), st(2, 5 + q), st(3, 5 + q), st(4, 5 + q),
-                                        st(5, 5 + q));
+            out[q] = weno_left<WT>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q), L) +
+                     weno_right<WT>(st(1, 5 + q), st(2, 5 + q), st(3, 5 + q), st(4, 5 + q),
+                                    st(5, 5 + q), L);
     } else {
         real Wa[5];
 #pragma unroll
@@ -392,11 +439,11 @@ __device__ __forceinline__ void face_flux(const S& st, const real Wl[5], const r
         Eig E;
         eigen<DIR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
         real ft[5];
-        ft[0] = char_component<DIR, BORGES, 0>(E, st);
-        ft[1] = char_component<DIR, BORGES, 1>(E, st);
-        ft[2] = char_component<DIR, BORGES, 2>(E, st);
-        ft[3] = char_component<DIR, BORGES, 3>(E, st);
-        ft[4] = char_component<DIR, BORGES, 4>(E, st);
+        ft[0] = char_component<DIR, WT, 0>(E, st, L);
+        ft[1] = char_component<DIR, WT, 1>(E, st, L);
+        ft[2] = char_component<DIR, WT, 2>(E, st, L);
+        ft[3] = char_component<DIR, WT, 3>(E, st, L);
+        ft[4] = char_component<DIR, WT, 4>(E, st, L);
         Rmul<DIR>(E, ft, out);
     }
 }
@@ -427,12 +474,13 @@ constexpr int TILE_L = 64;   // lanes along k
 // stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
 // values of the 13 cells the tile needs are computed once into LDS.  (bx, by, bz) = (k tile,
 // row tile, sweep tile).
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
-                                                  real* __restrict__ tile) {
+                                                  real* __restrict__ tile,
+                                                  const MlpPacked* __restrict__ mlp) {
     constexpr int NC = TILE_S + 5;
     const int lane = threadIdx.x, w = threadIdx.y;
     const int rows = TILE_L / kw;
@@ -455,10 +503,15 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
         }
     }
     __syncthreads();
-    const int fidx = s0 + w;  // face index 0..n-1
-    if (!kin || fidx >= n) return;
-    const int  i = fidx + NGC - 1;  // face i+1/2
-    const long c = (long)i * ss + (long)o * so + k;
+    const int  fidx  = s0 + w;  // face index 0..n-1
+    const bool valid = kin && fidx < n;
+    // the learned weights need every lane of the wavefront: lanes without a face compute the
+    // first face of the block from in-bounds data and store nothing
+    if constexpr (WT != 2) {
+        if (!valid) return;
+    }
+    const int  i = (valid ? fidx : 0) + NGC - 1;  // face i+1/2
+    const long c = (long)i * ss + (long)(valid ? o : NGC) * so + (valid ? k : NGC);
     real     Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
@@ -466,31 +519,33 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     }
     LdsSplit<TILE_L> st{&tile[(w * 10) * TILE_L + lane]};
     real           out[5];
-    face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
+    face_flux<DIR, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
                                                       const double* __restrict__ W,
                                                       double* __restrict__ f, Geo g,
-                                                      real gamma) {
+                                                      real gamma, const MlpPacked* mlp) {
     __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
-    flux_tile_xy_body<DIR, CONV, BORGES>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
-                                         TILE_L, tile);
+    flux_tile_xy_body<DIR, CONV, WT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
+                                     TILE_L, tile, mlp);
 }
 
 // z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
 // each, a block 8 waves; each row stages the split values of its kw + 5 cells in LDS.
 // (bx, by, bz) = (segment, row tile, i).
 constexpr int ZLDS_WAVE_MAX = 4 * (16 + 5) * 10;  // the largest per-wave need (kw = 16)
-template <int CONV, bool BORGES>
+template <int CONV, int WT>
 __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
                                                  const double* __restrict__ W,
                                                  double* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
-                                                 real* __restrict__ lds) {
+                                                 real* __restrict__ lds,
+                                                 const MlpPacked* __restrict__ mlp) {
     const int NC   = kw + 5;
     const int sub  = TILE_L / kw;                   // rows per wave
     const int lane = threadIdx.x, r = threadIdx.y;
@@ -512,10 +567,13 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
         }
     }
     __syncthreads();  // rows are independent; each wave only reads its own rows
-    const int fidx = f0 + kk;
-    if (!rowok || fidx >= g.nz) return;
-    const int  kf = fidx + NGC - 1;
-    const long c  = row + kf;
+    const int  fidx  = f0 + kk;
+    const bool valid = rowok && fidx < g.nz;
+    if constexpr (WT != 2) {  // (see flux_tile_xy_body)
+        if (!valid) return;
+    }
+    const int  kf = (valid ? fidx : 0) + NGC - 1;
+    const long c  = (valid ? row : (long)i * g.sx + (long)NGC * g.sy) + kf;
     real     Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
@@ -527,34 +585,40 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
         __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
     } st{&T[kk], NC};
     real out[5];
-    face_flux<2, CONV, BORGES>(st, Wl, Wr, gamma, out);
+    face_flux<2, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int CONV, bool BORGES>
+template <int CONV, int WT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      double* __restrict__ f, Geo g,
-                                                     real gamma) {
+                                                     real gamma, const MlpPacked* mlp) {
     __shared__ real tile[8 * (TILE_L + 5) * 10];
-    flux_tile_z_body<CONV, BORGES>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z, TILE_L,
-                                   tile);
+    flux_tile_z_body<CONV, WT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z, TILE_L,
+                               tile, mlp);
 }
 
 // one plane of faces (sweep index `i`), one thread per face, split values in registers;
 // block (64, 4) over the tangential indices, (bx, by) = (fast, slow) tile
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
                                                 const double* __restrict__ W,
                                                 double* __restrict__ f, const Geo& g, real gamma,
-                                                int i, int bx, int by) {
+                                                int i, int bx, int by,
+                                                const MlpPacked* __restrict__ mlp) {
     // tangential indices (t1 slow, t2 fast) over the interior
-    int t2 = threadIdx.x + bx * 64;
-    int t1 = threadIdx.y + by * 4;
-    int n1 = (DIR == 0) ? g.ny : g.nx;
-    int n2 = (DIR == 2) ? g.ny : g.nz;
-    if (t1 >= n1 || t2 >= n2) return;
+    int        t2    = threadIdx.x + bx * 64;
+    int        t1    = threadIdx.y + by * 4;
+    int        n1    = (DIR == 0) ? g.ny : g.nx;
+    int        n2    = (DIR == 2) ? g.ny : g.nz;
+    const bool valid = t1 < n1 && t2 < n2;
+    if constexpr (WT != 2) {  // (see flux_tile_xy_body)
+        if (!valid) return;
+    }
+    if (!valid) t1 = 0, t2 = 0;
     long c, s;
     if (DIR == 0) c = (long)i * g.sx + (long)(t1 + NGC) * g.sy + (t2 + NGC), s = g.sx;
     else if (DIR == 1) c = (long)(t1 + NGC) * g.sx + (long)i * g.sy + (t2 + NGC), s = g.sy;
@@ -569,17 +633,18 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
     }
     RegSplit st{sv};
     real   out[5];
-    face_flux<DIR, CONV, BORGES>(st, Wl, Wr, gamma, out);
+    face_flux<DIR, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
                                                     const double* __restrict__ W,
                                                     double* __restrict__ f, Geo g, real gamma,
-                                                    int i) {
-    flux_plane_body<DIR, CONV, BORGES>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y);
+                                                    int i, const MlpPacked* mlp) {
+    flux_plane_body<DIR, CONV, WT>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y, mlp);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1326,7 +1391,7 @@ __global__ void kb_halo(EnsTable T, int s) {
     halo_body(U, 5, Wp, 5, g, bd, bx, by, bz);
 }
 
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T, int s) {
     __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
     int             bx, by, bz;
@@ -1336,10 +1401,11 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     const int       kw  = E->t.kw_flux;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
-    flux_tile_xy_body<DIR, CONV, BORGES>(U, Wp, f, g, gam, bx, by, bz, kw, tile);
+    const MlpPacked* mlp = E->ph.mlp;
+    flux_tile_xy_body<DIR, CONV, WT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
 }
 
-template <int CONV, bool BORGES>
+template <int CONV, int WT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T, int s) {
     __shared__ real tile[8 * ZLDS_WAVE_MAX];
     int             bx, by, bz;
@@ -1349,10 +1415,11 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T,
     const int       kw  = E->t.kw_flux;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[2];
-    flux_tile_z_body<CONV, BORGES>(U, Wp, f, g, gam, bx, by, bz, kw, tile);
+    const MlpPacked* mlp = E->ph.mlp;
+    flux_tile_z_body<CONV, WT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
 }
 
-template <int DIR, int CONV, bool BORGES>
+template <int DIR, int CONV, int WT>
 __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_PLANE_X + DIR, bx, by, bz);
@@ -1361,7 +1428,8 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     const int       n   = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
-    flux_plane_body<DIR, CONV, BORGES>(U, Wp, f, g, gam, NGC + n - 1, bx, by);
+    const MlpPacked* mlp = E->ph.mlp;
+    flux_plane_body<DIR, CONV, WT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp);
 }
 
 template <int ORDER>
@@ -1408,6 +1476,25 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
 }
 
 // ------------------------------------------------------------------------------------------
+// the learned reconstruction on a batch of stencils (checks of the device function the face
+// kernels use); one stencil per lane, 64 lanes per block
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mlp_recon(const MlpPacked* __restrict__ P, long n,
+                                                  const double* __restrict__ S, int side,
+                                                  double* __restrict__ out, int* __restrict__ regime) {
+    const long s     = threadIdx.x + (long)blockIdx.x * 64;
+    const bool valid = s < n;
+    const long sc    = valid ? s : 0;
+    real       v[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] = (real)S[sc * 5 + k];
+    int  rg = 0;
+    real r  = side ? weno_learned<1>(v[0], v[1], v[2], v[3], v[4], Learn{P, valid}, &rg)
+                   : weno_learned<0>(v[0], v[1], v[2], v[3], v[4], Learn{P, valid}, &rg);
+    if (valid) out[s] = (double)r, regime[s] = rg;
+}
+
+// ------------------------------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------------------------------
 static dim3 cell_grid(int n2, int n1, int n0, dim3 blk) {
@@ -1441,18 +1528,18 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
 
-template <int DIR, int CONV, bool B>
+template <int DIR, int CONV, int B>
 static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
-                     hipStream_t s) {
+                     const MlpPacked* mlp, hipStream_t s) {
     dim3 blk(TILE_L, 8, 1);
     if constexpr (DIR == 2) {
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
-        k_flux_tile_z<CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma);
+        k_flux_tile_z<CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     } else {
         int  n = (DIR == 0) ? g.nx : g.ny;
         int  o = (DIR == 0) ? g.ny : g.nx;
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S - 1) / TILE_S);
-        k_flux_tile_xy<DIR, CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma);
+        k_flux_tile_xy<DIR, CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     }
     // the last face plane (sweep index NGC + n - 1)
     int  n  = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
@@ -1460,18 +1547,21 @@ static void flux_dir(const double* U, const double* W, double* f, Geo g, double 
     int  n2 = (DIR == 2) ? g.ny : g.nz;
     dim3 pb(64, 4, 1);
     dim3 pg((n2 + 63) / 64, (n1 + 3) / 4, 1);
-    k_flux_plane<DIR, CONV, B><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1);
+    k_flux_plane<DIR, CONV, B><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp);
 }
 
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
                           hipStream_t s) {
+    const MlpPacked* m = ph.mlp;
     if (ph.conv == 1) {
-        if (ph.borges) flux_dir<DIR, 1, true>(U, W, f, g, ph.gamma, s);
-        else flux_dir<DIR, 1, false>(U, W, f, g, ph.gamma, s);
+        if (ph.weights == 1) flux_dir<DIR, 1, 1>(U, W, f, g, ph.gamma, m, s);
+        else if (ph.weights == 2) flux_dir<DIR, 1, 2>(U, W, f, g, ph.gamma, m, s);
+        else flux_dir<DIR, 1, 0>(U, W, f, g, ph.gamma, m, s);
     } else {
-        if (ph.borges) flux_dir<DIR, 2, true>(U, W, f, g, ph.gamma, s);
-        else flux_dir<DIR, 2, false>(U, W, f, g, ph.gamma, s);
+        if (ph.weights == 1) flux_dir<DIR, 2, 1>(U, W, f, g, ph.gamma, m, s);
+        else if (ph.weights == 2) flux_dir<DIR, 2, 2>(U, W, f, g, ph.gamma, m, s);
+        else flux_dir<DIR, 2, 0>(U, W, f, g, ph.gamma, m, s);
     }
 }
 
@@ -1559,7 +1649,7 @@ static void l_selftest_div(long n, unsigned long long seed, unsigned long long* 
     k_selftest_div<<<1024, 256, 0, s>>>(n, seed, count);
 }
 
-template <int CONV, bool B>
+template <int CONV, int B>
 static void ens_fluxes(const EnsTable& T, const int* total, int s, hipStream_t st) {
     dim3 tb(TILE_L, 8, 1), pb(64, 4, 1);
     if (total[EF_FLUX_X]) kb_flux_tile_xy<0, CONV, B><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
@@ -1574,11 +1664,13 @@ template <int ORDER, bool SUTH>
 static void ens_stage(const EnsTable& T, const int* total, const Phys& ph, int s, hipStream_t st) {
     kb_halo<<<total[EF_HALO], dim3(64, 4, 1), 0, st>>>(T, s);
     if (ph.conv == 1) {
-        if (ph.borges) ens_fluxes<1, true>(T, total, s, st);
-        else ens_fluxes<1, false>(T, total, s, st);
+        if (ph.weights == 1) ens_fluxes<1, 1>(T, total, s, st);
+        else if (ph.weights == 2) ens_fluxes<1, 2>(T, total, s, st);
+        else ens_fluxes<1, 0>(T, total, s, st);
     } else if (ph.conv == 2) {
-        if (ph.borges) ens_fluxes<2, true>(T, total, s, st);
-        else ens_fluxes<2, false>(T, total, s, st);
+        if (ph.weights == 1) ens_fluxes<2, 1>(T, total, s, st);
+        else if (ph.weights == 2) ens_fluxes<2, 2>(T, total, s, st);
+        else ens_fluxes<2, 0>(T, total, s, st);
     }
     if constexpr (ORDER > 0) {
         kb_deriv1<ORDER><<<total[EF_DERIV1], 256, 0, st>>>(T, s);
@@ -1606,11 +1698,17 @@ static void l_ens_step(const EnsTable& T, const int* total, const Phys& ph, hipS
     }
 }
 
+static void l_mlp_recon(const MlpPacked* P, long n, const double* S, int side, double* out,
+                        int* regime, hipStream_t s) {
+    k_mlp_recon<<<(unsigned)((n + 63) / 64), 64, 0, s>>>(P, n, S, side, out, regime);
+}
+
 const Launchers& launchers() {
     static const Launchers L = {l_reset_min,    l_step_begin, l_halo,       l_c2p_faces,
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
-                                l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step};
+                                l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
+                                l_mlp_recon};
     return L;
 }
 

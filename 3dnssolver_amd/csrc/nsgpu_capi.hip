@@ -18,6 +18,7 @@
 
 #include "../../include/nsgpu.h"
 #include "ns_common.h"
+#include "ns_mlp.h"
 
 using namespace nsgpu;
 
@@ -58,6 +59,8 @@ struct nsgpu_ctx {
     double*            dtime = nullptr;   // device time accumulator
     double*            sums  = nullptr;   // [5]
     double*            Ulast = nullptr;   // buffer holding the last step's U
+    float*             mlp_params = nullptr;  // learned weights: FP32 parameters
+    MlpPacked*         mlp_packed = nullptr;  // and their packed BF16 fragments
     bool               timing = false;
     hipEvent_t         ev[2 * T_COUNT] = {};
     double             ms[T_COUNT]     = {};
@@ -100,7 +103,9 @@ int validate(const nsgpu_config* c) {
     if (!(c->Lx > 0) || !(c->Ly > 0) || !(c->Lz > 0)) return NSGPU_ERR_INVALID;
     if (!(c->gamma > 1.0) || !(c->cfl > 0)) return NSGPU_ERR_INVALID;
     if (c->conv < NSGPU_CONV_NONE || c->conv > NSGPU_CONV_WENO5CW) return NSGPU_ERR_INVALID;
-    if (c->weights != NSGPU_WEIGHTS_SHU && c->weights != NSGPU_WEIGHTS_BORGES) return NSGPU_ERR_INVALID;
+    if (c->weights != NSGPU_WEIGHTS_SHU && c->weights != NSGPU_WEIGHTS_BORGES &&
+        c->weights != NSGPU_WEIGHTS_LEARNED)
+        return NSGPU_ERR_INVALID;
     if (c->diff_order != 0 && c->diff_order != 2 && c->diff_order != 4 && c->diff_order != 6)
         return NSGPU_ERR_INVALID;
     if (c->math != NSGPU_MATH_EXACT && c->math != NSGPU_MATH_FAST && c->math != NSGPU_MATH_FP32)
@@ -143,6 +148,9 @@ int download_soa(nsgpu_ctx* c, const double* src, double* aos) {
     HIPCHK(hipStreamSynchronize(c->stream));
     return NSGPU_OK;
 }
+
+// the learned weights need their network
+bool net_ready(const nsgpu_ctx* c) { return c->ph.weights != NSGPU_WEIGHTS_LEARNED || c->ph.mlp; }
 
 // convection face fluxes of the current (U, W); fz stays zero for dim == 2
 void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W) {
@@ -218,7 +226,8 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     for (int a = 0; a < 3; a++) c->b.periodic[a] = (cfg->bc[lead[a]] == NSGPU_BC_PERIODIC);
     for (int f = 0; f < 6; f++) c->b.face_kind[f] = cfg->bc[f];
     c->ph.gamma = cfg->gamma, c->ph.mu = cfg->mu, c->ph.Pr = cfg->Pr, c->ph.cfl = cfg->cfl;
-    c->ph.conv = cfg->conv, c->ph.borges = (cfg->weights == NSGPU_WEIGHTS_BORGES);
+    c->ph.conv = cfg->conv, c->ph.weights = cfg->weights;
+    c->ph.mlp  = nullptr;  // set by nsgpu_set_mlp
     c->ph.diff_order = cfg->diff_order;
     c->ph.lamb       = (-2. / 3.) * cfg->mu;                           // centered_diffusion.cpp:101
     c->ph.two_mu     = 2.0 * cfg->mu;
@@ -250,6 +259,10 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     if (!rc) rc = alloc((double**)&c->flags, sizeof(double), c->stream);
     if (!rc) rc = alloc(&c->dtime, sizeof(double), c->stream);
     if (!rc) rc = alloc(&c->sums, 5 * sizeof(double), c->stream);
+    if (!rc && cfg->weights == NSGPU_WEIGHTS_LEARNED) {
+        rc = alloc((double**)&c->mlp_params, sizeof(float) * MLP_NPARAM, c->stream);
+        if (!rc) rc = alloc((double**)&c->mlp_packed, sizeof(MlpPacked), c->stream);
+    }
     for (int t = 0; t < 2 * T_COUNT && !rc; t++)
         if (hipEventCreate(&c->ev[t]) != hipSuccess) rc = NSGPU_ERR_HIP;
     c->Ulast = c->Ua;
@@ -268,7 +281,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     double* bufs[] = {c->W,  c->W2, c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub, c->F[0], c->F[1], c->F[2], c->Dx, c->Dy,
                       c->R,  c->stage_aos, c->partial, (double*)c->dtmin, (double*)c->flags,
-                      c->dtime, c->sums};
+                      c->dtime, c->sums, (double*)c->mlp_params, (double*)c->mlp_packed};
     for (double* p : bufs)
         if (p) hipFree(p);
     for (int t = 0; t < 2 * T_COUNT; t++)
@@ -316,7 +329,7 @@ int nsgpu_apply_boundary(nsgpu_ctx* c, double* aos) {
 }
 
 int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_aos, double* R_aos) {
-    if (!c || !W_aos || !U_aos || !R_aos) return NSGPU_ERR_INVALID;
+    if (!c || !W_aos || !U_aos || !R_aos || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     int rc = upload_soa(c, W_aos, c->W);
     if (!rc) rc = upload_soa(c, U_aos, c->Ub);
@@ -347,7 +360,7 @@ int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
 }
 
 int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int which) {
-    if (!c || !U_aos || !R_out || which < 1 || which > 3) return NSGPU_ERR_INVALID;
+    if (!c || !U_aos || !R_out || which < 1 || which > 3 || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     int rc = upload_soa(c, U_aos, c->Ua);
     if (rc) return rc;
@@ -374,7 +387,7 @@ int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int whi
 }
 
 int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
-    if (!c || nsteps < 0) return NSGPU_ERR_INVALID;
+    if (!c || nsteps < 0 || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     if (time_io) HIPCHK(hipMemcpyAsync(c->dtime, time_io, sizeof(double), hipMemcpyHostToDevice, c->stream));
     const Launchers& L = *c->L;
@@ -483,6 +496,59 @@ int nsgpu_selftest(int device, long n, unsigned long long seed, long* mismatches
     if (!rc && hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) rc = NSGPU_ERR_HIP;
     hipFree(d);
     *mismatches = (long)h;
+    return rc;
+}
+
+int nsgpu_set_mlp(nsgpu_ctx* c, const float* params) {
+    if (!c || !params || c->ph.weights != NSGPU_WEIGHTS_LEARNED) return NSGPU_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(c->mlp_params, params, sizeof(float) * MLP_NPARAM,
+                          hipMemcpyHostToDevice, c->stream));
+    mlp_pack(c->mlp_params, c->mlp_packed, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ph.mlp = c->mlp_packed;
+    return NSGPU_OK;
+}
+
+int nsgpu_mlp_recon(int device, int math, const float* params, long n, const double* S, int side,
+                    double* out, int* regime) {
+    if (!params || !S || !out || !regime || n < 1 || n > (1L << 30) || side < 0 || side > 1 ||
+        math < NSGPU_MATH_EXACT || math > NSGPU_MATH_FP32)
+        return NSGPU_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return NSGPU_ERR_NODEVICE;
+    if (device < 0 || device >= ndev) return NSGPU_ERR_NODEVICE;
+    HIPCHK(hipSetDevice(device));
+    const Launchers& L = (math == NSGPU_MATH_FAST)   ? fast::launchers()
+                         : (math == NSGPU_MATH_FP32) ? fp32::launchers()
+                                                     : exact::launchers();
+    float*     dp = nullptr;
+    MlpPacked* P  = nullptr;
+    double *   dS = nullptr, *dout = nullptr;
+    int*       drg = nullptr;
+    int        rc  = NSGPU_OK;
+    if (hipMalloc((void**)&dp, sizeof(float) * MLP_NPARAM) != hipSuccess ||
+        hipMalloc((void**)&P, sizeof(MlpPacked)) != hipSuccess ||
+        hipMalloc((void**)&dS, sizeof(double) * 5 * n) != hipSuccess ||
+        hipMalloc((void**)&dout, sizeof(double) * n) != hipSuccess ||
+        hipMalloc((void**)&drg, sizeof(int) * n) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc && hipMemcpy(dp, params, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc && hipMemcpy(dS, S, sizeof(double) * 5 * n, hipMemcpyHostToDevice) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc) {
+        mlp_pack(dp, P, nullptr);
+        L.mlp_recon(P, n, dS, side, dout, drg, nullptr);
+        if (hipGetLastError() != hipSuccess) rc = NSGPU_ERR_HIP;
+    }
+    if (!rc && hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc && hipMemcpy(regime, drg, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    for (void* q : {(void*)dp, (void*)P, (void*)dS, (void*)dout, (void*)drg})
+        if (q) hipFree(q);
     return rc;
 }
 
@@ -679,7 +745,8 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
     for (int b = 0; b < n; b++) {
         const nsgpu_ctx* c = members[b];
         if (!c || c->device != c0->device || c->L != c0->L || c->ph.conv != c0->ph.conv ||
-            c->ph.borges != c0->ph.borges || c->ph.diff_order != c0->ph.diff_order ||
+            c->ph.weights != c0->ph.weights || c->ph.diff_order != c0->ph.diff_order ||
+            (c->ph.weights == NSGPU_WEIGHTS_LEARNED && !c->ph.mlp) ||
             c->ph.suth != c0->ph.suth || c->g.dim != c0->g.dim)
             return NSGPU_ERR_INVALID;
         for (int a = 0; a < b; a++)

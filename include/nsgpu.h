@@ -56,7 +56,9 @@ enum {
  * no boundary object, domain.cpp:87-137). */
 enum { NSGPU_BC_PERIODIC = 0, NSGPU_BC_NEUMANN = 1 };
 enum { NSGPU_CONV_NONE = 0, NSGPU_CONV_WENO5 = 1, NSGPU_CONV_WENO5CW = 2 };
-enum { NSGPU_WEIGHTS_SHU = 0, NSGPU_WEIGHTS_BORGES = 1 };
+/* LEARNED (extension): the WENO-5 nonlinear weights come from a small network, see
+ * nsgpu_set_mlp below; the classic Shu weights are kept on smooth and flagged stencils */
+enum { NSGPU_WEIGHTS_SHU = 0, NSGPU_WEIGHTS_BORGES = 1, NSGPU_WEIGHTS_LEARNED = 2 };
 /* arithmetic of the kernels:
  *   EXACT  the reference's FP64 operation order, no contraction, correctly rounded division and
  *          sqrt: results are bitwise those of a strict (-ffp-contract=off) build of the reference
@@ -140,6 +142,35 @@ int nsgpu_timing(nsgpu_ctx* ctx, int enable);
 int nsgpu_timer_count(void);
 const char* nsgpu_timer_name(int i);
 int nsgpu_timer_read(nsgpu_ctx* ctx, int i, double* ms, long* launches);
+
+/* ---- Learned WENO-5 weights -------------------------------------------------------------------
+ * No reference counterpart (the reference's WENO5_weights_shu / _borges, weno-5.cpp:222-283, are
+ * what it replaces).  An 8-32-32-3 MLP (GELU, softmax) maps the five stencil values normalised
+ * by their range, (v - min) / (max - min), and their three smoothness indicators to the three
+ * weights.  Stencils with (max beta + 1e-6) / (min beta + 1e-6) below 3 (smooth) or above 1e3
+ * (flagged discontinuity) keep the classic Shu weights, bitwise as the SHU path computes them.
+ * Parameters: NSGPU_MLP_NPARAM floats, row-major W1[32][8], b1[32], W2[32][32], b2[32],
+ * W3[3][32], b3[3].  The kernels use them as bfloat16 with FP32 accumulation (matrix cores). */
+#define NSGPU_MLP_NPARAM 1443
+
+/* Network of a context created with weights = NSGPU_WEIGHTS_LEARNED (required before stepping). */
+int nsgpu_set_mlp(nsgpu_ctx* ctx, const float* params);
+
+/* Batch inference on GPU `device`: softmax weights W[n][3] of the features X[n][8]. */
+int nsgpu_mlp_forward(int device, const float* params, long n, const float* X, float* W);
+/* Mean over n samples of (sum_k w_k Q[s][k] - T[s])^2 and its parameter gradient. */
+int nsgpu_mlp_loss_grad(int device, const float* params, long n, const float* X, const float* Q,
+                        const float* T, double* loss, float* grad);
+/* Adam training on the GPU (lr, beta 0.9 / 0.999, eps 1e-8): `steps` steps on consecutive
+ * mini-batches of `batch` samples of (X, Q, T), wrapping around; params updated in place,
+ * loss_hist[step] = mini-batch loss before the step (may be NULL). */
+int nsgpu_mlp_train(int device, float* params, long n, const float* X, const float* Q,
+                    const float* T, int steps, long batch, float lr, float* loss_hist);
+/* The learned reconstruction of the face-flux kernels of math mode `math` on n stencils
+ * S[n][5]: side 0 = WENO5_left, 1 = WENO5_right; out[n], regime[n] (0 smooth, 1 network,
+ * 2 flagged). */
+int nsgpu_mlp_recon(int device, int math, const float* params, long n, const double* S, int side,
+                    double* out, int* regime);
 
 /* ---- Ensembles: many independent blocks of different sizes on one GPU ----------------------
  * No reference counterpart: the reference advances one block per MPI rank, each with its own

@@ -273,6 +273,10 @@ static void weno_weights(double fm2, double fm1, double f0, double fp1, double f
 
 /* Weno5::WENO5_left, weno-5.cpp:305-322 */
 static double weno_left(double a, double b, double c, double d, double e, int type) {
+    if (type == NSO_W_LEARNED) {
+        const double f[5] = {a, b, c, d, e};
+        return nso_weno5_learned_g(f, 0);
+    }
     double f0 = (2. * a - 7. * b + 11. * c) * onepersix;
     double f1 = (-b + 5. * c + 2. * d) * onepersix;
     double f2 = (2. * c + 5. * d - e) * onepersix;
@@ -283,6 +287,10 @@ static double weno_left(double a, double b, double c, double d, double e, int ty
 
 /* Weno5::WENO5_right, weno-5.cpp:285-303 */
 static double weno_right(double a, double b, double c, double d, double e, int type) {
+    if (type == NSO_W_LEARNED) {
+        const double f[5] = {a, b, c, d, e};
+        return nso_weno5_learned_g(f, 1);
+    }
     double f0 = (-a + 5. * b + 2. * c) * onepersix;
     double f1 = (2. * b + 5. * c - d) * onepersix;
     double f2 = (11. * c - 7. * d + 2. * e) * onepersix;

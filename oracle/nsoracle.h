@@ -26,7 +26,7 @@ extern "C" {
  * Front entry (decomposition.cpp:145-161). */
 enum { NSO_BC_PERIODIC = 0, NSO_BC_NEUMANN = 1 };
 enum { NSO_CONV_NONE = 0, NSO_CONV_WENO5 = 1, NSO_CONV_WENO5CW = 2 };
-enum { NSO_W_SHU = 0, NSO_W_BORGES = 1 };
+enum { NSO_W_SHU = 0, NSO_W_BORGES = 1, NSO_W_LEARNED = 2 };
 
 typedef struct {
     int    nx, ny, nz; /* interior cells of the block */
@@ -73,6 +73,37 @@ void   nso_eigenvectors(const double Wavg[5], double gamma, int dir, double L[25
 void   nso_fd(const nso_cfg* c, const double* f, double* df, int dir, int second, double d);
 int    nso_num_threads(void);
 double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double p);
+
+/* ---- learned WENO-5 weights (extension, nsmlp.c) -------------------------------------------
+ * An 8-32-32-3 MLP (GELU, softmax) maps the range-normalised stencil and its three smoothness
+ * indicators to the three WENO weights.  Faces whose indicator ratio
+ * (max beta + EPS) / (min beta + EPS) is below SMOOTH (smooth) or above FLAG (flagged
+ * discontinuity) keep the classic Shu weights.  Parameter layout (row-major):
+ * W1[32][8], b1[32], W2[32][32], b2[32], W3[3][32], b3[3]. */
+#define NSO_MLP_HIDDEN 32
+#define NSO_MLP_NPARAM (32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3)
+#define NSO_MLP_EPS 1e-6
+#define NSO_MLP_SMOOTH 3.0
+#define NSO_MLP_FLAG 1e3
+enum { NSO_REGIME_SMOOTH = 0, NSO_REGIME_MLP = 1, NSO_REGIME_FLAGGED = 2 };
+
+double nso_bf16(double v);
+/* x[0..4] = (v - min) / (max - min) (0 when max == min), x[5..7] = beta_0..2 of x */
+void   nso_mlp_features(const double v[5], double x[8]);
+int    nso_mlp_regime(const double x[8]);
+/* the three left-biased ENO candidates of the normalised stencil x[0..4] */
+void   nso_mlp_candidates(const double x[8], double q[3]);
+/* softmax weights; bf16 != 0 rounds the operands of the matrix products like the GPU kernels */
+void   nso_mlp_forward(const double* params, const double x[8], int bf16, double w[3]);
+/* mean squared error of sum_k w_k Q[s][k] against T[s] over n samples, and its gradient */
+double nso_mlp_loss_grad(const double* params, long n, const double* X, const double* Q,
+                         const double* T, int bf16, double* grad);
+/* learned reconstruction of one stencil, side 0 = WENO5_left, 1 = WENO5_right */
+void   nso_weno5_learned(const double* params, int bf16, const double f[5], int side,
+                         double out[1], int* regime);
+/* network used by nso_rhs / nso_step when cfg.weights == NSO_W_LEARNED */
+void   nso_set_mlp(const double* params, int bf16);
+double nso_weno5_learned_g(const double f[5], int side);
 
 #ifdef __cplusplus
 }

@@ -17,7 +17,9 @@ NGC = 3
 
 BC = {"periodic": 0, "neumann": 1}
 CONV = {"none": 0, "weno5": 1, "weno5cw": 2}
-WEIGHTS = {"shu": 0, "borges": 1}
+WEIGHTS = {"shu": 0, "borges": 1, "learned": 2}
+MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3
+REGIMES = {"smooth": 0, "mlp": 1, "flagged": 2}
 VISCOSITY = {"constant": 0, "sutherland": 1}
 SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 
@@ -112,6 +114,12 @@ class Oracle(_Lib):
         L.nso_weno5_right.restype = None
         L.nso_mu_sutherland.restype = ctypes.c_double
         L.nso_mu_sutherland.argtypes = [ctypes.c_double] * 5
+        L.nso_bf16.restype = ctypes.c_double
+        L.nso_bf16.argtypes = [ctypes.c_double]
+        L.nso_mlp_regime.restype = ctypes.c_int
+        L.nso_mlp_loss_grad.restype = ctypes.c_double
+        L.nso_mlp_loss_grad.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_long] + \
+            [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
 
     def dt(self, case, W):
         cfg = make_cfg(case)
@@ -160,6 +168,55 @@ class Oracle(_Lib):
         self.lib.nso_eigenvectors(_ptr(Wavg), ctypes.c_double(gamma), ctypes.c_int(direction),
                                   _ptr(L), _ptr(R))
         return L.reshape(5, 5), R.reshape(5, 5)
+
+    # ---- learned weights ------------------------------------------------------------------
+    def bf16(self, v):
+        return self.lib.nso_bf16(float(v))
+
+    def mlp_features(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        x = np.zeros(8)
+        self.lib.nso_mlp_features(_ptr(v), _ptr(x))
+        return x
+
+    def mlp_regime(self, x):
+        return self.lib.nso_mlp_regime(_ptr(np.ascontiguousarray(x, dtype=np.float64)))
+
+    def mlp_candidates(self, x):
+        q = np.zeros(3)
+        self.lib.nso_mlp_candidates(_ptr(np.ascontiguousarray(x, dtype=np.float64)), _ptr(q))
+        return q
+
+    def mlp_forward(self, params, x, bf16=False):
+        w = np.zeros(3)
+        self.lib.nso_mlp_forward(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                 _ptr(np.ascontiguousarray(x, dtype=np.float64)),
+                                 ctypes.c_int(int(bf16)), _ptr(w))
+        return w
+
+    def mlp_loss_grad(self, params, X, Q, T, bf16=False, grad=True):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        g = np.zeros(MLP_NPARAM) if grad else None
+        loss = self.lib.nso_mlp_loss_grad(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                          len(T), _ptr(X), _ptr(Q), _ptr(T), int(bf16),
+                                          _ptr(g) if grad else None)
+        return (loss, g) if grad else loss
+
+    def weno_learned(self, params, f, side, bf16=True):
+        """(value, regime) of the learned reconstruction; side 0 = left-, 1 = right-biased."""
+        out = np.zeros(1)
+        rg = ctypes.c_int(-1)
+        self.lib.nso_weno5_learned(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                                   ctypes.c_int(int(bf16)),
+                                   _ptr(np.ascontiguousarray(f, dtype=np.float64)),
+                                   ctypes.c_int(side), _ptr(out), ctypes.byref(rg))
+        return out[0], rg.value
+
+    def set_mlp(self, params, bf16=True):
+        self.lib.nso_set_mlp(_ptr(np.ascontiguousarray(params, dtype=np.float64)),
+                             ctypes.c_int(int(bf16)))
 
     def fd(self, case, f, direction, second, d):
         cfg = make_cfg(case)

@@ -44,7 +44,7 @@ def _create(case, **over):
 
 
 @pytest.mark.parametrize("over", [
-    dict(ngc=2), dict(nx=2), dict(dim=1), dict(diff_order=3), dict(conv=3), dict(weights=2),
+    dict(ngc=2), dict(nx=2), dict(dim=1), dict(diff_order=3), dict(conv=3), dict(weights=3),
     dict(math=5), dict(gamma=1.0), dict(Lx=0.0),
     dict(bc=[1, 0, 0, 0, 0, 0]),   # x not periodic (West Neumann) but East periodic
     dict(bc=[0, 0, 0, 0, 1, 0]),   # z not periodic (Front Neumann) but Back periodic
