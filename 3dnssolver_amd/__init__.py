@@ -42,7 +42,7 @@ EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len",
            "nsgpu_timer_read", "nsgpu_ensemble_create", "nsgpu_ensemble_destroy",
            "nsgpu_ensemble_set_tiles", "nsgpu_ensemble_get_tiles", "nsgpu_ensemble_step",
            "nsgpu_ensemble_tune", "nsgpu_set_mlp", "nsgpu_mlp_forward", "nsgpu_mlp_loss_grad",
-           "nsgpu_mlp_train", "nsgpu_mlp_recon"]
+           "nsgpu_mlp_train", "nsgpu_mlp_recon", "nsgpu_step_fixed"]
 
 
 class Config(ctypes.Structure):
@@ -92,6 +92,7 @@ def load_library(path=None):
     L.nsgpu_diffusion_solve.argtypes = [P, D, D]
     L.nsgpu_rhs.argtypes = [P, D, D, D, ctypes.c_int]
     L.nsgpu_step.argtypes = [P, ctypes.c_int, D, D]
+    L.nsgpu_step_fixed.argtypes = [P, ctypes.c_int, ctypes.c_double, D]
     L.nsgpu_get_dt.argtypes = [P, D]
     L.nsgpu_reduce.argtypes = [P, D, D, ctypes.POINTER(ctypes.c_uint)]
     L.nsgpu_stream.argtypes = [P]
@@ -290,6 +291,12 @@ class Solver:
         dt = np.zeros(1)
         self._check(self.L.nsgpu_step(self.h, nsteps, _ptr(t), _ptr(dt)))
         return float(t[0]), float(dt[0])
+
+    def step_fixed(self, nsteps, dt, time=0.0):
+        """nsteps steps with the prescribed dt; returns the time."""
+        t = np.array([time], dtype=np.float64)
+        self._check(self.L.nsgpu_step_fixed(self.h, nsteps, dt, _ptr(t)))
+        return float(t[0])
 
     @property
     def dt(self):

@@ -200,3 +200,36 @@ def config5(n_blocks=64, seed=5, edges=ENSEMBLE_EDGES):
         W = shear_W(case, M=M, amp=amp, seed=int(rng.integers(1 << 30)))
         out.append((case, W, dict(M=M, Re=Re, amp=amp)))
     return out
+
+
+def jump_W(case, seed=4, rho_ratio=8.0, p_ratio=10.0, width=0.02):
+    """Sod-like planar density / pressure jump at a seeded random orientation, periodic: the
+    slabs where sin(2 pi k.x + phi) > 0 hold the high state (rho 1, p 1), the others the low one
+    (rho 1/rho_ratio, p 1/p_ratio); k is a random integer lattice vector (so the planes are
+    periodic in [0,1]^3) and the jump is smoothed over `width` (a tanh of the phase)."""
+    rng = np.random.default_rng(seed)
+    while True:
+        k = rng.integers(-2, 3, size=3)
+        if (k != 0).sum() >= 2:
+            break
+    phi = rng.random() * 2.0 * np.pi
+    X, Y, Z = _centres(case)
+    Lx, Ly, Lz = case["L"]
+    s = np.sin(2 * np.pi * (k[0] * X / Lx + k[1] * Y / Ly + k[2] * Z / Lz) + phi)
+    h = 0.5 * (1.0 + np.tanh(s / width))
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    W[I + (0,)] = 1.0 / rho_ratio + (1.0 - 1.0 / rho_ratio) * h
+    W[I + (1,)] = 1.0 / p_ratio + (1.0 - 1.0 / p_ratio) * h
+    return W
+
+
+def config4(n=256, problem="shear"):
+    """Learned form evaluation problems on [0,1]^3: the double shear layer of config1 at n^3
+    (component-wise WENO-5 + CD4, Mach 0.3, Re 1000) or the Sod-like planar jump (density 8:1,
+    pressure 10:1, seeded orientation) with the same scheme."""
+    case, W = config1(n)
+    if problem == "jump":
+        case = dict(case, mu=1e-4)
+        W = jump_W(case)
+    return case, W

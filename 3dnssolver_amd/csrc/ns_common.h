@@ -163,6 +163,8 @@ struct Launchers {
     // 1 right-biased): out[n] and the regime of each stencil (0 smooth, 1 network, 2 flagged)
     void (*mlp_recon)(const MlpPacked* P, long n, const double* S, int side, double* out,
                       int* regime, hipStream_t s);
+    // overwrite the step's CFL minimum with a prescribed dt / CFL
+    void (*set_dt)(unsigned long long* bits, double dt_over_cfl, hipStream_t s);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

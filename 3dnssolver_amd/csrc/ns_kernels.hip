@@ -185,7 +185,8 @@ __device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, rea
 // ------------------------------------------------------------------------------------------
 // learned weights (ns_mlp.h; no reference counterpart).  The network sees the stencil of the
 // left-biased reconstruction; WENO5_right(a..e) is fed mirrored, (e..a).  Smooth and flagged
-// stencils keep the classic Shu reconstruction, computed by the same code as the classic path.
+// stencils keep the classic Shu reconstruction, computed by the same code as the classic path;
+// in between, the network's reconstruction enters with the share mlp_blend(indicator ratio).
 // Every lane of the wavefront must call this (the network uses matrix instructions); `active`
 // marks the lanes whose face is real.
 // ------------------------------------------------------------------------------------------
@@ -199,8 +200,8 @@ __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, rea
                                              const Learn& L, int* regime = nullptr) {
     const real classic = SIDE ? weno_right_c<false>(a, b, c, d, e) : weno_left_c<false>(a, b, c, d, e);
     const real v[5]    = {SIDE ? e : a, SIDE ? d : b, c, SIDE ? b : d, SIDE ? a : e};
-    float      x[8];
-    const int  rg   = mlp_features(v, x);
+    float      x[8], ratio;
+    const int  rg   = mlp_features(v, x, &ratio);
     const bool need = L.active && rg == 1;
     if (regime) *regime = rg;
     if (__ballot(need) == 0) return classic;  // wavefront-uniform
@@ -212,7 +213,8 @@ __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, rea
     const real f0 = (RL(2.) * v[0] - RL(7.) * v[1] + RL(11.) * v[2]) * ONEPERSIX;
     const real f1 = (-v[1] + RL(5.) * v[2] + RL(2.) * v[3]) * ONEPERSIX;
     const real f2 = (RL(2.) * v[2] + RL(5.) * v[3] - v[4]) * ONEPERSIX;
-    return (real)w[0] * f0 + (real)w[1] * f1 + (real)w[2] * f2;
+    const real net = (real)w[0] * f0 + (real)w[1] * f1 + (real)w[2] * f2;
+    return classic + (real)mlp_blend(ratio) * (net - classic);
 }
 
 // weights WT: 0 Shu, 1 Borges, 2 learned
@@ -709,6 +711,11 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* o
 
 __global__ void k_reset_min(unsigned long long* bits) {
     *bits = (unsigned long long)__double_as_longlong(1E10);  // max_dt = 1E10 (time_scheme.cpp:104)
+}
+
+// a prescribed time step: the stages read dt = bits * CFL
+__global__ void k_set_dt(unsigned long long* bits, double dt_over_cfl) {
+    *bits = (unsigned long long)__double_as_longlong(dt_over_cfl);
 }
 
 __global__ void k_zero_flags(unsigned* flags) { *flags = 0u; }
@@ -1506,6 +1513,9 @@ static void l_reset_min(unsigned long long* bits, hipStream_t s) {
     k_reset_min<<<1, 1, 0, s>>>(bits);
 }
 static void l_zero_flags(unsigned* flags, hipStream_t s) { k_zero_flags<<<1, 1, 0, s>>>(flags); }
+static void l_set_dt(unsigned long long* bits, double dt_over_cfl, hipStream_t s) {
+    k_set_dt<<<1, 1, 0, s>>>(bits, dt_over_cfl);
+}
 
 static void l_step_begin(const double* W, double* Ua, double* Un, double* Wst,
                          unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s) {
@@ -1708,7 +1718,7 @@ const Launchers& launchers() {
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
                                 l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
-                                l_mlp_recon};
+                                l_mlp_recon,    l_set_dt};
     return L;
 }
 

@@ -120,10 +120,19 @@ __device__ __forceinline__ void mlp_softmax(float z0, float z1, float z2, float 
     w[0] = e0 * r, w[1] = e1 * r, w[2] = e2 * r;
 }
 
+// share of the network in the reconstruction as a function of the indicator ratio: rises from
+// 0 at SMOOTH to 1 at 2 SMOOTH, falls from 1 at FLAG / 4 to 0 at FLAG (no jump in the weights
+// where a face changes regime)
+__device__ __forceinline__ float mlp_blend(float ratio) {
+    float up   = fminf(1.0f, (ratio - MLP_SMOOTH) / MLP_SMOOTH);
+    float down = fminf(1.0f, (MLP_FLAG - ratio) / (0.75f * MLP_FLAG));
+    return fmaxf(0.0f, up) * fmaxf(0.0f, down);
+}
+
 // range-normalised stencil and smoothness indicators (features 0..7) of a left-biased stencil,
-// and the regime: 0 smooth, 1 network, 2 flagged discontinuity
+// the regime (0 smooth, 1 network, 2 flagged discontinuity) and the indicator ratio
 template <class T>
-__device__ __forceinline__ int mlp_features(const T v[5], float x[8]) {
+__device__ __forceinline__ int mlp_features(const T v[5], float x[8], float* ratio_out = nullptr) {
     T m = v[0], M = v[0];
 #pragma unroll
     for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
@@ -142,6 +151,7 @@ __device__ __forceinline__ int mlp_features(const T v[5], float x[8]) {
     x[7] = (13.f / 12.f) * t1 * t1 + 0.25f * t2 * t2;
     float bmin = fminf(x[5], fminf(x[6], x[7])), bmax = fmaxf(x[5], fmaxf(x[6], x[7]));
     float ratio = (bmax + MLP_EPS) / (bmin + MLP_EPS);
+    if (ratio_out) *ratio_out = ratio;
     return ratio < MLP_SMOOTH ? 0 : (ratio > MLP_FLAG ? 2 : 1);
 }
 

@@ -386,7 +386,22 @@ int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int whi
     return rc;
 }
 
+namespace {
+int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double* dt_last);
+}
+
 int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
+    return step_impl(c, nsteps, 0.0, time_io, dt_last);
+}
+
+int nsgpu_step_fixed(nsgpu_ctx* c, int nsteps, double dt, double* time_io) {
+    if (!(dt > 0.0)) return NSGPU_ERR_INVALID;
+    return step_impl(c, nsteps, dt, time_io, nullptr);
+}
+
+namespace {
+// nsteps SSP-RK3 steps; fixed_dt > 0 replaces the CFL time step
+int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double* dt_last) {
     if (!c || nsteps < 0 || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     if (time_io) HIPCHK(hipMemcpyAsync(c->dtime, time_io, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -396,6 +411,7 @@ int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
             Scope sc(c, T_STEP_BEGIN);
             L.reset_min(&c->dtmin[0], c->stream);
             L.step_begin(c->W, nullptr, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
+            if (fixed_dt > 0.0) L.set_dt(&c->dtmin[0], fixed_dt / c->ph.cfl, c->stream);
         }
         // stage inputs Un -> Ua -> Ub: Un (Uprev of time_scheme.cpp:27) is never overwritten
         double* cur  = c->Un;
@@ -437,6 +453,7 @@ int nsgpu_step(nsgpu_ctx* c, int nsteps, double* time_io, double* dt_last) {
     if (dt_last) return nsgpu_get_dt(c, dt_last);
     return NSGPU_OK;
 }
+}  // namespace
 
 int nsgpu_get_dt(nsgpu_ctx* c, double* dt) {
     if (!c || !dt) return NSGPU_ERR_INVALID;

@@ -121,6 +121,10 @@ int nsgpu_rhs(nsgpu_ctx* ctx, double* U_aos, double* W_out, double* R_out, int w
  * (may be NULL) and stores the last dt in *dt_last (may be NULL).  Synchronises once at the end. */
 int nsgpu_step(nsgpu_ctx* ctx, int nsteps, double* time_io, double* dt_last);
 int nsgpu_get_dt(nsgpu_ctx* ctx, double* dt);
+/* No reference counterpart (the reference always takes the CFL step, time_scheme.cpp:16):
+ * nsteps steps with a prescribed dt, for convergence and resolution studies that compare runs
+ * at equal times. */
+int nsgpu_step_fixed(nsgpu_ctx* ctx, int nsteps, double dt, double* time_io);
 
 /* Conservation sums over the interior of the resident conservative state (sum of U_q * dx*dy*dz),
  * the CFL time-step minimum of the resident primitive state, and health flags

@@ -13,7 +13,9 @@
  * normalised by their local range plus the three smoothness indicators of the normalised
  * values; the softmax output keeps the weights positive and summing to one (consistency).
  * Classic WENO-5 is kept on faces classified smooth (all indicators alike: the classic weights
- * are already the ideal ones) and on faces flagged as discontinuous (indicators far apart).
+ * are already the ideal ones) and on faces flagged as discontinuous (indicators far apart); in
+ * between, the reconstruction is classic + blend(ratio) (network - classic), with a blend that
+ * rises and falls continuously so the weights do not jump where a face changes regime.
  */
 #include <math.h>
 #include <string.h>
@@ -76,6 +78,24 @@ void nso_mlp_features(const double v[5], double x[8]) {
     double r = M - m;
     for (int k = 0; k < 5; k++) x[k] = (r > 0.0) ? (v[k] - m) / r : 0.0;
     betas(x, x + 5);
+}
+
+double nso_mlp_ratio(const double x[8]) {
+    double bmin = x[5], bmax = x[5];
+    for (int k = 6; k < 8; k++) {
+        bmin = x[k] < bmin ? x[k] : bmin;
+        bmax = x[k] > bmax ? x[k] : bmax;
+    }
+    return (bmax + NSO_MLP_EPS) / (bmin + NSO_MLP_EPS);
+}
+
+/* share of the network: 0 at SMOOTH rising to 1 at 2 SMOOTH, 1 at FLAG / 4 falling to 0 at FLAG */
+double nso_mlp_blend(double ratio) {
+    double up   = (ratio - NSO_MLP_SMOOTH) / NSO_MLP_SMOOTH;
+    double down = (NSO_MLP_FLAG - ratio) / (0.75 * NSO_MLP_FLAG);
+    up          = up < 1.0 ? up : 1.0;
+    down        = down < 1.0 ? down : 1.0;
+    return (up > 0.0 ? up : 0.0) * (down > 0.0 ? down : 0.0);
 }
 
 int nso_mlp_regime(const double x[8]) {
@@ -207,13 +227,16 @@ double nso_weno5_learned_p(const double* params, int bf16, const double f[5], in
         else nso_weno5_left(f, NSO_W_SHU, &out);
         return out;
     }
-    double w[3];
+    double w[3], classic;
     nso_mlp_forward(params, x, bf16, w);
+    if (side) nso_weno5_right(f, NSO_W_SHU, &classic);
+    else nso_weno5_left(f, NSO_W_SHU, &classic);
     /* candidate polynomials of the (mirrored) left-biased stencil in the caller's units */
-    double c0 = (2. * v[0] - 7. * v[1] + 11. * v[2]) / 6.;
-    double c1 = (-v[1] + 5. * v[2] + 2. * v[3]) / 6.;
-    double c2 = (2. * v[2] + 5. * v[3] - v[4]) / 6.;
-    return w[0] * c0 + w[1] * c1 + w[2] * c2;
+    double c0  = (2. * v[0] - 7. * v[1] + 11. * v[2]) / 6.;
+    double c1  = (-v[1] + 5. * v[2] + 2. * v[3]) / 6.;
+    double c2  = (2. * v[2] + 5. * v[3] - v[4]) / 6.;
+    double net = w[0] * c0 + w[1] * c1 + w[2] * c2;
+    return classic + nso_mlp_blend(nso_mlp_ratio(x)) * (net - classic);
 }
 
 void nso_weno5_learned(const double* params, int bf16, const double f[5], int side, double out[1],

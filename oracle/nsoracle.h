@@ -78,7 +78,8 @@ double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double 
  * An 8-32-32-3 MLP (GELU, softmax) maps the range-normalised stencil and its three smoothness
  * indicators to the three WENO weights.  Faces whose indicator ratio
  * (max beta + EPS) / (min beta + EPS) is below SMOOTH (smooth) or above FLAG (flagged
- * discontinuity) keep the classic Shu weights.  Parameter layout (row-major):
+ * discontinuity) keep the classic Shu weights; in between the network's reconstruction enters
+ * with the share nso_mlp_blend(ratio).  Parameter layout (row-major):
  * W1[32][8], b1[32], W2[32][32], b2[32], W3[3][32], b3[3]. */
 #define NSO_MLP_HIDDEN 32
 #define NSO_MLP_NPARAM (32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3)
@@ -91,6 +92,10 @@ double nso_bf16(double v);
 /* x[0..4] = (v - min) / (max - min) (0 when max == min), x[5..7] = beta_0..2 of x */
 void   nso_mlp_features(const double v[5], double x[8]);
 int    nso_mlp_regime(const double x[8]);
+/* (max beta + EPS) / (min beta + EPS) and the network's share of the reconstruction at it:
+ * min(1, (r - SMOOTH) / SMOOTH) * min(1, (FLAG - r) / (0.75 FLAG)), clipped at 0 */
+double nso_mlp_ratio(const double x[8]);
+double nso_mlp_blend(double ratio);
 /* the three left-biased ENO candidates of the normalised stencil x[0..4] */
 void   nso_mlp_candidates(const double x[8], double q[3]);
 /* softmax weights; bf16 != 0 rounds the operands of the matrix products like the GPU kernels */

@@ -117,6 +117,9 @@ class Oracle(_Lib):
         L.nso_bf16.restype = ctypes.c_double
         L.nso_bf16.argtypes = [ctypes.c_double]
         L.nso_mlp_regime.restype = ctypes.c_int
+        L.nso_mlp_ratio.restype = ctypes.c_double
+        L.nso_mlp_blend.restype = ctypes.c_double
+        L.nso_mlp_blend.argtypes = [ctypes.c_double]
         L.nso_mlp_loss_grad.restype = ctypes.c_double
         L.nso_mlp_loss_grad.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_long] + \
             [ctypes.POINTER(ctypes.c_double)] * 3 + [ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -181,6 +184,12 @@ class Oracle(_Lib):
 
     def mlp_regime(self, x):
         return self.lib.nso_mlp_regime(_ptr(np.ascontiguousarray(x, dtype=np.float64)))
+
+    def mlp_ratio(self, x):
+        return self.lib.nso_mlp_ratio(_ptr(np.ascontiguousarray(x, dtype=np.float64)))
+
+    def mlp_blend(self, ratio):
+        return self.lib.nso_mlp_blend(float(ratio))
 
     def mlp_candidates(self, x):
         q = np.zeros(3)

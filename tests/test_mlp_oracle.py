@@ -155,23 +155,41 @@ def test_right_biased_is_mirrored_left(oracle, data):
         a, ra = oracle.weno_learned(params, v, 1)
         b, rb = oracle.weno_learned(params, v[::-1].copy(), 0)
         assert ra == rb
-        if ra == po.REGIMES["mlp"]:  # (the classic fallbacks differ in operation order)
-            assert a == b
+        if ra == po.REGIMES["mlp"]:  # (the classic parts differ in operation order)
+            assert a == pytest.approx(b, rel=1e-13, abs=1e-13 * np.abs(v).max())
             n += 1
     assert n > 20
+
+
+def test_blend_is_continuous_between_regimes(oracle):
+    b = oracle.mlp_blend
+    assert b(2.0) == 0.0 and b(3.0) == 0.0 and b(4.5) == 0.5 and b(6.0) == 1.0
+    assert b(250.0) == 1.0 and b(625.0) == 0.5 and b(1000.0) == 0.0 and b(5000.0) == 0.0
+    r = np.concatenate([np.linspace(2.9, 7, 2000), np.linspace(240, 1010, 2000)])
+    v = np.array([b(x) for x in r])
+    assert np.abs(np.diff(v)).max() < 2e-3  # no jump
 
 
 def test_ideal_weight_network_gives_linear_upwind_scheme(oracle, data):
-    # W = 0 and b3 = log(d) make the softmax output the ideal weights (0.1, 0.6, 0.3): on MLP
-    # faces the reconstruction is then the linear fifth-order upwind formula
+    # W = 0 and b3 = log(d) make the softmax output the ideal weights (0.1, 0.6, 0.3): where the
+    # network has the whole share the reconstruction is the linear fifth-order upwind formula,
+    # elsewhere in its regime the blend of that formula and the classic value
     params = np.zeros(NP)
     params[1440:1443] = np.log([0.1, 0.6, 0.3])
-    n = 0
+    full = part = 0
     for v in data[0]:
         val, rg = oracle.weno_learned(params, v, 0, bf16=False)
-        if rg == po.REGIMES["mlp"]:
-            a, b, c, d, e = v
-            ref = (2 * a - 13 * b + 47 * c + 27 * d - 3 * e) / 60.0
-            assert val == pytest.approx(ref, rel=1e-13, abs=1e-13 * np.abs(v).max())
-            n += 1
-    assert n > 20
+        if rg != po.REGIMES["mlp"]:
+            continue
+        a, b, c, d, e = v
+        lin = (2 * a - 13 * b + 47 * c + 27 * d - 3 * e) / 60.0
+        lam = oracle.mlp_blend(oracle.mlp_ratio(oracle.mlp_features(v)))
+        tol = 1e-13 * np.abs(v).max()
+        if lam == 1.0:
+            assert val == pytest.approx(lin, rel=1e-13, abs=tol)
+            full += 1
+        else:
+            cl = oracle.weno_left(v)
+            assert val == pytest.approx(cl + lam * (lin - cl), rel=1e-13, abs=tol)
+            part += 1
+    assert full > 20 and part > 5
