@@ -87,13 +87,20 @@ def tgv_W(case, M=0.1, seed=2, c0=None):
     return W
 
 
-def shear_W(case, M=0.3, amp=0.05, seed=1, c0=None):
+def shear_W(case, M=0.3, amp=0.05, seed=1, c0=None, smooth_phase=False):
     """3-D periodic double shear layer on [0,1]^3: u = U0 tanh((|y-0.5|-0.25)/0.02),
-    v = amp * U0 * sin(2 pi x + phi_k) with a seeded phase per z plane."""
+    v = amp * U0 * sin(2 pi x + phi(z)) with a seeded phase: drawn independently per z plane,
+    or with smooth_phase a seeded smooth periodic function of z (three Fourier modes), which
+    describes the same field at every resolution (for comparisons across grids)."""
     c0 = sound_speed(M) if c0 is None else c0
     V = M * c0
     rng = np.random.default_rng(seed)
-    phase = rng.random(case["nz"]) * 2.0 * np.pi
+    if smooth_phase:
+        z = (np.arange(case["nz"]) + 0.5) / case["nz"]
+        a, b = rng.random(3), rng.random(3) * 2.0 * np.pi
+        phase = sum(np.pi * a[m] / (m + 1) * np.sin(2 * np.pi * (m + 1) * z + b[m]) for m in range(3))
+    else:
+        phase = rng.random(case["nz"]) * 2.0 * np.pi
     X, Y, Z = _centres(case)
     Ly = case["L"][1]
     Lx = case["L"][0]
@@ -226,10 +233,13 @@ def jump_W(case, seed=4, rho_ratio=8.0, p_ratio=10.0, width=0.02):
 
 def config4(n=256, problem="shear"):
     """Learned form evaluation problems on [0,1]^3: the double shear layer of config1 at n^3
-    (component-wise WENO-5 + CD4, Mach 0.3, Re 1000) or the Sod-like planar jump (density 8:1,
-    pressure 10:1, seeded orientation) with the same scheme."""
+    (component-wise WENO-5 + CD4, Mach 0.3, Re 1000; the perturbation phase a smooth seeded
+    function of z so that grids of different resolution hold the same field) or the Sod-like
+    planar jump (density 8:1, pressure 10:1, seeded orientation) with the same scheme."""
     case, W = config1(n)
     if problem == "jump":
         case = dict(case, mu=1e-4)
         W = jump_W(case)
+    else:
+        W = shear_W(case, M=0.3, amp=0.05, seed=1, smooth_phase=True)
     return case, W

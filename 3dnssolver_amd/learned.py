@@ -107,10 +107,12 @@ def coarse_stencils(f, axis):
 
 def line_samples(f, axis, rng, keep=1.0):
     """Samples of every point of the periodic field f along `axis` (both orientations are
-    handled by the caller).  Returns (X, Q, T) of the network-regime stencils whose target is
-    reliable: the stencil's range is at least 1e-4 of the field's magnitude (not round-off or
-    seeded noise) and the fourth-derivative correction is at most a quarter of the
-    second-derivative one (the expansion of h holds; under-resolved fronts are left out)."""
+    handled by the caller): the network-regime stencils whose range is at least 1e-4 of the
+    field's magnitude (not round-off or seeded noise).  Where the fourth-derivative correction
+    exceeds a quarter of the second-derivative one the expansion of h does not hold (an
+    under-resolved front): there the target is the classic reconstruction, so the network
+    learns to keep the classic behaviour where the fine solution cannot say better.  Returns
+    (X, Q, T, S, Cn)."""
     V = coarse_stencils(f, axis)
     t, c2, c4 = (a.ravel() for a in face_target(f, axis, parts=True))
     if keep < 1.0:
@@ -119,9 +121,12 @@ def line_samples(f, axis, rng, keep=1.0):
     X, Q, rg, m, r = features(V)
     T = (t - m) / r
     mag = np.abs(f).max()
-    ok = (rg == 1) & (r >= 1e-4 * mag) & (c4 <= 0.25 * c2 + 1e-12 * mag) & (np.abs(T) < 3.0)
-    X, Q, T, r = X[ok], Q[ok], T[ok], r[ok]
-    return X, Q, T, r / mag, classic_normalised(X, Q, r)
+    ok = (rg == 1) & (r >= 1e-4 * mag)
+    reliable = (c4 <= 0.25 * c2 + 1e-12 * mag) & (np.abs(T) < 3.0)
+    X, Q, T, r, reliable = X[ok], Q[ok], T[ok], r[ok], reliable[ok]
+    Cn = classic_normalised(X, Q, r)
+    T = np.where(reliable, T, Cn)
+    return X, Q, T, r / mag, Cn
 
 
 def split_fluxes(Wi, gamma):
@@ -155,7 +160,8 @@ def snapshot_samples(Wi, gamma, rng, keep):
 def fine_runs(n_coarse=64):
     """Yields interior primitive states of 2x-resolution FP64 plain-WENO runs (GPU, EXACT math):
     double shear layers in the style of the 64^3 shear configuration (component-wise WENO-5,
-    three seeded variants of Mach number and perturbation) and the Taylor-Green vortex
+    three seeded variants of Mach number and perturbation, smooth perturbation phase) and the
+    Taylor-Green vortex
     (characteristic WENO-5), each at two times, with their gamma."""
     ns = importlib.import_module("3dnssolver_amd")
     C = importlib.import_module("3dnssolver_amd.cases")
@@ -165,7 +171,8 @@ def fine_runs(n_coarse=64):
         case, _ = C.config1(n)
         c0 = C.sound_speed(M)
         case = dict(case, mu=M * c0 / 1000.0)
-        runs.append((case, C.shear_W(case, M=M, amp=amp, seed=seed), (60, 120)))
+        runs.append((case, C.shear_W(case, M=M, amp=amp, seed=seed, smooth_phase=True),
+                     (60, 120)))
     case, W = C.config2(n)
     runs.append((case, W, (200, 400)))
     for case, W, steps in runs:
@@ -263,7 +270,7 @@ def evaluate(params, n=64, steps=100, problems=("shear", "jump"), maths=("exact"
     return out
 
 
-def train(X, Q, T, steps=300, batch=65536, lr=1e-3, seed=0, params=None):
+def train(X, Q, T, steps=3000, batch=65536, lr=3e-3, seed=0, params=None):
     """GPU training (Adam, the project's BF16-MFMA forward and backward kernels) from
     init_params(seed) or `params` on the arrays of training_set."""
     ns = importlib.import_module("3dnssolver_amd")
@@ -290,9 +297,9 @@ if __name__ == "__main__":
 
     ap = argparse.ArgumentParser(description="train the learned WENO-5 weights on the GPU")
     ap.add_argument("--samples", type=int, default=20_000_000)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--n-coarse", type=int, default=64)
     ap.add_argument("--out", default=WEIGHTS_PATH)
     a = ap.parse_args()

@@ -125,9 +125,14 @@ def ensemble_workload(args):
     cells = sum(c["nx"] * c["ny"] * c["nz"] for c, _, _ in members)
     print(f"[bench] ensemble of {len(members)} blocks, {cells / 1e6:.2f} Mcells, math={args.math}",
           file=sys.stderr, flush=True)
+    params = None
+    if args.weights == "learned":
+        params = importlib.import_module("3dnssolver_amd.learned").load()
     solvers = []
     for case, W, _ in members:
-        s = ns.Solver(case, math=args.math)
+        s = ns.Solver(dict(case, weights=args.weights), math=args.math)
+        if params is not None:
+            s.set_mlp(params)
         s.set_state(W, "primitive")
         solvers.append(s)
     ens = ns.Ensemble(solvers)
@@ -151,7 +156,9 @@ def ensemble_workload(args):
     # copies of the initial states)
     t_solo = 0.0
     for case, W, _ in members:
-        s = ns.Solver(case, math=args.math)
+        s = ns.Solver(dict(case, weights=args.weights), math=args.math)
+        if params is not None:
+            s.set_mlp(params)
         s.set_state(W, "primitive")
         s.step(1)
         t0 = time.perf_counter()
@@ -186,7 +193,7 @@ def ensemble_workload(args):
         "config": {"workload": f"{len(members)} periodic double shear layers on [0,1]^3, edges from "
                                f"{{32,48,64,96,128}} per axis, Mach U[0.2,0.6], Re U[500,5000], "
                                f"amplitude U[0.01,0.1], WENO-5 (Shu, LF) + CD4, SSP-RK3 CFL 0.5",
-                   "cells": cells, "math": args.math, "graph": True},
+                   "cells": cells, "math": args.math, "weights": args.weights, "graph": True},
         "autotuner": {"ms_default_tiles": ms_default, "ms_tuned": ms_tuned,
                       "tiles_kw_flux_kw_cell_march": tiles.tolist()},
         "ms_per_step_without_graph": round(ms_nograph, 4),
@@ -198,11 +205,74 @@ def ensemble_workload(args):
     }))
 
 
+def learned_workload(args):
+    """Learned form: the double shear layer at n^3 (component-wise WENO-5 + CD4).  Times SSP-RK3
+    steps of the plain FP64 method (Shu weights, EXACT math), of FP32 reconstruction with the Shu
+    weights and with the learned weights, and of FP64 with the learned weights; then, for the
+    shear layer and the Sod-like planar jump, the relative L2 errors of density and energy of
+    the plain FP64 run and of the FP32 + learned run against a 2n^3 FP64 run at the same time
+    (prescribed dt)."""
+    import numpy as np
+    ns = importlib.import_module("3dnssolver_amd")
+    C = importlib.import_module("3dnssolver_amd.cases")
+    lr = importlib.import_module("3dnssolver_amd.learned")
+    params = lr.load()
+    n = args.n
+    case, W = C.config4(n, "shear")
+    rates = {}
+    for math, weights in (("exact", "shu"), ("fp32", "shu"), ("fp32", "learned"),
+                          ("exact", "learned")):
+        s = ns.Solver(dict(case, weights=weights), math=math)
+        if weights == "learned":
+            s.set_mlp(params)
+        s.set_state(W, "primitive")
+        s.step(args.warmup)
+        print(f"[bench] {math}/{weights}: {args.steps} steps at {n}^3", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        s.step(args.steps)
+        el = time.perf_counter() - t0
+        rates[f"{math}_{weights}"] = {"value": round(n ** 3 * args.steps / el / 1e9, 6),
+                                      "ms_per_step": round(el / args.steps * 1e3, 4),
+                                      "health": s.reduce()[3]}
+        s.close()
+    errors = {}
+    ne = args.eval_n
+    for prob in ("shear", "jump"):
+        print(f"[bench] accuracy: {prob} {ne}^3 vs {2 * ne}^3 FP64", file=sys.stderr, flush=True)
+        c, w = C.config4(ne, prob)
+        fc, fw = C.config4(2 * ne, prob)
+        dt, t_end, ref = lr.reference(c, w, fc, fw, args.eval_steps)
+        e = {"t_end": t_end, "steps": args.eval_steps, "n": ne}
+        for math, weights in (("exact", "shu"), ("fp32", "shu"), ("fp32", "learned")):
+            e[f"{math}_{weights}"] = lr.coarse_error(c, w, dt, args.eval_steps, ref, math,
+                                                     weights, params)
+        errors[prob] = e
+    base = rates["exact_shu"]["value"]
+    print(json.dumps({
+        "metric": "learned form: FP32 + learned weights vs plain FP64, Gcell-RK3-steps/s and L2 "
+                  "vs a 2x-resolution FP64 run",
+        "value": rates["fp32_learned"]["value"], "unit": "Gcell-RK3-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+        "dtype": "f32+bf16 network+f64", "data": "synthetic",
+        "config": {"workload": f"double shear layer {n}^3 on [0,1]^3, Mach 0.3, Re 1000, "
+                               f"WENO-5 component-wise (LF) + CD4, SSP-RK3 CFL 0.5; accuracy on "
+                               f"the shear layer and a Sod-like planar jump (8:1, 10:1) at "
+                               f"{ne}^3 vs {2 * ne}^3", "grid": [n, n, n]},
+        "speedup_vs_fp64_plain": round(rates["fp32_learned"]["value"] / base, 4),
+        "rates": rates, "l2_vs_2x_fp64": errors,
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["tgv", "precision", "ensemble"], default="tgv",
+    ap.add_argument("--workload", choices=["tgv", "precision", "ensemble", "learned"],
+                    default="tgv",
                     help="tgv: the headline run; precision: FP32-reconstruction form; "
-                         "ensemble: batched form")
+                         "ensemble: batched form; learned: learned-weights form")
+    ap.add_argument("--weights", choices=["shu", "learned"], default="shu",
+                    help="WENO weights of the ensemble members")
+    ap.add_argument("--eval-n", type=int, default=64)
+    ap.add_argument("--eval-steps", type=int, default=100)
     ap.add_argument("--blocks", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +300,9 @@ def main():
         return
     if args.workload == "ensemble":
         ensemble_workload(args)
+        return
+    if args.workload == "learned":
+        learned_workload(args)
         return
 
     ns = importlib.import_module("3dnssolver_amd")
