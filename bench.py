@@ -337,11 +337,18 @@ def main():
     flop_launch = FLOP_PER_FACE[args.conv] * faces
     achieved = flop_launch / (dom_ms / dom_n * 1e-3) / 1e12
     traffic = None
+    tr = {}
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         tr = json.load(open(tpath))
         traffic = tr.get(f"{dom}_{args.math}_{n}")
     step_frac = value * 1e9 * FLOP_PER_CELL_STEP[args.conv] / (FP64_PEAK_TFLOPS * 1e12)
+    # the residual + stage-update kernel is HBM-bound: 37 planes read (fx, fy, fz, Un, Ucur:
+    # 5 each; W, Dx, Dy: 4 each) and 10 written (Uout, Wout) per launch, each once
+    res_ms, res_n = timers["residual_update"]
+    res_bytes = 47 * 8.0 * cells
+    res_traffic = tr.get(f"residual_update_{args.math}_{n}") if os.path.exists(tpath) else None
+    res_achieved = res_bytes / (res_ms / res_n * 1e-3) / 1e9
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -374,6 +381,12 @@ def main():
             "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
             "flop_per_launch": flop_launch, "avg_launch_ms": round(dom_ms / dom_n, 5),
             "step_frac": round(step_frac, 5),
+        },
+        "roofline_residual": {
+            "bound": "hbm", "kernel": "residual_update", "achieved": round(res_achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(res_achieved / HBM_PEAK_GBS, 4),
+            "traffic": res_traffic, "bytes_per_launch": res_bytes,
+            "avg_launch_ms": round(res_ms / res_n, 5),
         },
         "kernel_ms_per_step": {k: round(v[0] / args.timing_steps, 4) for k, v in timers.items()},
         "health": {"status": rc, "flags": flags, "dt_last": dt},
