@@ -243,3 +243,65 @@ def config4(n=256, problem="shear"):
     else:
         W = shear_W(case, M=0.3, amp=0.05, seed=1, smooth_phase=True)
     return case, W
+
+
+# ---- the reference's initial conditions (testcases.cpp:5-180) ---------------------------------
+# states [rho, p, u, v, w] of the quadrants NW (x <= Lx/2, y <= Ly/2), NE, SW, SE in the x-y
+# plane (the z-direction Sod uses the z-y plane).  North is the j-low side, as in the
+# reference's boundary naming.  Configuration 17 of the reference lists only four values for SW
+# and reads a fifth past the end of its vector (testcases.cpp:113); w = 0 stands in here.
+_SOD = ((1.0, 1.0, 0.0, 0.0, 0.0), (0.125, 0.1, 0.0, 0.0, 0.0))
+RIEMANN = {
+    "sod_x": (_SOD[0], _SOD[1], _SOD[0], _SOD[1], "xy"),
+    "sod_y": (_SOD[0], _SOD[0], _SOD[1], _SOD[1], "xy"),
+    "sod_z": (_SOD[0], _SOD[1], _SOD[0], _SOD[1], "zy"),
+    "case3": ((0.5323, 0.3, 1.206, 0.0, 0.0), (1.5, 1.5, 0.0, 0.0, 0.0),
+              (0.138, 0.029, 1.206, -1.206, 0.0), (0.5323, 0.3, 0.0, -1.206, 0.0), "xy"),
+    "case4": ((0.5065, 0.35, 0.8939, 0.0, 0.0), (1.1, 1.1, 0.0, 0.0, 0.0),
+              (1.1, 1.1, 0.8939, -0.8938, 0.0), (0.5065, 0.35, 0.0, -0.8939, 0.0), "xy"),
+    "case6": ((2.0, 1.0, 0.75, -0.5, 0.0), (1.0, 1.0, 0.75, 0.5, 0.0),
+              (1.0, 1.0, -0.75, -0.5, 0.0), (3.0, 1.0, -0.75, 0.5, 0.0), "xy"),
+    "case12": ((1.0, 1.0, 0.7276, 0.0, 0.0), (0.5313, 0.4, 0.0, 0.0, 0.0),
+               (0.8, 1.0, 0.0, 0.0, 0.0), (1.0, 1.0, 0.0, -0.7276, 0.0), "xy"),
+    "case15": ((0.5197, 0.4, -0.6259, 0.3, 0.0), (1.0, 1.0, 0.1, 0.3, 0.0),
+               (0.8, 0.4, 0.1, 0.3, 0.0), (0.5313, 0.4, 0.1, -0.4276, 0.0), "xy"),
+    "case17": ((2.0, 1.0, 0.0, 0.3, 0.0), (1.0, 1.0, 0.0, 0.4, 0.0),
+               (1.0625, 0.4, 0.0, -0.2145, 0.0), (0.5197, 0.4, 0.0, 1.1259, 0.0), "xy"),
+}
+RIEMANN_IDS = {name: i for i, name in enumerate(RIEMANN)}  # the reference harness's numbering
+
+
+def reference_coords(case):
+    """Cell centres as Grid computes them (grid.cpp:40-56): x = dx * 0.5 + i * dx, block origin 0."""
+    out = []
+    for n, L in zip((case["nx"], case["ny"], case["nz"]), case["L"]):
+        d = L / n
+        out.append(d * 0.5 + np.arange(n) * d)
+    return np.meshgrid(*out, indexing="ij")
+
+
+def riemann_W(case, name):
+    """The reference's quadrant initial condition `name` (a key of RIEMANN) on the block."""
+    NW, NE, SW, SE, plane = RIEMANN[name]
+    X, Y, Z = reference_coords(case)
+    Lx, Ly, Lz = case["L"]
+    a, La = (X, Lx) if plane == "xy" else (Z, Lz)
+    lo_a, lo_y = a <= La / 2.0, Y <= Ly / 2.0
+    W = _empty(case)
+    I = (slice(NGC, -NGC),) * 3
+    for q in range(5):
+        W[I + (q,)] = np.where(lo_a & lo_y, NW[q], np.where(~lo_a & lo_y, NE[q],
+                                                         np.where(lo_a & ~lo_y, SW[q], SE[q])))
+    return W
+
+
+def riemann_case(name, n=64, nz=4):
+    """A walled block for the reference's Riemann problems: [0,1]^2 x [0, nz/n], Neumann faces in
+    x and y, periodic in z (the 2-D configurations are uniform in z), component-wise WENO-5,
+    inviscid, CFL 0.5."""
+    case = dict(base_case(n, n, nz), L=(1.0, 1.0, nz / n), conv="weno5", diff_order=0, mu=0.0,
+                bc=["neumann", "neumann", "neumann", "neumann", "periodic", "periodic"])
+    if name == "sod_z":
+        case = dict(base_case(nz, n, n), L=(nz / n, 1.0, 1.0), conv="weno5", diff_order=0,
+                    mu=0.0, bc=["periodic", "periodic", "neumann", "neumann", "neumann", "neumann"])
+    return case, riemann_W(case, name)

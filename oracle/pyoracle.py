@@ -244,6 +244,15 @@ class Reference(_Lib):
         if not os.path.exists(path):
             raise FileNotFoundError(path)
         super().__init__(path, "ref_")
+        self.lib.ref_testcase.restype = ctypes.c_int
+
+    def testcase(self, case, ident):
+        """The reference's initial condition number `ident` (testcases.cpp) on the block."""
+        cfg = make_cfg(case)
+        W = np.zeros(padded_shape(case))
+        if self.lib.ref_testcase(ctypes.byref(cfg), ctypes.c_int(ident), _ptr(W)) != 0:
+            raise ValueError("unknown test case")
+        return W
 
 
 def reference_available(flavour="strict"):

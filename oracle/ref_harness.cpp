@@ -24,6 +24,7 @@
 #include "domain.hpp"
 #include "grid.hpp"
 #include "params.hpp"
+#include "testcases.hpp"
 // eq_of_state.hpp comes in through time_scheme.hpp only: its include guard is mismatched
 // (eq_of_state.hpp:1-2), so a second inclusion would redefine EqOfState.
 #include "time_scheme.hpp"
@@ -141,6 +142,26 @@ int ref_step(const nso_cfg* c, double* W_io, double* U_out, int nsteps, double* 
     std::memcpy(U_out, &U(0, 0, 0, 0), len(U) * sizeof(double));
     delete conv;
     delete diff;
+    return 0;
+}
+
+// The reference's initial conditions (testcases.cpp:5-180) on the block of `c`: id 0..2 Sod along
+// x, y, z; 3..8 the 2-D Riemann configurations 3, 4, 6, 12, 15, 17.  W_out gets the whole padded
+// primitive array (halos as a fresh Domain leaves them: zero).
+int ref_testcase(const nso_cfg* c, int id, double* W_out) {
+    ensure_mpi();
+    Params        p = make_params(c);
+    Decomposition dec(p);
+    Grid          grid(p, dec);
+    Domain        domain(p, dec);
+    typedef void (*ic_fn)(Domain&, const Grid&, const Params&);
+    static const ic_fn fns[] = {testcases::sod_xdir, testcases::sod_ydir, testcases::sod_zdir,
+                                testcases::case3,    testcases::case4,    testcases::case6,
+                                testcases::case12,   testcases::case15,   testcases::case17};
+    if (id < 0 || id >= (int)(sizeof(fns) / sizeof(fns[0]))) return -1;
+    fns[id](domain, grid, p);
+    Array4d& W = domain.get_NS_primitive_array();
+    std::memcpy(W_out, &W(0, 0, 0, 0), len(W) * sizeof(double));
     return 0;
 }
 
