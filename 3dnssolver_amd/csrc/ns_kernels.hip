@@ -771,6 +771,9 @@ __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, 
                                           const Bnd& b, int bx, int by, int bz) {
     int face = bz;
     int axis = face >> 1;
+    // a 2-D block thinner than the halo (nz < 3): its z halos are never read (no z sweep, 2-D
+    // viscous terms) and their periodic copy would read halo cells being written: left as is
+    if (axis == 2 && g.nz < NGC) return;
     // face order W,E,N,S,F,B: low side for W (0), N (2), B (5); high side for E, S, F
     bool low = (face == 0 || face == 2 || face == 5);
     int  n   = (axis == 0) ? g.nx : (axis == 1) ? g.ny : g.nz;

@@ -246,6 +246,14 @@ class Reference(_Lib):
         super().__init__(path, "ref_")
         self.lib.ref_testcase.restype = ctypes.c_int
 
+    def write_binary(self, case, W, it, directory):
+        """The reference's BinaryWriter output (mesh and fields files) of W into directory/results."""
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        if self.lib.ref_write_binary(ctypes.byref(cfg), _ptr(W), ctypes.c_int(it),
+                                     str(directory).encode()) != 0:
+            raise OSError("reference writer failed")
+
     def testcase(self, case, ident):
         """The reference's initial condition number `ident` (testcases.cpp) on the block."""
         cfg = make_cfg(case)

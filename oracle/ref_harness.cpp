@@ -11,6 +11,8 @@
 // create_diffusion_operator (driver.cpp:74-105) would create.  MPI is the image's MPICH, run as a
 // single self-initialised rank.
 #include <cstring>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <mpi.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -24,6 +26,7 @@
 #include "domain.hpp"
 #include "grid.hpp"
 #include "params.hpp"
+#include "io/io_binary.hpp"
 #include "testcases.hpp"
 // eq_of_state.hpp comes in through time_scheme.hpp only: its include guard is mismatched
 // (eq_of_state.hpp:1-2), so a second inclusion would redefine EqOfState.
@@ -163,6 +166,25 @@ int ref_testcase(const nso_cfg* c, int id, double* W_out) {
     Array4d& W = domain.get_NS_primitive_array();
     std::memcpy(W_out, &W(0, 0, 0, 0), len(W) * sizeof(double));
     return 0;
+}
+
+// BinaryWriter::write_mesh and write_fields (io/io_binary.cpp:3-75) of the primitive array W_in
+// for iteration `iter`, into <dir>/results/ (the writer's relative path); returns 0 on success.
+int ref_write_binary(const nso_cfg* c, const double* W_in, int iter, const char* dir) {
+    ensure_mpi();
+    Params        p = make_params(c);
+    Decomposition dec(p);
+    Grid          grid(p, dec);
+    Domain        domain(p, dec);
+    Array4d&      W = domain.get_NS_primitive_array();
+    std::memcpy(&W(0, 0, 0, 0), W_in, len(W) * sizeof(double));
+    char cwd[4096];
+    if (!getcwd(cwd, sizeof cwd) || chdir(dir) != 0) return -1;
+    mkdir("results", 0755);
+    BinaryWriter writer;
+    writer.write_mesh(grid);
+    writer.write_fields(domain, (size_t)iter);
+    return chdir(cwd) == 0 ? 0 : -1;
 }
 
 int ref_num_threads() {
