@@ -123,17 +123,35 @@ def config1(n=64):
     return case, shear_W(case, M=M, amp=0.05, seed=1)
 
 
-def config2(n=256):
+def sutherland(case, mu_ref, T_ref, C_rel=110.4 / 300.0):
+    """Switches a case to the Sutherland law with mu = mu_ref at the temperature T_ref (the
+    library's temperature is T = gamma p / rho, the square of the sound speed) and a Sutherland
+    constant C_rel * T_ref (air: 110.4 K over a 300 K reference).  The library evaluates
+    mu(T) = mu (1 + C) / (T + C) * T^1.5, so mu and C are rescaled to put the reference point at
+    T_ref."""
+    C = C_rel * T_ref
+    case.update(viscosity="sutherland", suth_C=C,
+                mu=mu_ref * (T_ref + C) / ((1.0 + C) * T_ref * math.sqrt(T_ref)))
+    return case
+
+
+def config2(n=256, viscosity="constant"):
     """Plain method, one GPU subdomain: Taylor-Green vortex, 256^3, [0,2pi]^3, characteristic
-    WENO-5 (Shu) + CD4 viscous terms, Mach 0.1, Re = 1600, CFL 0.5; 200 steps."""
+    WENO-5 (Shu) + CD4 viscous terms, Mach 0.1, Re = 1600, CFL 0.5; 200 steps.  viscosity
+    "constant" is the reference's path (mu fixed); "sutherland" gives mu = rho0 U0 L / Re at the
+    initial temperature c0^2 and lets it follow the local temperature."""
     M, Re = 0.1, 1600.0
     c0 = sound_speed(M)
     case = dict(base_case(n, n, n), L=(2 * math.pi,) * 3, conv="weno5cw", weights="shu",
                 diff_order=4, mu=1.0 * M * c0 * 1.0 / Re)
+    if viscosity == "sutherland":
+        sutherland(case, case["mu"], c0 * c0)
+    elif viscosity != "constant":
+        raise ValueError(viscosity)
     return case, tgv_W(case, M=M, seed=2)
 
 
-def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None):
+def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None, speed_frac=1.0):
     """Decaying isotropic turbulence on [0, 2pi]^3: seeded random-phase solenoidal velocity with
     energy spectrum E(k) ~ k^4 exp(-2 (k/k0)^2), uniform density and pressure, rms velocity
     Mt * c0.  Returns (W, u_prime, taylor_lambda) with u' the rms of one component and
@@ -159,9 +177,9 @@ def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None):
     vel = [np.fft.irfftn(cc, s=(n, n, n)) for cc in comps]
     urms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
     if c0 is None:
-        # c0 with max |u_i| + c0 = alpha = 3 (the LF split constant)
+        # c0 with max |u_i| + c0 = speed_frac * alpha (alpha = 3, the LF split constant)
         vmax = max(np.abs(v).max() for v in vel) / urms
-        c0 = ALPHA / (1.0 + Mt * vmax)
+        c0 = speed_frac * ALPHA / (1.0 + Mt * vmax)
     scale = Mt * c0 / urms
     vel = [v * scale for v in vel]
     dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(vel[0]), s=(n, n, n))
@@ -176,12 +194,19 @@ def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None):
     return W, uprime, lam
 
 
-def config3(n=512):
+def config3(n=512, speed_frac=0.85, cfl=0.3):
     """Precision form: decaying isotropic turbulence, n^3 periodic on [0, 2pi]^3, turbulent Mach
-    0.5, Re_lambda = 100, characteristic WENO-5 (Shu) + CD4, CFL 0.5; mu = rho u' lambda / 100."""
+    0.5, Re_lambda = 100, characteristic WENO-5 (Shu) + CD4; mu = rho u' lambda / 100.
+
+    The time step is the reference's (CFL x the minimum over cells and axes of dx / max|u +- c|,
+    time_scheme.cpp:98-136) while the flux split uses the fixed alpha = 3 on every axis, so the
+    linear stability of WENO-5 + SSP-RK3 needs 3 * CFL * alpha / max|u +- c| below about 1.4.
+    With max|u_i| + c0 = 0.85 alpha at the start (headroom for the shocklets' rise in |u| + c),
+    CFL 0.3 keeps that number near 1.1; at CFL 0.5 this flow blows up within 150 steps in FP64
+    with either WENO form, with or without viscosity (measured at 256^3 and 512^3)."""
     case = dict(base_case(n, n, n), L=(2 * math.pi,) * 3, conv="weno5cw", weights="shu",
-                diff_order=4)
-    W, uprime, lam = isotropic_W(case, Mt=0.5, k0=4.0, seed=3)
+                diff_order=4, cfl=cfl)
+    W, uprime, lam = isotropic_W(case, Mt=0.5, k0=4.0, seed=3, speed_frac=speed_frac)
     case["mu"] = 1.0 * uprime * lam / 100.0
     return case, W
 
@@ -193,7 +218,12 @@ def config5(n_blocks=64, seed=5, edges=ENSEMBLE_EDGES):
     """Batched form: n_blocks independent periodic double shear layers on [0,1]^3, each edge
     length drawn uniformly from `edges` (independently per axis), Mach uniform in [0.2, 0.6],
     Re uniform in [500, 5000], perturbation amplitude uniform in [0.01, 0.1]; component-wise
-    WENO-5 (Shu) + CD4, CFL 0.5.  Returns a list of (case, W, params)."""
+    WENO-5 (Shu) + CD4.  Returns a list of (case, W, params).
+
+    Sound speed c0 = 0.9 alpha / (1 + M), so max|u| + c starts 10 % below the LF constant, and
+    CFL 0.3: with the reference's time step (minimum over axes, not a sum) the stability number
+    3 * CFL * alpha / max|u +- c| reaches 2 on blocks whose finest axis is not the flow axis, and
+    such blocks amplify round-off at CFL 0.5; at 0.3 every block stays below 1.2."""
     rng = np.random.default_rng(seed)
     out = []
     for b in range(n_blocks):
@@ -201,10 +231,10 @@ def config5(n_blocks=64, seed=5, edges=ENSEMBLE_EDGES):
         M = float(rng.uniform(0.2, 0.6))
         Re = float(rng.uniform(500.0, 5000.0))
         amp = float(rng.uniform(0.01, 0.1))
-        c0 = sound_speed(M)
+        c0 = 0.9 * sound_speed(M)
         case = dict(base_case(nx, ny, nz), conv="weno5", weights="shu", diff_order=4,
-                    mu=1.0 * M * c0 * 1.0 / Re)
-        W = shear_W(case, M=M, amp=amp, seed=int(rng.integers(1 << 30)))
+                    mu=1.0 * M * c0 * 1.0 / Re, cfl=0.3)
+        W = shear_W(case, M=M, amp=amp, seed=int(rng.integers(1 << 30)), c0=c0)
         out.append((case, W, dict(M=M, Re=Re, amp=amp)))
     return out
 

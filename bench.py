@@ -1,7 +1,8 @@
 """Benchmark of the RHS + SSP-RK3 path on one MI355X.
 
 Workload: the Taylor-Green vortex on a 256^3 periodic block over [0, 2pi]^3, characteristic-wise
-WENO-5 (Shu weights, Lax-Friedrichs split) + CD4 viscous terms, FP64, SSP-RK3 at CFL 0.5.
+WENO-5 (Shu weights, Lax-Friedrichs split) + CD4 viscous terms with the Sutherland law, FP64,
+SSP-RK3 at CFL 0.5 (--viscosity constant runs the reference's constant-mu path instead).
 A step is one full SSP-RK3 step (3 RHS evaluations + CFL reduction + stage updates) with the state
 resident in HBM.  Metric: Gcell-RK3-steps/s = nx*ny*nz*steps / wall time of the timed steps.
 
@@ -46,7 +47,8 @@ def cpu_baseline_child(n, steps):
     print(json.dumps({"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-RK3-steps/s",
                       "cores": lib.num_threads(), "kind": kind,
                       "sample": f"{steps} SSP-RK3 step(s) of the same workload at {n}^3 "
-                                f"({el:.1f} s wall, OpenMP)"}))
+                                f"({el:.1f} s wall, OpenMP), constant mu: the reference's code "
+                                f"path never calls its Sutherland law"}))
 
 
 def run_cpu_baseline(n, steps, timeout):
@@ -81,9 +83,14 @@ def precision_workload(args):
         s.set_state(W0, "primitive")
         s.step(args.warmup)
         print(f"[bench] {math}: {args.steps} steps", file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        s.step(args.steps)
-        el = time.perf_counter() - t0
+        el, done = 0.0, 0
+        while done < args.steps:  # chunks of 100 steps, one progress line each
+            k = min(100, args.steps - done)
+            t0 = time.perf_counter()
+            s.step(k)
+            el += time.perf_counter() - t0
+            done += k
+            print(f"[bench] {math}: {done}/{args.steps} steps", file=sys.stderr, flush=True)
         out[math] = n ** 3 * args.steps / el / 1e9
         U = s.get_state("conservative")[3:-3, 3:-3, 3:-3]
         states[math] = (U[..., 0].copy(), U[..., 1].copy())
@@ -104,7 +111,7 @@ def precision_workload(args):
         "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"decaying isotropic turbulence {n}^3, [0,2pi]^3, E(k)~k^4 exp(-2(k/4)^2), "
-                               f"Mt 0.5, Re_lambda 100, WENO-5cw + CD4, SSP-RK3 CFL 0.5",
+                               f"Mt 0.5, Re_lambda 100, WENO-5cw + CD4, SSP-RK3 CFL {case['cfl']}",
                    "grid": [n, n, n]},
         "fp64_value": round(out["exact"], 6), "speedup_vs_fp64": round(out["fp32"] / out["exact"], 4),
         "errors_vs_fp64": errs,
@@ -192,7 +199,8 @@ def ensemble_workload(args):
         "dtype": {"exact": "f64", "fast": "f64", "fp32": "f32+f64"}[args.math], "data": "synthetic",
         "config": {"workload": f"{len(members)} periodic double shear layers on [0,1]^3, edges from "
                                f"{{32,48,64,96,128}} per axis, Mach U[0.2,0.6], Re U[500,5000], "
-                               f"amplitude U[0.01,0.1], WENO-5 (Shu, LF) + CD4, SSP-RK3 CFL 0.5",
+                               f"amplitude U[0.01,0.1], c0 = 0.9*3/(1+M), WENO-5 (Shu, LF) + CD4, "
+                               f"SSP-RK3 CFL {members[0][0]['cfl']}",
                    "cells": cells, "math": args.math, "weights": args.weights, "graph": True},
         "autotuner": {"ms_default_tiles": ms_default, "ms_tuned": ms_tuned,
                       "tiles_kw_flux_kw_cell_march": tiles.tolist()},
@@ -281,6 +289,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--math", choices=["exact", "fast", "fp32"], default="exact")
     ap.add_argument("--conv", choices=["weno5cw", "weno5"], default="weno5cw")
+    ap.add_argument("--viscosity", choices=["constant", "sutherland"], default="sutherland",
+                    help="the Taylor-Green workload's Sutherland law, or constant mu (the only "
+                         "law the reference's code path calls)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -307,7 +318,7 @@ def main():
 
     ns = importlib.import_module("3dnssolver_amd")
     C = importlib.import_module("3dnssolver_amd.cases")
-    case, W0 = C.config2(args.n)
+    case, W0 = C.config2(args.n, viscosity=args.viscosity)
     case["conv"] = args.conv
     n = args.n
     print(f"[bench] {n}^3 {args.conv} cd4 math={args.math}: setup", file=sys.stderr, flush=True)
@@ -347,7 +358,8 @@ def main():
     # 5 each; W, Dx, Dy: 4 each) and 10 written (Uout, Wout) per launch, each once
     res_ms, res_n = timers["residual_update"]
     res_bytes = 47 * 8.0 * cells
-    res_traffic = tr.get(f"residual_update_{args.math}_{n}") if os.path.exists(tpath) else None
+    suffix = "" if args.viscosity == "constant" else "_" + args.viscosity
+    res_traffic = tr.get(f"residual_update_{args.math}_{n}{suffix}") if os.path.exists(tpath) else None
     res_achieved = res_bytes / (res_ms / res_n * 1e-3) / 1e9
 
     cpu = None
@@ -371,8 +383,9 @@ def main():
         "config": {
             "workload": f"Taylor-Green vortex {n}^3 periodic, [0,2pi]^3, Mach 0.1, Re 1600, "
                         f"{'characteristic' if args.conv == 'weno5cw' else 'component-wise'} "
-                        f"WENO-5 (Shu, LF split) + CD4 viscous, SSP-RK3, CFL 0.5",
-            "grid": [n, n, n], "math": args.math,
+                        f"WENO-5 (Shu, LF split) + CD4 viscous ({args.viscosity} mu), SSP-RK3, "
+                        f"CFL 0.5",
+            "grid": [n, n, n], "math": args.math, "viscosity": args.viscosity,
             "state_resident_in_hbm": True,
         },
         "roofline": {

@@ -134,11 +134,13 @@ def test_sutherland_matches_oracle_bitwise(ns, oracle, bc):
         s.close()
 
 
-def test_config1_100_steps_32cubed_bitwise(ns, oracle):
-    # config-1 physics (shear layer, component-wise WENO-5, CD4) at 32^3 for 100 steps
+@pytest.mark.parametrize("n", [32, 64])
+def test_config1_100_steps_bitwise(ns, oracle, n):
+    # the double shear layer (component-wise WENO-5, CD4, Re 1000, Mach 0.3) for 100 steps, at
+    # 32^3 and at its full 64^3: fields, dt, time and the conservation sums
     C = cases_module()
-    case, W0 = C.config1(32)
-    print("[gpu] config1 32^3 x 100 steps", flush=True)
+    case, W0 = C.config1(n)
+    print(f"[gpu] config1 {n}^3 x 100 steps", flush=True)
     Wo, _, dto = oracle.step(case, W0, 100)
     s = ns.Solver(case, math="exact")
     s.set_state(W0, "primitive")
@@ -146,6 +148,12 @@ def test_config1_100_steps_32cubed_bitwise(ns, oracle):
     assert dt == dto[-1]
     assert t == pytest.approx(dto.sum(), rel=1e-14)
     np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    Uo = oracle.p2c(case, Wo)
+    vol = np.prod(np.array(case["L"]) / np.array([case["nx"], case["ny"], case["nz"]]))
+    sums, _, flags, _ = s.reduce()
+    ref = po.interior(Uo).reshape(-1, 5).sum(axis=0) * vol
+    np.testing.assert_allclose(sums, ref, rtol=1e-12, atol=1e-13 * np.abs(po.interior(Uo)).sum() * vol)
+    assert not (flags & 3)
     s.close()
 
 

@@ -52,3 +52,20 @@ def test_sod_problem_on_oracle_moves_the_shock_right(oracle):
     assert rho[20] > 0.125 + 1e-2 and rho[12] < 1.0 - 1e-2
     # uniform in y and z (a 1-D problem on a 3-D block)
     np.testing.assert_array_equal(I[:, :, :, 0], np.broadcast_to(I[:, :1, :1, 0], I.shape[:3]))
+
+
+def test_sutherland_case_hits_reference_viscosity():
+    # the library's law mu(T) = mu (1 + C) / (T + C) T^1.5 with T = gamma p / rho: after
+    # cases.sutherland the viscosity at the reference temperature is the requested one, and the
+    # ratio mu(T) / mu(T_ref) is air's Sutherland curve in T / T_ref
+    C = cases_module()
+    case, _ = C.config2(8, viscosity="sutherland")
+    const, _ = C.config2(8)
+    T_ref = C.sound_speed(0.1) ** 2
+    law = lambda T: case["mu"] * (1 + case["suth_C"]) / (T + case["suth_C"]) * T ** 1.5
+    assert law(T_ref) == pytest.approx(const["mu"], rel=1e-14)
+    for r in (0.5, 2.0):
+        air = (300.0 + 110.4) / (300.0 * r + 110.4) * r ** 1.5
+        assert law(r * T_ref) / law(T_ref) == pytest.approx(air, rel=1e-13)
+    with pytest.raises(ValueError):
+        C.config2(8, viscosity="power")
