@@ -33,6 +33,8 @@ MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3
 MATH = {"exact": 0, "fast": 1, "fp32": 2}
 STATE = {"conservative": 0, "primitive": 1}
 VISCOSITY = {"constant": 0, "sutherland": 1}
+ALPHA_MODE = {"const3": 0, "local_max": 1}
+PHYSICS = {"reference": 0, "corrected_convection": 1}
 SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 
 EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len", "nsgpu_set_state",
@@ -55,6 +57,7 @@ class Config(ctypes.Structure):
         ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
         ("bc", ctypes.c_int * 6), ("math", ctypes.c_int),
         ("viscosity", ctypes.c_int), ("suth_C", ctypes.c_double),
+        ("alpha_mode", ctypes.c_int), ("physics", ctypes.c_int),
     ]
 
 
@@ -142,6 +145,8 @@ def make_config(case, math="exact"):
     c.math = MATH[math]
     c.viscosity = VISCOSITY[case.get("viscosity", "constant")]
     c.suth_C = case.get("suth_C", SUTH_C)
+    c.alpha_mode = ALPHA_MODE[case.get("alpha_mode", "const3")]
+    c.physics = PHYSICS[case.get("physics", "reference")]
     return c
 
 

@@ -174,7 +174,7 @@ def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None, speed_frac=1.0):
     # solenoidal projection: u_hat -= k (k . u_hat) / k^2
     dot = (KX * comps[0] + KY * comps[1] + KZ * comps[2]) / K2
     comps = [comps[0] - KX * dot, comps[1] - KY * dot, comps[2] - KZ * dot]
-    vel = [np.fft.irfftn(cc, s=(n, n, n)) for cc in comps]
+    vel = [np.fft.irfftn(cc, s=(n, n, n), axes=(0, 1, 2)) for cc in comps]
     urms = np.sqrt(np.mean(vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2))
     if c0 is None:
         # c0 with max |u_i| + c0 = speed_frac * alpha (alpha = 3, the LF split constant)
@@ -182,7 +182,7 @@ def isotropic_W(case, Mt=0.5, k0=4.0, seed=3, c0=None, speed_frac=1.0):
         c0 = speed_frac * ALPHA / (1.0 + Mt * vmax)
     scale = Mt * c0 / urms
     vel = [v * scale for v in vel]
-    dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(vel[0]), s=(n, n, n))
+    dudx = np.fft.irfftn(1j * KX * np.fft.rfftn(vel[0]), s=(n, n, n), axes=(0, 1, 2))
     uprime = np.sqrt(np.mean(vel[0] ** 2))
     lam = uprime / np.sqrt(np.mean(dudx ** 2))
     W = _empty(case)

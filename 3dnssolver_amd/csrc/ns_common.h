@@ -42,6 +42,8 @@ struct Phys {
     double heat;        // gamma / (Pr * (gamma - 1.0))  (centered_diffusion.cpp:161)
     int    conv;        // 0 none, 1 component-wise, 2 characteristic
     int    weights;     // 0 Shu, 1 Borges, 2 learned (network `mlp`)
+    int    alpha_mode;  // 0 alpha = 3 (the reference), 1 per-face maximum eigenvalue
+    int    physics;     // bit 0: corrected characteristic projection (true c, order, Roe state)
     int    diff_order;  // 0, 2, 4, 6
     int    suth;        // Sutherland viscosity
     double suth_C;      // Sutherland constant (T0 = 1)

@@ -259,6 +259,57 @@ __device__ __forceinline__ void split_cell(const double* __restrict__ U, const d
     }
 }
 
+// the per-face Lax-Friedrichs constant (alpha_mode LOCAL_MAX): the split depends on the face, so
+// a cell stores s[0..4] = F, s[5..9] = U and s[10] = its eigenvalue max(|q_n - c|, |q_n + c|)
+// (the value convection_flux.cpp:235-238 computes and leaves unused)
+template <int DIR>
+__device__ __forceinline__ void split_cell_la(const double* __restrict__ U,
+                                              const double* __restrict__ W, long c, long P,
+                                              real gamma, real s[11]) {
+    real rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
+    real qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;
+    real cs  = sqrt(dv(gamma * p, rho));
+#if !NS_FAST
+    real H = dv(cs * cs, gamma - RL(1.0)) + RL(0.5) * (u * u + v * v + w * w);
+#else
+    real H = dv(gamma, gamma - RL(1.0)) * dv(p, rho) + RL(0.5) * (u * u + v * v + w * w);
+#endif
+    real m = rho * qn;
+    s[0]   = m;
+    s[1]   = m * H;
+    s[2]   = (DIR == 0) ? m * u + p : m * u;
+    s[3]   = (DIR == 1) ? m * v + p : m * v;
+    s[4]   = (DIR == 2) ? m * w + p : m * w;
+#pragma unroll
+    for (int q = 0; q < 5; q++) s[5 + q] = U[c + q * P];
+    real e1 = fabs(qn - cs), e2 = fabs(qn + cs);
+    s[10]   = (e1 < e2) ? e2 : e1;  // std::max
+}
+
+// split values of one face from the stored F, U of its cells and the face's constant a:
+// (F + a U) / 2 and (F - a U) / 2 as in weno-5.cpp:94-110
+template <class S>
+struct AlphaSplit {
+    const S& raw;
+    real     a;
+    __device__ real operator()(int ii, int v) const {
+        return (v < 5) ? (raw(ii, v) + a * raw(ii, v + 5)) * RL(0.5)
+                       : (raw(ii, v - 5) - a * raw(ii, v)) * RL(0.5);
+    }
+};
+
+// Weno5::get_max_alpha (weno-5.cpp:324-330) over the cells i-2 .. i+2 of the face
+template <class S>
+__device__ __forceinline__ real face_alpha(const S& raw) {
+    real m = raw(0, 10);
+#pragma unroll
+    for (int ii = 1; ii < 5; ii++) {
+        real d = raw(ii, 10);
+        m      = (m < d) ? d : m;
+    }
+    return m;
+}
+
 // ------------------------------------------------------------------------------------------
 // Characteristic projection (convection_flux.cpp:63-175, weno-5cw.cpp:4-28)
 // ------------------------------------------------------------------------------------------
@@ -291,7 +342,7 @@ struct Eig {
     real L0[5], L2[5], L3[5];  // rows 0, 2, 3 of L (all entries depend on the state)
 };
 
-template <int DIR>
+template <int DIR, bool TRUEC>
 __device__ __forceinline__ void eigen(real rho, real p, real u, real v, real w,
                                       real gamma, Eig& E) {
     E.vel[0]  = u;
@@ -304,7 +355,7 @@ __device__ __forceinline__ void eigen(real rho, real p, real u, real v, real w,
     real q2 = u * u + v * v + w * w;
     real K  = gamma - RL(1.0);
 #if !NS_FAST
-    real c2   = dv(gamma, p * rho);
+    real c2   = TRUEC ? dv(gamma * p, rho) : dv(gamma, p * rho);
     real c    = sqrt(c2);
     real H    = dv(c2, gamma - RL(1.0)) + RL(0.5) * q2;
     real K_pc = dv(K, RL(2.0) * c2);
@@ -316,8 +367,8 @@ __device__ __forceinline__ void eigen(real rho, real p, real u, real v, real w,
     for (int a = 0; a < 3; a++) E.L2[1 + a] = dv(K * E.vel[a], rc2);
     E.L2[4] = dv(-K, rc2);
 #else
-    real c2   = dv(gamma, p * rho);
-    real ic2  = dv(p * rho, gamma);  // 1 / c^2
+    real c2   = TRUEC ? dv(gamma * p, rho) : dv(gamma, p * rho);
+    real ic2  = TRUEC ? dv(rho, gamma * p) : dv(p * rho, gamma);  // 1 / c^2
     real c    = sqrt(c2);
     real H    = dv(c2, K) + RL(0.5) * q2;
     real K_pc = RL(0.5) * K * ic2;
@@ -403,7 +454,13 @@ __device__ __forceinline__ void Rmul(const Eig& E, const real ft[5], real out[5]
 // (0..4 fL, 5..9 fR) of cell i-2+ii, ii = 0..5; Wl, Wr are the primitives of cells i and i+1
 // (Weno5::{x,y,z}_sweep weno-5.cpp:75-220, Weno5cw::{x,y,z}_sweep weno-5cw.cpp:30-262)
 // ------------------------------------------------------------------------------------------
-template <int DIR, int WT, int Q, class S>
+// stored position [rho, rho E, rho u, rho v, rho w] of conserved component p of
+// [rho, rho u, rho v, rho w, rho E]
+__device__ constexpr int stored_of(int p) { return p == 0 ? 0 : (p == 4 ? 1 : p + 1); }
+
+// CORR: L acts on the conserved order (physics CONVECTION); otherwise on the stored order as the
+// reference applies it
+template <int DIR, int WT, int Q, bool CORR, class S>
 __device__ __forceinline__ real char_component(const Eig& E, const S& st, const Learn& L) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
 #if NS_ROW_FENCE
@@ -415,8 +472,9 @@ __device__ __forceinline__ real char_component(const Eig& E, const S& st, const 
         real sl[5], sr[5];
 #pragma unroll
         for (int p = 0; p < 5; p++) {
-            sl[p] = st(ii, p);
-            sr[p] = st(ii + 1, 5 + p);
+            const int sp = CORR ? stored_of(p) : p;
+            sl[p]        = st(ii, sp);
+            sr[p]        = st(ii + 1, 5 + sp);
         }
         a[ii] = Lrow<DIR, Q>(E, sl);
         b[ii] = Lrow<DIR, Q>(E, sr);
@@ -425,9 +483,36 @@ __device__ __forceinline__ real char_component(const Eig& E, const S& st, const 
            weno_right<WT>(b[0], b[1], b[2], b[3], b[4], L);
 }
 
-template <int DIR, int CONV, int WT, class S>
-__device__ __forceinline__ void face_flux(const S& st, const real Wl[5], const real Wr[5],
+// Roe average of the primitives of the two cells of a face (physics CONVECTION), returned as
+// primitives [rho, p, u, v, w] with p = rho c^2 / gamma
+__device__ __forceinline__ void roe_state(const real l[5], const real r[5], real gamma,
+                                          real out[5]) {
+    real sl = sqrt(l[0]), sr = sqrt(r[0]);
+    real den = sl + sr;
+    real gg  = dv(gamma, gamma - RL(1.0));
+    real hl  = gg * dv(l[1], l[0]) + RL(0.5) * (l[2] * l[2] + l[3] * l[3] + l[4] * l[4]);
+    real hr  = gg * dv(r[1], r[0]) + RL(0.5) * (r[2] * r[2] + r[3] * r[3] + r[4] * r[4]);
+    real u   = dv(sl * l[2] + sr * r[2], den);
+    real v   = dv(sl * l[3] + sr * r[3], den);
+    real w   = dv(sl * l[4] + sr * r[4], den);
+    real H   = dv(sl * hl + sr * hr, den);
+    real c2  = (gamma - RL(1.0)) * (H - RL(0.5) * (u * u + v * v + w * w));
+    real rho = sl * sr;
+    out[0] = rho, out[1] = dv(rho * c2, gamma), out[2] = u, out[3] = v, out[4] = w;
+}
+
+// OPT bit 0: per-face alpha (st holds F, U and the eigenvalue per cell); bit 1: corrected
+// characteristic projection
+template <int DIR, int CONV, int WT, int OPT, class S>
+__device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const real Wr[5],
                                           real gamma, real out[5], const Learn& L) {
+    if constexpr (OPT & 1) {
+        AlphaSplit<S> st{raw, face_alpha(raw)};
+        face_flux<DIR, CONV, WT, OPT & ~1>(st, Wl, Wr, gamma, out, L);
+        return;
+    } else {
+    const S& st = raw;
+    constexpr bool CORR = (OPT & 2) != 0;
     if constexpr (CONV == 1) {
 #pragma unroll
         for (int q = 0; q < 5; q++)
@@ -436,30 +521,51 @@ __device__ __forceinline__ void face_flux(const S& st, const real Wl[5], const r
                                     st(5, 5 + q), L);
     } else {
         real Wa[5];
+        if constexpr (CORR) {
+            roe_state(Wl, Wr, gamma, Wa);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 5; q++) Wa[q] = RL(0.5) * (Wl[q] + Wr[q]);
+            for (int q = 0; q < 5; q++) Wa[q] = RL(0.5) * (Wl[q] + Wr[q]);
+        }
         Eig E;
-        eigen<DIR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
+        eigen<DIR, CORR>(Wa[0], Wa[1], Wa[2], Wa[3], Wa[4], gamma, E);
         real ft[5];
-        ft[0] = char_component<DIR, WT, 0>(E, st, L);
-        ft[1] = char_component<DIR, WT, 1>(E, st, L);
-        ft[2] = char_component<DIR, WT, 2>(E, st, L);
-        ft[3] = char_component<DIR, WT, 3>(E, st, L);
-        ft[4] = char_component<DIR, WT, 4>(E, st, L);
+        ft[0] = char_component<DIR, WT, 0, CORR>(E, st, L);
+        ft[1] = char_component<DIR, WT, 1, CORR>(E, st, L);
+        ft[2] = char_component<DIR, WT, 2, CORR>(E, st, L);
+        ft[3] = char_component<DIR, WT, 3, CORR>(E, st, L);
+        ft[4] = char_component<DIR, WT, 4, CORR>(E, st, L);
         Rmul<DIR>(E, ft, out);
+        if constexpr (CORR) {  // conserved order back to the stored order
+            real m0 = out[1], m1 = out[2], m2 = out[3], e = out[4];
+            out[1] = e, out[2] = m0, out[3] = m1, out[4] = m2;
+        }
+    }
     }
 }
 
+template <int NV>
 struct RegSplit {  // split values held in registers: s[ii][v]
-    const real (*s)[10];
+    const real (*s)[NV];
     __device__ real operator()(int ii, int v) const { return s[ii][v]; }
 };
 
-template <int LANES>
+template <int LANES, int NV>
 struct LdsSplit {  // split values in LDS, layout [cell][v][lane]
     const real* base;  // &tile[first cell][0][lane]
-    __device__ real operator()(int ii, int v) const { return base[(ii * 10 + v) * LANES]; }
+    __device__ real operator()(int ii, int v) const { return base[(ii * NV + v) * LANES]; }
 };
+
+// values stored per cell: the split (10), or F, U and the eigenvalue (11) for a per-face alpha
+template <int OPT> constexpr int split_nv() { return (OPT & 1) ? 11 : 10; }
+
+template <int DIR, int OPT>
+__device__ __forceinline__ void stage_cell(const double* __restrict__ U,
+                                           const double* __restrict__ W, long c, long P,
+                                           real gamma, real* sv) {
+    if constexpr (OPT & 1) split_cell_la<DIR>(U, W, c, P, gamma, sv);
+    else split_cell<DIR>(U, W, c, P, gamma, sv);
+}
 
 // ------------------------------------------------------------------------------------------
 // face-flux kernels.  Face i+1/2 of the sweep direction is stored at cell i.  The sweep index
@@ -476,7 +582,7 @@ constexpr int TILE_L = 64;   // lanes along k
 // stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
 // values of the 13 cells the tile needs are computed once into LDS.  (bx, by, bz) = (k tile,
 // row tile, sweep tile).
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ f, const Geo& g,
@@ -484,6 +590,7 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   real* __restrict__ tile,
                                                   const MlpPacked* __restrict__ mlp) {
     constexpr int NC = TILE_S + 5;
+    constexpr int NV = split_nv<OPT>();
     const int lane = threadIdx.x, w = threadIdx.y;
     const int rows = TILE_L / kw;
     const int k    = NGC + bx * kw + (lane & (kw - 1));
@@ -498,10 +605,10 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     for (int t = w; t < NC; t += TILE_S) {
         int cs = s0 + t;
         if (kin && cs <= n + 2 * NGC - 1) {
-            real sv[10];
-            split_cell<DIR>(U, W, (long)cs * ss + (long)o * so + k, g.P, gamma, sv);
+            real sv[NV];
+            stage_cell<DIR, OPT>(U, W, (long)cs * ss + (long)o * so + k, g.P, gamma, sv);
 #pragma unroll
-            for (int v = 0; v < 10; v++) tile[(t * 10 + v) * TILE_L + lane] = sv[v];
+            for (int v = 0; v < NV; v++) tile[(t * NV + v) * TILE_L + lane] = sv[v];
         }
     }
     __syncthreads();
@@ -519,35 +626,37 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
 #pragma unroll
         for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + ss + q * g.P];
     }
-    LdsSplit<TILE_L> st{&tile[(w * 10) * TILE_L + lane]};
-    real           out[5];
-    face_flux<DIR, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    LdsSplit<TILE_L, NV> st{&tile[(w * NV) * TILE_L + lane]};
+    real               out[5];
+    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
                                                       const double* __restrict__ W,
                                                       double* __restrict__ f, Geo g,
                                                       real gamma, const MlpPacked* mlp) {
-    __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
-    flux_tile_xy_body<DIR, CONV, WT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
-                                     TILE_L, tile, mlp);
+    __shared__ real tile[(TILE_S + 5) * split_nv<OPT>() * TILE_L];
+    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
+                                          TILE_L, tile, mlp);
 }
 
 // z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
 // each, a block 8 waves; each row stages the split values of its kw + 5 cells in LDS.
 // (bx, by, bz) = (segment, row tile, i).
-constexpr int ZLDS_WAVE_MAX = 4 * (16 + 5) * 10;  // the largest per-wave need (kw = 16)
-template <int CONV, int WT>
+// the largest per-wave need (kw = 16: 4 rows of 21 cells)
+template <int OPT> constexpr int zlds_wave_max() { return 4 * (16 + 5) * split_nv<OPT>(); }
+template <int CONV, int WT, int OPT>
 __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
                                                  const double* __restrict__ W,
                                                  double* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
                                                  real* __restrict__ lds,
                                                  const MlpPacked* __restrict__ mlp) {
+    constexpr int NV = split_nv<OPT>();
     const int NC   = kw + 5;
     const int sub  = TILE_L / kw;                   // rows per wave
     const int lane = threadIdx.x, r = threadIdx.y;
@@ -558,14 +667,14 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
     const int f0   = bx * kw;                       // first face of the segment (0-based)
     const bool rowok = (j < NGC + g.ny);            // every wave reaches the barrier below
     const long row = (long)i * g.sx + (long)j * g.sy;
-    real*      T   = lds + (r * sub + sr) * (NC * 10);
+    real*      T   = lds + (r * sub + sr) * (NC * NV);
     for (int t = kk; t < NC; t += kw) {
         int cs = f0 + t;  // padded k of the cell
         if (rowok && cs <= g.nz + 2 * NGC - 1) {
-            real sv[10];
-            split_cell<2>(U, W, row + cs, g.P, gamma, sv);
+            real sv[NV];
+            stage_cell<2, OPT>(U, W, row + cs, g.P, gamma, sv);
 #pragma unroll
-            for (int v = 0; v < 10; v++) T[v * NC + t] = sv[v];
+            for (int v = 0; v < NV; v++) T[v * NC + t] = sv[v];
         }
     }
     __syncthreads();  // rows are independent; each wave only reads its own rows
@@ -587,25 +696,25 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
         __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
     } st{&T[kk], NC};
     real out[5];
-    face_flux<2, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    face_flux<2, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int CONV, int WT>
+template <int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      double* __restrict__ f, Geo g,
                                                      real gamma, const MlpPacked* mlp) {
-    __shared__ real tile[8 * (TILE_L + 5) * 10];
-    flux_tile_z_body<CONV, WT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z, TILE_L,
-                               tile, mlp);
+    __shared__ real tile[8 * (TILE_L + 5) * split_nv<OPT>()];
+    flux_tile_z_body<CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
+                                    TILE_L, tile, mlp);
 }
 
 // one plane of faces (sweep index `i`), one thread per face, split values in registers;
 // block (64, 4) over the tangential indices, (bx, by) = (fast, slow) tile
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
                                                 const double* __restrict__ W,
                                                 double* __restrict__ f, const Geo& g, real gamma,
@@ -625,28 +734,29 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
     if (DIR == 0) c = (long)i * g.sx + (long)(t1 + NGC) * g.sy + (t2 + NGC), s = g.sx;
     else if (DIR == 1) c = (long)(t1 + NGC) * g.sx + (long)i * g.sy + (t2 + NGC), s = g.sy;
     else c = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy + i, s = 1;
-    real sv[6][10];
+    constexpr int NV = split_nv<OPT>();
+    real          sv[6][NV];
 #pragma unroll
-    for (int ii = 0; ii < 6; ii++) split_cell<DIR>(U, W, c + (ii - 2) * s, g.P, gamma, sv[ii]);
+    for (int ii = 0; ii < 6; ii++) stage_cell<DIR, OPT>(U, W, c + (ii - 2) * s, g.P, gamma, sv[ii]);
     real Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
         for (int q = 0; q < 5; q++) Wl[q] = W[c + q * g.P], Wr[q] = W[c + s + q * g.P];
     }
-    RegSplit st{sv};
-    real   out[5];
-    face_flux<DIR, CONV, WT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    RegSplit<NV> st{sv};
+    real         out[5];
+    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
                                                     const double* __restrict__ W,
                                                     double* __restrict__ f, Geo g, real gamma,
                                                     int i, const MlpPacked* mlp) {
-    flux_plane_body<DIR, CONV, WT>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y, mlp);
+    flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y, mlp);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1270,7 +1380,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict
         else {
             double cs = sqrt(ph.gamma * w[1] / w[0]);
             double m  = fmax(fabs(w[2]), fmax(fabs(w[3]), fabs(w[4]))) + cs;
-            if (m > LF_ALPHA) fl |= 4u;
+            if (ph.alpha_mode == 0 && m > LF_ALPHA) fl |= 4u;
             double d = local_dt(w, ph.gamma, g.dx, g.dy, g.dz);
             dtl      = (d < dtl) ? d : dtl;
         }
@@ -1401,9 +1511,9 @@ __global__ void kb_halo(EnsTable T, int s) {
     halo_body(U, 5, Wp, 5, g, bd, bx, by, bz);
 }
 
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T, int s) {
-    __shared__ real tile[(TILE_S + 5) * 10 * TILE_L];
+    __shared__ real tile[(TILE_S + 5) * split_nv<OPT>() * TILE_L];
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, DIR == 0 ? EF_FLUX_X : EF_FLUX_Y, bx, by, bz);
     const Geo       g   = E->g;
@@ -1412,12 +1522,12 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_tile_xy_body<DIR, CONV, WT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
+    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
 }
 
-template <int CONV, int WT>
+template <int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T, int s) {
-    __shared__ real tile[8 * ZLDS_WAVE_MAX];
+    __shared__ real tile[8 * zlds_wave_max<OPT>()];
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_FLUX_Z, bx, by, bz);
     const Geo       g   = E->g;
@@ -1426,10 +1536,10 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T,
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[2];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_tile_z_body<CONV, WT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
+    flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
 }
 
-template <int DIR, int CONV, int WT>
+template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_PLANE_X + DIR, bx, by, bz);
@@ -1439,7 +1549,7 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_plane_body<DIR, CONV, WT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp);
+    flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp);
 }
 
 template <int ORDER>
@@ -1541,18 +1651,18 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
 
-template <int DIR, int CONV, int B>
+template <int DIR, int CONV, int B, int OPT>
 static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
                      const MlpPacked* mlp, hipStream_t s) {
     dim3 blk(TILE_L, 8, 1);
     if constexpr (DIR == 2) {
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
-        k_flux_tile_z<CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
+        k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     } else {
         int  n = (DIR == 0) ? g.nx : g.ny;
         int  o = (DIR == 0) ? g.ny : g.nx;
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S - 1) / TILE_S);
-        k_flux_tile_xy<DIR, CONV, B><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
+        k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     }
     // the last face plane (sweep index NGC + n - 1)
     int  n  = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
@@ -1560,21 +1670,45 @@ static void flux_dir(const double* U, const double* W, double* f, Geo g, double 
     int  n2 = (DIR == 2) ? g.ny : g.nz;
     dim3 pb(64, 4, 1);
     dim3 pg((n2 + 63) / 64, (n1 + 3) / 4, 1);
-    k_flux_plane<DIR, CONV, B><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp);
+    k_flux_plane<DIR, CONV, B, OPT><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp);
+}
+
+// kernel options of a configuration: bit 0 per-face alpha, bit 1 corrected characteristic
+// projection (characteristic sweeps only).  The learned weights run with neither (the host
+// refuses the combination).
+static int flux_opt(const Phys& ph) {
+    return (ph.alpha_mode == 1 ? 1 : 0) | ((ph.conv == 2 && (ph.physics & 1)) ? 2 : 0);
+}
+
+template <int DIR, int CONV, int B>
+static void flux_dir_opt(const double* U, const double* W, double* f, Geo g, double gamma,
+                         const MlpPacked* m, int opt, hipStream_t s) {
+    if constexpr (B == 2) {
+        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, s);
+    } else if constexpr (CONV == 1) {
+        if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, s);
+        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, s);
+    } else {
+        if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, s);
+        else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, s);
+        else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, s);
+        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, s);
+    }
 }
 
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
                           hipStream_t s) {
-    const MlpPacked* m = ph.mlp;
+    const MlpPacked* m   = ph.mlp;
+    const int        opt = flux_opt(ph);
     if (ph.conv == 1) {
-        if (ph.weights == 1) flux_dir<DIR, 1, 1>(U, W, f, g, ph.gamma, m, s);
-        else if (ph.weights == 2) flux_dir<DIR, 1, 2>(U, W, f, g, ph.gamma, m, s);
-        else flux_dir<DIR, 1, 0>(U, W, f, g, ph.gamma, m, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, s);
+        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, s);
     } else {
-        if (ph.weights == 1) flux_dir<DIR, 2, 1>(U, W, f, g, ph.gamma, m, s);
-        else if (ph.weights == 2) flux_dir<DIR, 2, 2>(U, W, f, g, ph.gamma, m, s);
-        else flux_dir<DIR, 2, 0>(U, W, f, g, ph.gamma, m, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, s);
+        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, s);
     }
 }
 
@@ -1662,28 +1796,44 @@ static void l_selftest_div(long n, unsigned long long seed, unsigned long long* 
     k_selftest_div<<<1024, 256, 0, s>>>(n, seed, count);
 }
 
-template <int CONV, int B>
-static void ens_fluxes(const EnsTable& T, const int* total, int s, hipStream_t st) {
+template <int CONV, int B, int OPT>
+static void ens_fluxes_t(const EnsTable& T, const int* total, int s, hipStream_t st) {
     dim3 tb(TILE_L, 8, 1), pb(64, 4, 1);
-    if (total[EF_FLUX_X]) kb_flux_tile_xy<0, CONV, B><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_X]) kb_flux_plane<0, CONV, B><<<total[EF_PLANE_X], pb, 0, st>>>(T, s);
-    if (total[EF_FLUX_Y]) kb_flux_tile_xy<1, CONV, B><<<total[EF_FLUX_Y], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_Y]) kb_flux_plane<1, CONV, B><<<total[EF_PLANE_Y], pb, 0, st>>>(T, s);
-    if (total[EF_FLUX_Z]) kb_flux_tile_z<CONV, B><<<total[EF_FLUX_Z], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_Z]) kb_flux_plane<2, CONV, B><<<total[EF_PLANE_Z], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_X]) kb_flux_tile_xy<0, CONV, B, OPT><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_X]) kb_flux_plane<0, CONV, B, OPT><<<total[EF_PLANE_X], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Y]) kb_flux_tile_xy<1, CONV, B, OPT><<<total[EF_FLUX_Y], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Y]) kb_flux_plane<1, CONV, B, OPT><<<total[EF_PLANE_Y], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Z]) kb_flux_tile_z<CONV, B, OPT><<<total[EF_FLUX_Z], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Z]) kb_flux_plane<2, CONV, B, OPT><<<total[EF_PLANE_Z], pb, 0, st>>>(T, s);
+}
+
+template <int CONV, int B>
+static void ens_fluxes(const EnsTable& T, const int* total, int s, int opt, hipStream_t st) {
+    if constexpr (B == 2) {
+        ens_fluxes_t<CONV, 2, 0>(T, total, s, st);
+    } else if constexpr (CONV == 1) {
+        if (opt & 1) ens_fluxes_t<1, B, 1>(T, total, s, st);
+        else ens_fluxes_t<1, B, 0>(T, total, s, st);
+    } else {
+        if (opt == 1) ens_fluxes_t<2, B, 1>(T, total, s, st);
+        else if (opt == 2) ens_fluxes_t<2, B, 2>(T, total, s, st);
+        else if (opt == 3) ens_fluxes_t<2, B, 3>(T, total, s, st);
+        else ens_fluxes_t<2, B, 0>(T, total, s, st);
+    }
 }
 
 template <int ORDER, bool SUTH>
 static void ens_stage(const EnsTable& T, const int* total, const Phys& ph, int s, hipStream_t st) {
     kb_halo<<<total[EF_HALO], dim3(64, 4, 1), 0, st>>>(T, s);
+    const int opt = flux_opt(ph);
     if (ph.conv == 1) {
-        if (ph.weights == 1) ens_fluxes<1, 1>(T, total, s, st);
-        else if (ph.weights == 2) ens_fluxes<1, 2>(T, total, s, st);
-        else ens_fluxes<1, 0>(T, total, s, st);
+        if (ph.weights == 1) ens_fluxes<1, 1>(T, total, s, opt, st);
+        else if (ph.weights == 2) ens_fluxes<1, 2>(T, total, s, opt, st);
+        else ens_fluxes<1, 0>(T, total, s, opt, st);
     } else if (ph.conv == 2) {
-        if (ph.weights == 1) ens_fluxes<2, 1>(T, total, s, st);
-        else if (ph.weights == 2) ens_fluxes<2, 2>(T, total, s, st);
-        else ens_fluxes<2, 0>(T, total, s, st);
+        if (ph.weights == 1) ens_fluxes<2, 1>(T, total, s, opt, st);
+        else if (ph.weights == 2) ens_fluxes<2, 2>(T, total, s, opt, st);
+        else ens_fluxes<2, 0>(T, total, s, opt, st);
     }
     if constexpr (ORDER > 0) {
         kb_deriv1<ORDER><<<total[EF_DERIV1], 256, 0, st>>>(T, s);

@@ -115,6 +115,11 @@ int validate(const nsgpu_config* c) {
     if (c->viscosity != NSGPU_VISC_CONSTANT && c->viscosity != NSGPU_VISC_SUTHERLAND)
         return NSGPU_ERR_INVALID;
     if (c->viscosity == NSGPU_VISC_SUTHERLAND && !(c->suth_C > 0)) return NSGPU_ERR_INVALID;
+    if (c->alpha_mode != NSGPU_ALPHA_CONST3 && c->alpha_mode != NSGPU_ALPHA_LOCAL_MAX)
+        return NSGPU_ERR_INVALID;
+    if (c->physics & ~NSGPU_PHYS_CONVECTION) return NSGPU_ERR_INVALID;
+    if (c->weights == NSGPU_WEIGHTS_LEARNED && (c->alpha_mode || c->physics))
+        return NSGPU_ERR_UNSUPPORTED;
     // periodicity per axis from West / South / Front (decomposition.cpp:145-161); every face of a
     // non-periodic axis must be Neumann (domain.cpp:87-137 leaves other kinds without a boundary)
     static const int lead[3] = {0, 3, 4};
@@ -229,6 +234,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     for (int f = 0; f < 6; f++) c->b.face_kind[f] = cfg->bc[f];
     c->ph.gamma = cfg->gamma, c->ph.mu = cfg->mu, c->ph.Pr = cfg->Pr, c->ph.cfl = cfg->cfl;
     c->ph.conv = cfg->conv, c->ph.weights = cfg->weights;
+    c->ph.alpha_mode = cfg->alpha_mode, c->ph.physics = cfg->physics;
     c->ph.mlp  = nullptr;  // set by nsgpu_set_mlp
     c->ph.diff_order = cfg->diff_order;
     c->ph.lamb       = (-2. / 3.) * cfg->mu;                           // centered_diffusion.cpp:101
@@ -766,7 +772,8 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
         if (!c || c->device != c0->device || c->L != c0->L || c->ph.conv != c0->ph.conv ||
             c->ph.weights != c0->ph.weights || c->ph.diff_order != c0->ph.diff_order ||
             (c->ph.weights == NSGPU_WEIGHTS_LEARNED && !c->ph.mlp) ||
-            c->ph.suth != c0->ph.suth || c->g.dim != c0->g.dim)
+            c->ph.suth != c0->ph.suth || c->g.dim != c0->g.dim ||
+            c->ph.alpha_mode != c0->ph.alpha_mode || c->ph.physics != c0->ph.physics)
             return NSGPU_ERR_INVALID;
         for (int a = 0; a < b; a++)
             if (members[a] == c) return NSGPU_ERR_INVALID;

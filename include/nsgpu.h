@@ -73,6 +73,20 @@ enum { NSGPU_STATE_CONSERVATIVE = 0, NSGPU_STATE_PRIMITIVE = 1 };
  * T = gamma p / rho (Viscosity::mu_sutherland, viscosity.cpp:5-11, which the reference defines
  * but never calls) feeding the dmu slots of the same residual */
 enum { NSGPU_VISC_CONSTANT = 0, NSGPU_VISC_SUTHERLAND = 1 };
+/* Lax-Friedrichs constant of the flux split (extension): CONST3 is the reference's alpha = 3.0
+ * (ConvectionFlux::max_eigenvalue, convection_flux.cpp:241); LOCAL_MAX takes, per face i+1/2, the
+ * largest max(|q_n - c|, |q_n + c|) over the cells i-2 .. i+2 (the eigenvalue that line leaves
+ * commented out, maximised by Weno5::get_max_alpha, weno-5.cpp:324-330).  With LOCAL_MAX the
+ * split stays upwind for any sound speed, and nsgpu_reduce raises no alpha flag. */
+enum { NSGPU_ALPHA_CONST3 = 0, NSGPU_ALPHA_LOCAL_MAX = 1 };
+/* corrections of reference behaviour (extension, bit mask; 0 = the reference):
+ *   CONVECTION  the characteristic projection of Weno5cw with the eigenvectors of the true flux
+ *               Jacobian: c^2 = gamma p / rho (convection_flux.cpp:139 has gamma / (p rho)),
+ *               L and R in the order of the stored conserved vector (the reference builds them
+ *               for [rho, rho u, rho v, rho w, rho E] and applies them to [rho, rho E, rho u,
+ *               rho v, rho w], weno-5cw.cpp:92-93), and the Roe-averaged face state instead of
+ *               the mean of the primitives (weno-5cw.cpp:67-70) */
+enum { NSGPU_PHYS_REFERENCE = 0, NSGPU_PHYS_CONVECTION = 1 };
 enum { NSGPU_RHS_CONVECTION = 1, NSGPU_RHS_DIFFUSION = 2, NSGPU_RHS_BOTH = 3 };
 
 typedef struct {
@@ -88,6 +102,9 @@ typedef struct {
     int    math;        /* NSGPU_MATH_* */
     int    viscosity;   /* NSGPU_VISC_*: CONSTANT (the reference path) or SUTHERLAND */
     double suth_C;      /* Sutherland constant over the reference temperature (T0 = 1) */
+    int    alpha_mode;  /* NSGPU_ALPHA_*; 0 = the reference */
+    int    physics;     /* NSGPU_PHYS_* bits; 0 = the reference.  The learned weights run with
+                           alpha_mode 0 and physics 0 only (NSGPU_ERR_UNSUPPORTED otherwise) */
 } nsgpu_config;
 
 /* Create a context on GPU `device` (HIP ordinal).  Allocates the block's device arrays. */

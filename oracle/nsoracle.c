@@ -335,6 +335,26 @@ static void form_flux(const nso_cfg* c, const double* W, int dir, double* F) {
  * the constant 3.0, so Weno5::get_max_alpha (weno-5.cpp:324-330) of five of them is 3.0. */
 static const double lf_alpha = 3.0;
 
+/* the eigenvalue convection_flux.cpp:235-238 computes and leaves unused (alpha_mode LOCAL_MAX):
+ * max(|q_n - c|, |q_n + c|), std::max(a, b) = (a < b) ? b : a */
+static double cell_eig(double g, const double* w, int dir) {
+    double q_n  = w[2] * unit_normal[dir][0] + w[3] * unit_normal[dir][1] + w[4] * unit_normal[dir][2];
+    double c    = sqrt(g * w[1] / w[0]);
+    double eig1 = fabs(q_n - c);
+    double eig2 = fabs(q_n + c);
+    return (eig1 < eig2) ? eig2 : eig1;
+}
+
+static double max_d(double a, double b) { return (a < b) ? b : a; }
+
+/* Weno5::get_max_alpha (weno-5.cpp:324-330) over the cells i-2 .. i+2 of face i+1/2 */
+static double face_alpha(const nso_cfg* c, const double* W, size_t base, size_t s, int dir) {
+    if (c->alpha_mode != NSO_ALPHA_LOCAL_MAX) return lf_alpha;
+    double d[5];
+    for (int ii = 0; ii < 5; ii++) d[ii] = cell_eig(c->gamma, &W[base + (ii - 2) * s], dir);
+    return max_d(max_d(max_d(max_d(d[0], d[1]), d[2]), d[3]), d[4]);
+}
+
 /* ConvectionFlux::orthogonal_unit_vectors, convection_flux.cpp:63-115 */
 static void orthogonal_unit_vectors(const double n[3], double out[6]) {
     double nx = n[0], ny = n[1], nz = n[2];
@@ -369,14 +389,15 @@ static void orthogonal_unit_vectors(const double n[3], double out[6]) {
     out[0] = lx, out[1] = ly, out[2] = lz, out[3] = mx, out[4] = my, out[5] = mz;
 }
 
-/* ConvectionFlux::eigenvectors, convection_flux.cpp:117-175 (c2 = gamma/(p*rho) as written) */
+/* ConvectionFlux::eigenvectors, convection_flux.cpp:117-175 (c2 = gamma/(p*rho) as written;
+ * true_c: gamma*p/rho) */
 static void eigenvectors(double rho, double p, double u, double v, double w, double gamma,
                          const double n[3], const double l[3], const double m[3], double* L,
-                         double* R) {
+                         double* R, int true_c) {
     double nx = n[0], ny = n[1], nz = n[2];
     double lx = l[0], ly = l[1], lz = l[2];
     double mx = m[0], my = m[1], mz = m[2];
-    double c2   = gamma / (p * rho);
+    double c2   = true_c ? gamma * p / rho : gamma / (p * rho);
     double c    = sqrt(c2);
     double qn   = u * nx + v * ny + w * nz;
     double ql   = u * lx + v * ly + w * lz;
@@ -423,7 +444,57 @@ void nso_eigenvectors(const double Wavg[5], double gamma, int dir, double L[25],
     double o[6];
     orthogonal_unit_vectors(unit_normal[dir], o);
     eigenvectors(Wavg[0], Wavg[1], Wavg[2], Wavg[3], Wavg[4], gamma, unit_normal[dir], o, o + 3, L,
-                 R);
+                 R, 0);
+}
+
+/* Roe average of two primitive states (physics CONVECTION): sqrt(rho)-weighted velocity and
+ * total enthalpy, rho = sqrt(rho_l rho_r), c^2 = (gamma - 1)(H - |u|^2 / 2), returned as the
+ * primitive state [rho, p, u, v, w] with p = rho c^2 / gamma */
+static void roe_state(double g, const double* l, const double* r, double* out) {
+    double sl  = sqrt(l[0]), sr = sqrt(r[0]);
+    double den = sl + sr;
+    double hl  = g / (g - 1.0) * (l[1] / l[0]) + 0.5 * (l[2] * l[2] + l[3] * l[3] + l[4] * l[4]);
+    double hr  = g / (g - 1.0) * (r[1] / r[0]) + 0.5 * (r[2] * r[2] + r[3] * r[3] + r[4] * r[4]);
+    double u   = (sl * l[2] + sr * r[2]) / den;
+    double v   = (sl * l[3] + sr * r[3]) / den;
+    double w   = (sl * l[4] + sr * r[4]) / den;
+    double H   = (sl * hl + sr * hr) / den;
+    double c2  = (g - 1.0) * (H - 0.5 * (u * u + v * v + w * w));
+    double rho = sl * sr;
+    out[0] = rho, out[1] = rho * c2 / g, out[2] = u, out[3] = v, out[4] = w;
+}
+
+/* conserved index p of [rho, rho u, rho v, rho w, rho E] -> stored index [rho, rho E, ...] */
+static const int cons_to_stored[5] = {0, 2, 3, 4, 1};
+
+/* the face state and eigenvectors of the characteristic sweep, with L and R permuted to act on
+ * stored-order vectors when the convection is corrected */
+static void face_eigensystem(const nso_cfg* c, const double* wl, const double* wr, int dir,
+                             const double o[6], double* Wf, double* L, double* R) {
+    int corr = (c->physics & NSO_PHYS_CONVECTION) != 0;
+    if (corr) roe_state(c->gamma, wl, wr, Wf);
+    else
+        for (int q = 0; q < NQ; q++) Wf[q] = 0.5 * (wl[q] + wr[q]);
+    double Lc[25], Rc[25];
+    eigenvectors(Wf[0], Wf[1], Wf[2], Wf[3], Wf[4], c->gamma, unit_normal[dir], o, o + 3, Lc, Rc,
+                 corr);
+    if (!corr) {
+        memcpy(L, Lc, sizeof Lc), memcpy(R, Rc, sizeof Rc);
+        return;
+    }
+    /* L_s[q][cons_to_stored[p]] = L[q][p]; R_s[cons_to_stored[p]][q] = R[p][q] */
+    for (int q = 0; q < 5; q++)
+        for (int p = 0; p < 5; p++) {
+            L[q * 5 + cons_to_stored[p]] = Lc[q * 5 + p];
+            R[cons_to_stored[p] * 5 + q] = Rc[p * 5 + q];
+        }
+}
+
+void nso_face_eigensystem(const nso_cfg* c, const double Wl[5], const double Wr[5], int dir,
+                          double Wface[5], double L[25], double R[25]) {
+    double o[6];
+    orthogonal_unit_vectors(unit_normal[dir], o);
+    face_eigensystem(c, Wl, Wr, dir, o, Wface, L, R);
 }
 
 /* Weno5cw::compute_num_flux, weno-5cw.cpp:4-28, with matrix::square_matrix_matrix_product and
@@ -474,12 +545,12 @@ static void sweep(const nso_cfg* c, int dir, int charwise, const double* F, cons
     lo[dir] -= 1;
     double o[6];
     orthogonal_unit_vectors(unit_normal[dir], o);
-    const double alpha = lf_alpha;
 #pragma omp parallel for schedule(static)
     for (long i = lo[0]; i < hi[0]; i++)
         for (long j = lo[1]; j < hi[1]; j++)
             for (long k = lo[2]; k < hi[2]; k++) {
                 size_t base = ((((size_t)i) * P.NY + j) * P.NZ + k) * NQ;
+                const double alpha = face_alpha(c, W, base, s, dir);
                 if (!charwise) {
                     for (int q = 0; q < NQ; q++) {
                         double fl[5], fr[5];
@@ -493,11 +564,29 @@ static void sweep(const nso_cfg* c, int dir, int charwise, const double* F, cons
                         double right = weno_right(fr[0], fr[1], fr[2], fr[3], fr[4], c->weights);
                         f[base + q]  = left + right;
                     }
+                } else if (c->physics & NSO_PHYS_CONVECTION) {
+                    /* the same product in the conserved order: fL[p][ii] holds the stored
+                     * component cons_to_stored[p], the result lands back in stored order */
+                    double Wf[5], L[25], R[25], fL[25], fR[25], out[5];
+                    double wl[5], wr[5];
+                    for (int q = 0; q < NQ; q++) wl[q] = W[base + q], wr[q] = W[base + s + q];
+                    roe_state(c->gamma, wl, wr, Wf);
+                    eigenvectors(Wf[0], Wf[1], Wf[2], Wf[3], Wf[4], c->gamma, unit_normal[dir], o,
+                                 o + 3, L, R, 1);
+                    for (int ii = 0; ii < 5; ii++)
+                        for (int p = 0; p < NQ; p++) {
+                            size_t a       = base + (size_t)(ii - 2) * s + cons_to_stored[p];
+                            size_t b       = a + s;
+                            fL[p * 5 + ii] = (F[a] + alpha * U[a]) * 0.5;
+                            fR[p * 5 + ii] = (F[b] - alpha * U[b]) * 0.5;
+                        }
+                    compute_num_flux(fL, fR, L, R, c->weights, out);
+                    for (int p = 0; p < NQ; p++) f[base + cons_to_stored[p]] = out[p];
                 } else {
                     double Wavg[5], L[25], R[25], fL[25], fR[25], out[5];
                     for (int q = 0; q < NQ; q++) Wavg[q] = 0.5 * (W[base + q] + W[base + s + q]);
                     eigenvectors(Wavg[0], Wavg[1], Wavg[2], Wavg[3], Wavg[4], c->gamma,
-                                 unit_normal[dir], o, o + 3, L, R);
+                                 unit_normal[dir], o, o + 3, L, R, 0);
                     for (int ii = 0; ii < 5; ii++)
                         for (int q = 0; q < NQ; q++) {
                             size_t a       = base + (size_t)(ii - 2) * s + q;
@@ -742,9 +831,16 @@ static void diffusion(const nso_cfg* c, const double* W, double* R) {
 /* RHS and time stepping                                                                      */
 /* ------------------------------------------------------------------------------------------ */
 
+double nso_face_alpha(const nso_cfg* c, const double* W, int dir, long i, long j, long k) {
+    pad_t P = pad_of(c);
+    return face_alpha(c, W, ((((size_t)i) * P.NY + j) * P.NZ + k) * NQ, axis_stride(P, dir), dir);
+}
+
 static int cfg_ok(const nso_cfg* c) {
     if (c->nx < 1 || c->ny < 1 || c->nz < 1) return 0;
     if (c->conv < 0 || c->conv > 2) return 0;
+    if (c->alpha_mode < 0 || c->alpha_mode > 1) return 0;
+    if (c->physics & ~NSO_PHYS_CONVECTION) return 0;
     if (c->diff_order != 0 && c->diff_order != 2 && c->diff_order != 4 && c->diff_order != 6)
         return 0;
     return 1;

@@ -39,7 +39,29 @@ typedef struct {
     int    bc[6];      /* NSO_BC_*, W,E,N,S,F,B */
     int    viscosity;  /* 0: constant mu (the reference path); 1: Sutherland law (extension) */
     double suth_C;     /* Sutherland constant over the reference temperature (T0 = 1) */
+    int    alpha_mode; /* NSO_ALPHA_*: the Lax-Friedrichs constant of the flux split */
+    int    physics;    /* NSO_PHYS_* bits: corrections of reference behaviour (extension) */
 } nso_cfg;
+
+/* CONST3: alpha = 3.0 everywhere (convection_flux.cpp:241, the reference).  LOCAL_MAX: the
+ * eigenvalue that line has commented out, max(|q_n - c|, |q_n + c|) with c = sqrt(gamma p / rho),
+ * taken per face as the maximum over the cells i-2 .. i+2 (Weno5::get_max_alpha,
+ * weno-5.cpp:324-330, weno-5cw.cpp:60-64). */
+enum { NSO_ALPHA_CONST3 = 0, NSO_ALPHA_LOCAL_MAX = 1 };
+/* CONVECTION: the characteristic projection of Weno5cw uses the eigenvectors of the true flux
+ * Jacobian: c^2 = gamma p / rho (convection_flux.cpp:139 writes gamma / (p rho)), L and R in the
+ * order of the conserved vector they act on (the reference builds them for
+ * [rho, rho u, rho v, rho w, rho E] and applies them to [rho, rho E, rho u, rho v, rho w]), and the
+ * Roe-averaged face state instead of the mean of the primitives (weno-5cw.cpp:67-70). */
+enum { NSO_PHYS_CONVECTION = 1 };
+
+/* the face state, L and R that the characteristic sweep applies at the face between the cells
+ * with primitives Wl and Wr, L and R acting on vectors in the stored order [rho, rho E, rho u,
+ * rho v, rho w] (row-major 5x5) */
+void nso_face_eigensystem(const nso_cfg* c, const double Wl[5], const double Wr[5], int dir,
+                          double Wface[5], double L[25], double R[25]);
+/* the Lax-Friedrichs constant of the face i+1/2 along dir (padded indices of cell i) */
+double nso_face_alpha(const nso_cfg* c, const double* W, int dir, long i, long j, long k);
 
 /* number of doubles of one padded 5-variable array */
 size_t nso_array_len(const nso_cfg* c);

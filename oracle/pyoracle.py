@@ -22,6 +22,8 @@ MLP_NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3
 REGIMES = {"smooth": 0, "mlp": 1, "flagged": 2}
 VISCOSITY = {"constant": 0, "sutherland": 1}
 SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
+ALPHA_MODE = {"const3": 0, "local_max": 1}
+PHYSICS = {"reference": 0, "corrected_convection": 1}
 
 
 class NsoCfg(ctypes.Structure):
@@ -33,6 +35,7 @@ class NsoCfg(ctypes.Structure):
         ("conv", ctypes.c_int), ("weights", ctypes.c_int), ("diff_order", ctypes.c_int),
         ("bc", ctypes.c_int * 6),
         ("viscosity", ctypes.c_int), ("suth_C", ctypes.c_double),
+        ("alpha_mode", ctypes.c_int), ("physics", ctypes.c_int),
     ]
 
 
@@ -51,6 +54,8 @@ def make_cfg(case):
         c.bc[f] = BC[name]
     c.viscosity = VISCOSITY[case.get("viscosity", "constant")]
     c.suth_C = case.get("suth_C", SUTH_C)
+    c.alpha_mode = ALPHA_MODE[case.get("alpha_mode", "const3")]
+    c.physics = PHYSICS[case.get("physics", "reference")]
     return c
 
 
@@ -171,6 +176,26 @@ class Oracle(_Lib):
         self.lib.nso_eigenvectors(_ptr(Wavg), ctypes.c_double(gamma), ctypes.c_int(direction),
                                   _ptr(L), _ptr(R))
         return L.reshape(5, 5), R.reshape(5, 5)
+
+    def face_eigensystem(self, case, Wl, Wr, direction):
+        """(face state, L, R) of the characteristic sweep between two cells; L and R act on
+        vectors in the stored order [rho, rho E, rho u, rho v, rho w]."""
+        cfg = make_cfg(case)
+        Wl = np.ascontiguousarray(Wl, dtype=np.float64)
+        Wr = np.ascontiguousarray(Wr, dtype=np.float64)
+        Wf, L, R = np.zeros(5), np.zeros(25), np.zeros(25)
+        self.lib.nso_face_eigensystem(ctypes.byref(cfg), _ptr(Wl), _ptr(Wr), ctypes.c_int(direction),
+                                      _ptr(Wf), _ptr(L), _ptr(R))
+        return Wf, L.reshape(5, 5), R.reshape(5, 5)
+
+    def face_alpha(self, case, W, direction, i, j, k):
+        """Lax-Friedrichs constant of face i+1/2 (padded cell indices) along `direction`."""
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        f = self.lib.nso_face_alpha
+        f.restype = ctypes.c_double
+        return f(ctypes.byref(cfg), _ptr(W), ctypes.c_int(direction), ctypes.c_long(i),
+                 ctypes.c_long(j), ctypes.c_long(k))
 
     # ---- learned weights ------------------------------------------------------------------
     def bf16(self, v):

@@ -91,12 +91,19 @@ DiffusionScheme* make_diffusion(const Params& p) {
 
 size_t len(const Array4d& a) { return a.NX() * a.NY() * a.NZ() * a.NQ(); }
 
+// the options the reference's own code path has: constant mu, alpha = 3, its eigen system
+bool reference_semantics(const nso_cfg* c) {
+    return c->viscosity == 0 && c->alpha_mode == NSO_ALPHA_CONST3 && c->physics == 0 &&
+           c->weights != NSO_W_LEARNED;
+}
+
 } // namespace
 
 extern "C" {
 
 // One stage body of TimeScheme::step (time_scheme.cpp:33-41): BC, exchange, c2p, R = conv+diff.
 int ref_rhs(const nso_cfg* c, double* U_io, double* W_out, double* R_out, int which) {
+    if (!reference_semantics(c)) return -1;
     ensure_mpi();
     Params        p = make_params(c);
     Decomposition dec(p);
@@ -125,6 +132,7 @@ int ref_rhs(const nso_cfg* c, double* U_io, double* W_out, double* R_out, int wh
 
 // nsteps x TimeScheme::step (time_scheme.cpp:3-70) from primitive W.
 int ref_step(const nso_cfg* c, double* W_io, double* U_out, int nsteps, double* dts) {
+    if (!reference_semantics(c)) return -1;
     ensure_mpi();
     Params        p = make_params(c);
     Decomposition dec(p);

@@ -45,13 +45,21 @@ def _create(case, **over):
 
 @pytest.mark.parametrize("over", [
     dict(ngc=2), dict(nx=2), dict(dim=1), dict(diff_order=3), dict(conv=3), dict(weights=3),
-    dict(math=5), dict(gamma=1.0), dict(Lx=0.0),
+    dict(math=5), dict(gamma=1.0), dict(Lx=0.0), dict(alpha_mode=2), dict(physics=2),
     dict(bc=[1, 0, 0, 0, 0, 0]),   # x not periodic (West Neumann) but East periodic
     dict(bc=[0, 0, 0, 0, 1, 0]),   # z not periodic (Front Neumann) but Back periodic
 ])
 def test_invalid_configs_rejected(over):
     C = cases_module()
     assert _create(C.base_case(8, 8, 8), **over) == -1
+
+
+def test_learned_weights_with_scheme_options_unsupported():
+    # validated before any device call: runs without a GPU
+    C = cases_module()
+    case = dict(C.base_case(8, 8, 8), weights="learned")
+    assert _create(case, alpha_mode=1) == -7
+    assert _create(case, physics=1) == -7
 
 
 def test_strerror_and_null_args():
