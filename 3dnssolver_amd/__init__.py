@@ -34,7 +34,7 @@ MATH = {"exact": 0, "fast": 1, "fp32": 2}
 STATE = {"conservative": 0, "primitive": 1}
 VISCOSITY = {"constant": 0, "sutherland": 1}
 ALPHA_MODE = {"const3": 0, "local_max": 1}
-PHYSICS = {"reference": 0, "corrected_convection": 1}
+PHYSICS = {"reference": 0, "corrected_convection": 1, "corrected_viscous": 2, "corrected": 3}
 SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 
 EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len", "nsgpu_set_state",

@@ -43,7 +43,8 @@ struct Phys {
     int    conv;        // 0 none, 1 component-wise, 2 characteristic
     int    weights;     // 0 Shu, 1 Borges, 2 learned (network `mlp`)
     int    alpha_mode;  // 0 alpha = 3 (the reference), 1 per-face maximum eigenvalue
-    int    physics;     // bit 0: corrected characteristic projection (true c, order, Roe state)
+    int    physics;     // bit 0: corrected characteristic projection (true c, order, Roe state);
+                        // bit 1: full 3-D viscous terms
     int    diff_order;  // 0, 2, 4, 6
     int    suth;        // Sutherland viscosity
     double suth_C;      // Sutherland constant (T0 = 1)
@@ -143,7 +144,8 @@ struct Launchers {
     void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
     void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);
     // R (interior) = R_in (or 0) + convection(fx,fy,fz) + diffusion(W, Dx, Dy); then either
-    // stores R (st == nullptr) or applies the SSP-RK3 stage update
+    // stores R (st == nullptr) or applies the SSP-RK3 stage update.  With the full 3-D viscous
+    // terms (physics bit 1) Dx is the 4-plane output of visc3d and Dy is unused.
     void (*residual)(const double* fx, const double* fy, const double* fz, const double* W,
                      const double* Dx, const double* Dy, const double* Rin, double* Rout,
                      const StageArgs* st, const unsigned long long* dtmin, int which, Geo g,
@@ -167,6 +169,11 @@ struct Launchers {
                       int* regime, hipStream_t s);
     // overwrite the step's CFL minimum with a prescribed dt / CFL
     void (*set_dt)(unsigned long long* bits, double dt_over_cfl, hipStream_t s);
+    // full 3-D viscous terms (physics bit 1): first derivatives of (u, v, w, p/rho) into Dx, Dy,
+    // Dz (4 planes each) with their periodic halos, then the terms into Rv (energy, x, y, z
+    // momentum), which the residual reads in place of Dx
+    void (*visc3d)(const double* W, double* Dx, double* Dy, double* Dz, double* Rv, Geo g,
+                   Phys ph, Bnd b, hipStream_t s);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

@@ -1191,7 +1191,9 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
 // and the primitives of the new state (eq_of_state.cpp:61-87).  Without a stage (st.Uout null)
 // R is stored instead (operator-level calls).
 // 256 threads as in deriv1_body: (bx, by, bz) = (k tile, j tile, chunk of `march` planes i)
-template <int ORDER, bool SQ, bool SUTH>
+// VC (full 3-D viscous terms): the viscous terms were formed by k_visc3d and Dx holds them
+// (4 planes: energy, x, y, z momentum); Dy is unused
+template <int ORDER, bool SQ, bool SUTH, bool VC = false>
 __device__ __forceinline__ void residual_body(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
@@ -1212,7 +1214,7 @@ __device__ __forceinline__ void residual_body(
     const bool visc = (ORDER > 0) && (which & 2);
     real       wx[4][O + 1], dyx[2][O + 1];
     double     fxm[5];
-    if (visc) {
+    if (visc && !VC) {
 #pragma unroll
         for (int t = 0; t < 2 * H; t++) {
             const long ct = c0 + (long)(i0 + NGC - H + t) * g.sx;
@@ -1271,7 +1273,16 @@ __device__ __forceinline__ void residual_body(
                 have[q] = true;
             }
         }
-        if constexpr (ORDER > 0) {
+        if constexpr (ORDER > 0 && VC) {
+            if (visc) {
+                const double ene = Dx[c], xm = Dx[P + c], ym = Dx[2 * P + c], zm = Dx[3 * P + c];
+                R[1]    = have[1] ? R[1] + ene : ene;
+                R[2]    = have[2] ? R[2] + xm : xm;
+                R[3]    = have[3] ? R[3] + ym : ym;
+                R[4]    = have[4] ? R[4] + zm : zm;
+                have[1] = have[2] = have[3] = have[4] = true;
+            }
+        } else if constexpr (ORDER > 0) {
             if (visc) {
                 const long cn = c + (long)H * g.sx;  // plane entering the window
 #pragma unroll
@@ -1315,14 +1326,184 @@ __device__ __forceinline__ void residual_body(
     }
 }
 
-template <int ORDER, bool SQ, bool SUTH>
+template <int ORDER, bool SQ, bool SUTH, bool VC = false>
 __global__ __launch_bounds__(256) void k_residual(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
     const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
     const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
-    residual_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Dy, Rin, Rout, st, dtmin, which, g, ph,
-                                   blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
+    residual_body<ORDER, SQ, SUTH, VC>(fx, fy, fz, W, Dx, Dy, Rin, Rout, st, dtmin, which, g, ph,
+                                       blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
+}
+
+// ------------------------------------------------------------------------------------------
+// full 3-D viscous terms (physics VISCOUS, extension): CenteredDiffusion::compute_residual
+// (centered_diffusion.cpp:70-182) for all three axes, each second derivative on its own axis,
+// the viscous work tau_ij du_i/dx_j and the heat flux from T = p / rho.  With g[a][d] =
+// du_a/dx_d, div = tr g, lamb = -2/3 mu:
+//   divtau[a] = sum_d dmu_d (g[a][d] + g[d][a]) + mu (lap u_a + d_a div) + dlamb_a div
+//               + lamb d_a div
+//   energy    = -(sum_a u_a divtau[a]) - sum_{a,d} tau[a][d] g[a][d]
+//               - heat sum_d (dmu_d dT/dx_d + mu d2T/dx_d2)
+// Three passes: first derivatives of (u, v, w, T) along x, y, z; their periodic halos across
+// the other two axes (zero beyond a wall, like centered_diffusion.cpp:22-24); the terms per cell
+// (mixed derivatives d_a d_d u_d = df_da(du_d/dx_d) from the derivative planes).
+// ------------------------------------------------------------------------------------------
+
+// u, v, w or T = p / rho (f = 0..3) of the cell at offset o
+__device__ __forceinline__ real visc_field(const double* __restrict__ W, long P, long o, int f) {
+    return (f < 3) ? (real)W[(2 + f) * P + o] : dv((real)W[P + o], (real)W[o]);
+}
+
+// one thread per interior cell: block (64, 4) over (k, j), blockIdx.z = i
+template <int ORDER>
+__global__ __launch_bounds__(256) void k_deriv3d(const double* __restrict__ W,
+                                                 double* __restrict__ Dx, double* __restrict__ Dy,
+                                                 double* __restrict__ Dz, Geo g) {
+    constexpr int H = ORDER / 2;
+    const int     k = threadIdx.x + blockIdx.x * 64, j = threadIdx.y + blockIdx.y * 4;
+    const int     i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long P = g.P;
+    const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        Dx[c + f * P] = fd_sum<ORDER, false>(
+            [&](int t) { return visc_field(W, P, c + (long)(t - H) * g.sx, f); }, g.dx, g.idx);
+        Dy[c + f * P] = fd_sum<ORDER, false>(
+            [&](int t) { return visc_field(W, P, c + (long)(t - H) * g.sy, f); }, g.dy, g.idy);
+        Dz[c + f * P] = fd_sum<ORDER, false>(
+            [&](int t) { return visc_field(W, P, c + (t - H), f); }, g.dz, g.idz);
+    }
+}
+
+// periodic halos of the derivative planes the mixed derivatives read: du/dx across y and z,
+// dv/dy across x and z, dw/dz across x and y.  blockIdx.z = (combination, low/high, layer);
+// block (64, 4) over the face (fast, slow tangential index)
+__global__ void k_deriv3d_halo(double* Dx, double* Dy, double* Dz, Geo g, Bnd b) {
+    const int combo = blockIdx.z / (2 * NGC);
+    const int h     = blockIdx.z % NGC;
+    const bool low  = ((blockIdx.z / NGC) & 1) == 0;
+    int d, a;  // derivative plane set (and its field), exchange axis
+    switch (combo) {
+    case 0: d = 0, a = 1; break;
+    case 1: d = 0, a = 2; break;
+    case 2: d = 1, a = 0; break;
+    case 3: d = 1, a = 2; break;
+    case 4: d = 2, a = 0; break;
+    default: d = 2, a = 1; break;
+    }
+    if (!b.periodic[a]) return;
+    const int  n[3]  = {g.nx, g.ny, g.nz};
+    const long st[3] = {g.sx, g.sy, 1};
+    const int  ta = (a == 0) ? 1 : 0, tb = (a == 2) ? 1 : 2;  // slow, fast tangential axes
+    const int  f = threadIdx.x + blockIdx.x * 64, sl = threadIdx.y + blockIdx.y * 4;
+    if (sl >= n[ta] || f >= n[tb]) return;
+    double*    D    = (d == 0) ? Dx : (d == 1) ? Dy : Dz;
+    const long base = (long)d * g.P + (long)(sl + NGC) * st[ta] + (long)(f + NGC) * st[tb];
+    const int  dst  = low ? h : NGC + n[a] + h;
+    const int  src  = low ? n[a] + h : NGC + h;
+    D[base + (long)dst * st[a]] = D[base + (long)src * st[a]];
+}
+
+template <int ORDER, bool SUTH>
+__global__ __launch_bounds__(256) void k_visc3d(const double* __restrict__ W,
+                                                const double* __restrict__ Dx,
+                                                const double* __restrict__ Dy,
+                                                const double* __restrict__ Dz,
+                                                double* __restrict__ Rv, Geo g, Phys ph) {
+    constexpr int H = ORDER / 2;
+    const int     k = threadIdx.x + blockIdx.x * 64, j = threadIdx.y + blockIdx.y * 4;
+    const int     i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long    P     = g.P;
+    const long    c     = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
+    const long    st[3] = {g.sx, g.sy, 1};
+    const double* D[3]  = {Dx, Dy, Dz};
+    const double  dd[3] = {g.dx, g.dy, g.dz}, idd[3] = {g.idx, g.idy, g.idz};
+    const double  d2[3] = {g.dxx, g.dyy, g.dz * g.dz};
+    const double  id2[3] = {g.idxx, g.idyy, g.idz * g.idz};
+    real gr[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) gr[a][d] = (real)D[d][c + a * P];
+    const real div = (gr[0][0] + gr[1][1]) + gr[2][2];
+    real S[3][4];  // d2 f / dx_d2, f = u, v, w, T
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+            S[d][f] = fd_sum<ORDER, true>(
+                [&](int t) { return visc_field(W, P, c + (long)(t - H) * st[d], f); }, d2[d],
+                id2[d]);
+    real lap[3], ddiv[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lap[a] = (S[0][a] + S[1][a]) + S[2][a];
+        real x[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            if (d == a) {
+                x[d] = S[a][a];
+            } else {
+                const double* Dd = D[d] + (long)d * P + c;
+                x[d] = fd_sum<ORDER, false>([&](int t) { return (real)Dd[(long)(t - H) * st[a]]; },
+                                            dd[a], idd[a]);
+            }
+        }
+        ddiv[a] = (x[0] + x[1]) + x[2];
+    }
+    real mu, dmu[3] = {RL(0.), RL(0.), RL(0.)};
+    if constexpr (SUTH) {
+        mu = mu_suth(W[c], W[P + c], ph);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            dmu[d] = fd_sum<ORDER, false>(
+                [&](int t) {
+                    const long o = c + (long)(t - H) * st[d];
+                    return mu_suth(W[o], W[P + o], ph);
+                },
+                dd[d], idd[d]);
+    } else {
+        mu = (real)ph.mu;
+    }
+    const real lamb = (-RL(2.) / RL(3.)) * mu;
+    real       divtau[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        real t = mu * (lap[a] + ddiv[a]);
+        if constexpr (SUTH) {
+            real m = dmu[0] * (gr[a][0] + gr[0][a]);
+            m += dmu[1] * (gr[a][1] + gr[1][a]);
+            m += dmu[2] * (gr[a][2] + gr[2][a]);
+            t = (m + t) + (-(RL(2.) / RL(3.)) * dmu[a]) * div;
+        }
+        divtau[a] = t + lamb * ddiv[a];
+    }
+    const real u = (real)W[2 * P + c], v = (real)W[3 * P + c], w = (real)W[4 * P + c];
+    const real work = (u * divtau[0] + v * divtau[1]) + w * divtau[2];
+    real       phi  = RL(0.);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            real tau = mu * (gr[a][d] + gr[d][a]);
+            if (a == d) tau = tau + lamb * div;
+            phi = (a == 0 && d == 0) ? tau * gr[a][d] : phi + tau * gr[a][d];
+        }
+    real q[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        real t = mu * S[d][3];
+        if constexpr (SUTH) t = dmu[d] * (real)D[d][c + 3 * P] + t;
+        q[d] = -(real)ph.heat * t;
+    }
+    const real ene = ((-work) - phi) + ((q[0] + q[1]) + q[2]);
+    Rv[c]         = ene;
+    Rv[P + c]     = -divtau[0];
+    Rv[2 * P + c] = -divtau[1];
+    Rv[3 * P + c] = -divtau[2];
 }
 
 __global__ void k_time_accum(double* t, const unsigned long long* dtmin, double cfl) {
@@ -1739,7 +1920,9 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
                            StageArgs a, const unsigned long long* dtmin, int which, Geo g,
                            Phys ph, hipStream_t s) {
     dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
-    if (ph.suth) {
+    if (ORDER > 0 && (ph.physics & 2)) {
+        k_residual<ORDER, false, false, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    } else if (ph.suth) {
         if (g.square_xy)
             k_residual<ORDER, true, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
         else
@@ -1764,6 +1947,25 @@ static void l_residual(const double* fx, const double* fy, const double* fz, con
     case 6: residual_order<6>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
     default: residual_order<0>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
     }
+}
+
+template <int ORDER>
+static void visc3d_order(const double* W, double* Dx, double* Dy, double* Dz, double* Rv, Geo g,
+                         Phys ph, Bnd b, hipStream_t s) {
+    dim3 grd((g.nz + 63) / 64, (g.ny + 3) / 4, g.nx);
+    k_deriv3d<ORDER><<<grd, BLK, 0, s>>>(W, Dx, Dy, Dz, g);
+    int  m2 = g.nz > g.ny ? g.nz : g.ny, m1 = g.ny > g.nx ? g.ny : g.nx;
+    dim3 hg((m2 + 63) / 64, (m1 + 3) / 4, 6 * 2 * NGC);
+    k_deriv3d_halo<<<hg, BLK, 0, s>>>(Dx, Dy, Dz, g, b);
+    if (ph.suth) k_visc3d<ORDER, true><<<grd, BLK, 0, s>>>(W, Dx, Dy, Dz, Rv, g, ph);
+    else k_visc3d<ORDER, false><<<grd, BLK, 0, s>>>(W, Dx, Dy, Dz, Rv, g, ph);
+}
+
+static void l_visc3d(const double* W, double* Dx, double* Dy, double* Dz, double* Rv, Geo g,
+                     Phys ph, Bnd b, hipStream_t s) {
+    if (ph.diff_order == 2) visc3d_order<2>(W, Dx, Dy, Dz, Rv, g, ph, b, s);
+    else if (ph.diff_order == 4) visc3d_order<4>(W, Dx, Dy, Dz, Rv, g, ph, b, s);
+    else visc3d_order<6>(W, Dx, Dy, Dz, Rv, g, ph, b, s);
 }
 
 static void l_aos_to_soa(const double* aos, double* soa, Geo g, hipStream_t s) {
@@ -1871,7 +2073,7 @@ const Launchers& launchers() {
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
                                 l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
-                                l_mlp_recon,    l_set_dt};
+                                l_mlp_recon,    l_set_dt,     l_visc3d};
     return L;
 }
 

@@ -50,6 +50,8 @@ struct nsgpu_ctx {
     double*            F[3] = {nullptr, nullptr, nullptr};  // face fluxes per direction
     double*            Dx = nullptr;  // first derivatives of rho, p, u, v (4 planes each)
     double*            Dy = nullptr;
+    double*            Dz = nullptr;  // full 3-D viscous terms: d/dz of u, v, w, p/rho
+    double*            Rv = nullptr;  // full 3-D viscous terms per cell (4 planes)
     double*            R  = nullptr;  // residual (operator-level calls)
     double*            stage_aos = nullptr;  // host-layout staging (5 planes)
     double*            partial = nullptr;    // reduction partials
@@ -117,7 +119,8 @@ int validate(const nsgpu_config* c) {
     if (c->viscosity == NSGPU_VISC_SUTHERLAND && !(c->suth_C > 0)) return NSGPU_ERR_INVALID;
     if (c->alpha_mode != NSGPU_ALPHA_CONST3 && c->alpha_mode != NSGPU_ALPHA_LOCAL_MAX)
         return NSGPU_ERR_INVALID;
-    if (c->physics & ~NSGPU_PHYS_CONVECTION) return NSGPU_ERR_INVALID;
+    if (c->physics & ~(NSGPU_PHYS_CONVECTION | NSGPU_PHYS_VISCOUS)) return NSGPU_ERR_INVALID;
+    if ((c->physics & NSGPU_PHYS_VISCOUS) && c->dim != 3) return NSGPU_ERR_INVALID;
     if (c->weights == NSGPU_WEIGHTS_LEARNED && (c->alpha_mode || c->physics))
         return NSGPU_ERR_UNSUPPORTED;
     // periodicity per axis from West / South / Front (decomposition.cpp:145-161); every face of a
@@ -179,8 +182,18 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W) {
 void diffusion_derivs(nsgpu_ctx* c, const double* W) {
     if (!c->ph.diff_order) return;
     Scope s(c, T_DERIV);
+    if (c->ph.physics & NSGPU_PHYS_VISCOUS) {
+        c->L->visc3d(W, c->Dx, c->Dy, c->Dz, c->Rv, c->g, c->ph, c->b, c->stream);
+        return;
+    }
     c->L->deriv1(W, c->Dx, c->Dy, c->g, c->ph, c->stream);
     c->L->deriv_halo(c->Dx, c->Dy, c->g, c->b, c->stream);
+}
+
+// what the residual reads as its viscous input: the derivative planes, or the precomputed full
+// 3-D viscous terms
+const double* visc_in(const nsgpu_ctx* c) {
+    return (c->ph.physics & NSGPU_PHYS_VISCOUS) ? c->Rv : c->Dx;
 }
 
 }  // namespace
@@ -259,6 +272,10 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     for (int d = 0; d < 3 && !rc; d++) rc = alloc(&c->F[d], five, c->stream);
     if (!rc) rc = alloc(&c->Dx, four, c->stream);
     if (!rc) rc = alloc(&c->Dy, four, c->stream);
+    if (!rc && (cfg->physics & NSGPU_PHYS_VISCOUS)) {
+        rc = alloc(&c->Dz, four, c->stream);
+        if (!rc) rc = alloc(&c->Rv, four, c->stream);
+    }
     if (!rc) rc = alloc(&c->R, five, c->stream);
     if (!rc) rc = alloc(&c->stage_aos, five, c->stream);
     c->partial_max = 4096;
@@ -288,7 +305,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     double* bufs[] = {c->W,  c->W2, c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub, c->F[0], c->F[1], c->F[2], c->Dx, c->Dy,
-                      c->R,  c->stage_aos, c->partial, (double*)c->dtmin, (double*)c->flags,
+                      c->Dz, c->Rv, c->R,  c->stage_aos, c->partial, (double*)c->dtmin, (double*)c->flags,
                       c->dtime, c->sums, (double*)c->mlp_params, (double*)c->mlp_packed};
     for (double* p : bufs)
         if (p) hipFree(p);
@@ -346,7 +363,7 @@ int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_ao
     convection_fluxes(c, c->Ub, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, c->Dx, c->Dy, c->R, c->R, nullptr,
+        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, c->R, c->R, nullptr,
                        nullptr, NSGPU_RHS_CONVECTION, c->g, c->ph, c->stream);
     }
     return download_soa(c, c->R, R_aos);
@@ -361,7 +378,7 @@ int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
     diffusion_derivs(c, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, c->Dx, c->Dy, c->R, c->R, nullptr,
+        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, c->R, c->R, nullptr,
                        nullptr, NSGPU_RHS_DIFFUSION, c->g, c->ph, c->stream);
     }
     return download_soa(c, c->R, R_aos);
@@ -384,7 +401,7 @@ int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int whi
     if (which & 2) diffusion_derivs(c, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, c->Dx, c->Dy, nullptr, c->R, nullptr,
+        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, nullptr, c->R, nullptr,
                        nullptr, which, c->g, c->ph, c->stream);
     }
     rc = download_soa(c, c->Ua, U_aos);
@@ -439,7 +456,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st};
             {
                 Scope sc(c, T_RESIDUAL);
-                L.residual(c->F[0], c->F[1], c->F[2], wcur, c->Dx, c->Dy, nullptr, nullptr, &a,
+                L.residual(c->F[0], c->F[1], c->F[2], wcur, visc_in(c), c->Dy, nullptr, nullptr, &a,
                            &c->dtmin[0], NSGPU_RHS_BOTH, c->g, c->ph, c->stream);
             }
             double* t = cur;
@@ -778,6 +795,8 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
         for (int a = 0; a < b; a++)
             if (members[a] == c) return NSGPU_ERR_INVALID;
     }
+    // the batched kernels carry the reference's viscous terms only
+    if (c0->ph.physics & NSGPU_PHYS_VISCOUS) return NSGPU_ERR_UNSUPPORTED;
     HIPCHK(hipSetDevice(c0->device));
     nsgpu_ensemble* e = new (std::nothrow) nsgpu_ensemble();
     if (!e) return NSGPU_ERR_INVALID;

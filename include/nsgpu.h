@@ -85,8 +85,17 @@ enum { NSGPU_ALPHA_CONST3 = 0, NSGPU_ALPHA_LOCAL_MAX = 1 };
  *               L and R in the order of the stored conserved vector (the reference builds them
  *               for [rho, rho u, rho v, rho w, rho E] and applies them to [rho, rho E, rho u,
  *               rho v, rho w], weno-5cw.cpp:92-93), and the Roe-averaged face state instead of
- *               the mean of the primitives (weno-5cw.cpp:67-70) */
-enum { NSGPU_PHYS_REFERENCE = 0, NSGPU_PHYS_CONVECTION = 1 };
+ *               the mean of the primitives (weno-5cw.cpp:67-70)
+ *   VISCOUS     the full 3-D viscous and heat-flux terms (3-D blocks only): what
+ *               CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) computes for x
+ *               and y, for all three axes, with each second derivative on its own axis
+ *               (cd-n.cpp:74-86 runs the x stencil), Tauyy from dv/dy (:143), the viscous work
+ *               tau_ij du_i/dx_j (:147-148 multiply Tauxy by dv/dy), and the heat flux from the
+ *               derivatives of T = p / rho (:161-166 use the value 1 for p in the y term and
+ *               +2 / 1 where the expansion of d2(p/rho) has -2 / 2).  Not in ensembles
+ *               (NSGPU_ERR_UNSUPPORTED) */
+enum { NSGPU_PHYS_REFERENCE = 0, NSGPU_PHYS_CONVECTION = 1, NSGPU_PHYS_VISCOUS = 2,
+       NSGPU_PHYS_CORRECTED = 3 };
 enum { NSGPU_RHS_CONVECTION = 1, NSGPU_RHS_DIFFUSION = 2, NSGPU_RHS_BOTH = 3 };
 
 typedef struct {

@@ -827,6 +827,130 @@ static void diffusion(const nso_cfg* c, const double* W, double* R) {
     free(MU), free(DMUX), free(DMUY);
 }
 
+/* Full 3-D viscous terms (physics VISCOUS): what CenteredDiffusion::compute_residual
+ * (centered_diffusion.cpp:70-182) computes in 2-D, for all three axes and without the quirks of
+ * the reference's expressions (each second derivative on its own axis, Tauyy from dv/dy, the
+ * viscous work from tau_ij du_i/dx_j, the heat flux from the temperature p/rho with its true
+ * derivatives).  Expanded like the reference's: with g[a][d] = du_a/dx_d, div = tr g,
+ * lamb = -2/3 mu,
+ *   tau[a][d]  = mu (g[a][d] + g[d][a]) + [a == d] lamb div
+ *   divtau[a]  = sum_d dmu_d (g[a][d] + g[d][a]) + mu (lap u_a + d_a div)
+ *                + dlamb_a div + lamb d_a div
+ *   energy     = -(sum_a u_a divtau[a]) - sum_{a,d} tau[a][d] g[a][d] + sum_d dq_d/dx_d,
+ *                dq_d/dx_d = -heat (dmu_d dT/dx_d + mu d2T/dx_d2),  T = p / rho
+ * Derivative arrays: first derivatives of (u, v, w, T) on the interior, exchanged like
+ * centered_diffusion.cpp:22-24 (periodic wrap, zero beyond a wall), pure second derivatives on
+ * their own axis, mixed second derivatives d_a d_d u_d = df_da(du_d/dx_d). */
+static void visc3d_residual(const nso_cfg* c, const double* W, const double* G, const double* D[3],
+                            const double* S[3], const double* X[3][3], const double* MU,
+                            const double* DMU[3], double* R) {
+    pad_t        P    = pad_of(c);
+    const double heat = c->gamma / (c->Pr * (c->gamma - 1.0));
+#pragma omp parallel for schedule(static)
+    for (long i = NGC; i < NGC + c->nx; i++)
+        for (long j = NGC; j < NGC + c->ny; j++)
+            for (long k = NGC; k < NGC + c->nz; k++) {
+                size_t b  = ((((size_t)i) * P.NY + j) * P.NZ + k) * NQ;
+                double mu = MU ? MU[b] : c->mu;
+                double dmu[3] = {0.0, 0.0, 0.0};
+                if (MU)
+                    for (int d = 0; d < 3; d++) dmu[d] = DMU[d][b];
+                double lamb = (-2. / 3.) * mu;
+                double g[3][3], lap[3], ddiv[3];
+                for (int a = 0; a < 3; a++)
+                    for (int d = 0; d < 3; d++) g[a][d] = D[d][b + a];
+                double div = (g[0][0] + g[1][1]) + g[2][2];
+                for (int a = 0; a < 3; a++) {
+                    lap[a]  = (S[0][b + a] + S[1][b + a]) + S[2][b + a];
+                    double x[3];
+                    for (int d = 0; d < 3; d++) x[d] = (d == a) ? S[a][b + a] : X[a][d][b + d];
+                    ddiv[a] = (x[0] + x[1]) + x[2];
+                }
+                double divtau[3];
+                for (int a = 0; a < 3; a++) {
+                    double t = mu * (lap[a] + ddiv[a]);
+                    if (MU) {
+                        double m = dmu[0] * (g[a][0] + g[0][a]);
+                        m += dmu[1] * (g[a][1] + g[1][a]);
+                        m += dmu[2] * (g[a][2] + g[2][a]);
+                        t = (m + t) + (-(2. / 3.) * dmu[a]) * div;
+                    }
+                    divtau[a] = t + lamb * ddiv[a];
+                }
+                const double* w    = &W[b];
+                double        work = (w[2] * divtau[0] + w[3] * divtau[1]) + w[4] * divtau[2];
+                double        phi  = 0.0;
+                for (int a = 0; a < 3; a++)
+                    for (int d = 0; d < 3; d++) {
+                        double tau = mu * (g[a][d] + g[d][a]);
+                        if (a == d) tau = tau + lamb * div;
+                        phi = (a == 0 && d == 0) ? tau * g[a][d] : phi + tau * g[a][d];
+                    }
+                double q[3];
+                for (int d = 0; d < 3; d++) {
+                    double Tg = D[d][b + 3], Tdd = S[d][b + 3];
+                    double t  = mu * Tdd;
+                    if (MU) t = dmu[d] * Tg + t;
+                    q[d] = -heat * t;
+                }
+                double ene = ((-work) - phi) + ((q[0] + q[1]) + q[2]);
+                R[b + 1] += ene;
+                R[b + 2] += -divtau[0];
+                R[b + 3] += -divtau[1];
+                R[b + 4] += -divtau[2];
+            }
+    (void)G;
+}
+
+static void diffusion_3d(const nso_cfg* c, const double* W, double* R) {
+    pad_t  P     = pad_of(c);
+    double dd[3] = {c->Lx / c->nx, c->Ly / c->ny, c->Lz / c->nz};
+    /* G = (u, v, w, p / rho) wherever W is defined (interior and face halos) */
+    double* G = alloc_zero(c);
+    for (size_t a = 0; a < nso_array_len(c); a += NQ) {
+        G[a + 0] = W[a + 2], G[a + 1] = W[a + 3], G[a + 2] = W[a + 4];
+        G[a + 3] = (W[a] != 0.0) ? W[a + 1] / W[a] : 0.0;
+    }
+    double *D[3], *S[3], *X[3][3] = {{0}}, *MU = NULL, *DMU[3] = {NULL, NULL, NULL};
+    for (int d = 0; d < 3; d++) {
+        D[d] = alloc_zero(c);
+        S[d] = alloc_zero(c);
+        nso_fd(c, G, D[d], d, 0, dd[d]);
+        nso_fd(c, G, S[d], d, 1, dd[d] * dd[d]);
+        communicate(c, D[d]);
+    }
+    /* X[a][d] = df_da(dG/dx_d), a != d */
+    for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++)
+            if (a != d) {
+                X[a][d] = alloc_zero(c);
+                nso_fd(c, D[d], X[a][d], a, 0, dd[a]);
+            }
+    if (c->viscosity == 1) {
+        MU = alloc_zero(c);
+        for (long i = 0; i < P.NX; i++)
+            for (long j = 0; j < P.NY; j++)
+                for (long k = 0; k < P.NZ; k++) {
+                    int out = (i < NGC || i >= NGC + c->nx) + (j < NGC || j >= NGC + c->ny) +
+                              (k < NGC || k >= NGC + c->nz);
+                    if (out > 1) continue;
+                    const double* w       = &AT(W, P, i, j, k, 0);
+                    AT(MU, P, i, j, k, 0) = nso_mu_sutherland(c->mu, c->suth_C, c->gamma, w[0], w[1]);
+                }
+        for (int d = 0; d < 3; d++) {
+            DMU[d] = alloc_zero(c);
+            nso_fd(c, MU, DMU[d], d, 0, dd[d]);
+        }
+    }
+    visc3d_residual(c, W, G, (const double**)D, (const double**)S,
+                    (const double* (*)[3])X, MU, (const double**)DMU, R);
+    free(G), free(MU);
+    for (int d = 0; d < 3; d++) {
+        free(D[d]), free(S[d]), free(DMU[d]);
+        for (int a = 0; a < 3; a++) free(X[a][d]);
+    }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* RHS and time stepping                                                                      */
 /* ------------------------------------------------------------------------------------------ */
@@ -840,7 +964,8 @@ static int cfg_ok(const nso_cfg* c) {
     if (c->nx < 1 || c->ny < 1 || c->nz < 1) return 0;
     if (c->conv < 0 || c->conv > 2) return 0;
     if (c->alpha_mode < 0 || c->alpha_mode > 1) return 0;
-    if (c->physics & ~NSO_PHYS_CONVECTION) return 0;
+    if (c->physics & ~(NSO_PHYS_CONVECTION | NSO_PHYS_VISCOUS)) return 0;
+    if ((c->physics & NSO_PHYS_VISCOUS) && c->dim != 3) return 0;
     if (c->diff_order != 0 && c->diff_order != 2 && c->diff_order != 4 && c->diff_order != 6)
         return 0;
     return 1;
@@ -852,7 +977,10 @@ int nso_rhs(const nso_cfg* c, double* U, double* W, double* R, int which) {
     nso_c2p(c, U, W);
     memset(R, 0, nso_array_len(c) * sizeof(double));
     if ((which & 1) && c->conv != NSO_CONV_NONE) convection(c, W, U, R);
-    if ((which & 2) && c->diff_order != 0) diffusion(c, W, R);
+    if ((which & 2) && c->diff_order != 0) {
+        if (c->physics & NSO_PHYS_VISCOUS) diffusion_3d(c, W, R);
+        else diffusion(c, W, R);
+    }
     return 0;
 }
 

@@ -53,7 +53,9 @@ enum { NSO_ALPHA_CONST3 = 0, NSO_ALPHA_LOCAL_MAX = 1 };
  * order of the conserved vector they act on (the reference builds them for
  * [rho, rho u, rho v, rho w, rho E] and applies them to [rho, rho E, rho u, rho v, rho w]), and the
  * Roe-averaged face state instead of the mean of the primitives (weno-5cw.cpp:67-70). */
-enum { NSO_PHYS_CONVECTION = 1 };
+/* VISCOUS: the full 3-D viscous and heat-flux terms instead of the reference's 2-D subset with
+ * its quirks (centered_diffusion.cpp:70-182); 3-D blocks only. */
+enum { NSO_PHYS_CONVECTION = 1, NSO_PHYS_VISCOUS = 2 };
 
 /* the face state, L and R that the characteristic sweep applies at the face between the cells
  * with primitives Wl and Wr, L and R acting on vectors in the stored order [rho, rho E, rho u,

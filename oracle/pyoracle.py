@@ -23,7 +23,7 @@ REGIMES = {"smooth": 0, "mlp": 1, "flagged": 2}
 VISCOSITY = {"constant": 0, "sutherland": 1}
 SUTH_C = 110.4 / 300.0  # Sutherland constant of air over a 300 K reference temperature
 ALPHA_MODE = {"const3": 0, "local_max": 1}
-PHYSICS = {"reference": 0, "corrected_convection": 1}
+PHYSICS = {"reference": 0, "corrected_convection": 1, "corrected_viscous": 2, "corrected": 3}
 
 
 class NsoCfg(ctypes.Structure):

@@ -45,7 +45,8 @@ def _create(case, **over):
 
 @pytest.mark.parametrize("over", [
     dict(ngc=2), dict(nx=2), dict(dim=1), dict(diff_order=3), dict(conv=3), dict(weights=3),
-    dict(math=5), dict(gamma=1.0), dict(Lx=0.0), dict(alpha_mode=2), dict(physics=2),
+    dict(math=5), dict(gamma=1.0), dict(Lx=0.0), dict(alpha_mode=2), dict(physics=4),
+    dict(physics=2, dim=2),  # the full 3-D viscous terms need a 3-D block
     dict(bc=[1, 0, 0, 0, 0, 0]),   # x not periodic (West Neumann) but East periodic
     dict(bc=[0, 0, 0, 0, 1, 0]),   # z not periodic (Front Neumann) but Back periodic
 ])

@@ -1,6 +1,7 @@
 """GPU parity of the scheme options without a reference counterpart: the per-face
-Lax-Friedrichs constant (alpha_mode "local_max") and the corrected characteristic projection
-(physics "corrected_convection").  The oracle is pinned for both in test_options_oracle.py;
+Lax-Friedrichs constant (alpha_mode "local_max"), the corrected characteristic projection
+(physics "corrected_convection") and the full 3-D viscous terms (physics "corrected_viscous").
+The oracle is pinned for all three in test_options_oracle.py;
 here the kernels must match it bitwise in EXACT math (RHS and whole SSP-RK3 steps, tile and
 last-plane kernels, walls and periodic axes), to round-off in FAST / FP32, and ensemble members
 must equal the same blocks stepped alone.
@@ -147,3 +148,57 @@ def test_local_alpha_runs_unit_sound_speed_shear_layer(ns, oracle):
     _, _, flags, rc = s.reduce()
     assert rc == 0 and flags == 0
     s.close()
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("viscosity", ["constant", "sutherland"])
+@pytest.mark.parametrize("bc", [["periodic"] * 6,
+                                ["neumann", "neumann", "periodic", "periodic", "neumann", "neumann"]])
+def test_full_viscous_rhs_matches_oracle_bitwise(ns, oracle, order, viscosity, bc):
+    C = cases_module()
+    case = option_case("weno5cw", "const3", "corrected_viscous", bc=bc, order=order)
+    case["mu"] = 0.02
+    if viscosity == "sutherland":
+        case = C.sutherland(case, 0.02, 1.0)
+    W = C.random_smooth_W(case, 44)
+    U = oracle.p2c(case, W)
+    print(f"[gpu] full viscous rhs cd{order} {viscosity} {bc[0]}/{bc[2]}", flush=True)
+    s = ns.Solver(case, math="exact")
+    for which in (2, 3):
+        _, _, R = s.rhs(U, which)
+        _, _, Ro = oracle.rhs(case, U, which)
+        np.testing.assert_array_equal(po.interior(R), po.interior(Ro))
+    s.close()
+
+
+@pytest.mark.parametrize("math,tol", [("exact", 0.0), ("fast", 1e-10), ("fp32", 1e-4)])
+def test_corrected_physics_steps_match_oracle(ns, oracle, math, tol):
+    # every correction and the per-face constant together, walls on x and z
+    C = cases_module()
+    case = option_case("weno5cw", "local_max", "corrected",
+                       bc=["neumann", "neumann", "periodic", "periodic", "neumann", "neumann"])
+    case["mu"] = 0.01
+    W0 = C.random_smooth_W(case, 45)
+    print(f"[gpu] corrected physics steps {math}", flush=True)
+    Wo, _, dto = oracle.step(case, W0, 5)
+    s = ns.Solver(case, math=math)
+    s.set_state(W0, "primitive")
+    s.step(5)
+    Wg = po.interior(s.get_state("primitive"))
+    if tol == 0.0:
+        np.testing.assert_array_equal(Wg, po.interior(Wo))
+    else:
+        e = rel_err(Wg, po.interior(Wo))
+        assert e.max() < tol, e
+    s.close()
+
+
+def test_full_viscous_refused_by_ensembles(ns):
+    C = cases_module()
+    case = option_case("weno5", "const3", "corrected_viscous")
+    a = ns.Solver(case, math="exact")
+    b = ns.Solver(case, math="exact")
+    with pytest.raises(ns.NsgpuError):
+        ns.Ensemble([a, b])
+    a.close()
+    b.close()

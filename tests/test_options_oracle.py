@@ -239,3 +239,113 @@ def test_local_alpha_stabilises_unit_sound_speed_shear_layer(oracle):
     U0, U1 = oracle.p2c(case, W0), oracle.p2c(case, Wl)
     np.testing.assert_allclose(po.interior(U1)[..., 0].sum(), po.interior(U0)[..., 0].sum(),
                                rtol=1e-12)
+
+
+# --- full 3-D viscous terms (physics "corrected_viscous") ------------------------------------
+
+def smooth_periodic_W(case):
+    """Band-limited 3-D fields on the periodic block (cell centres), halos left to the oracle."""
+    C = cases_module()
+    n = (case["nx"], case["ny"], case["nz"])
+    L = case["L"]
+    x, y, z = (2 * np.pi * (np.arange(n[a]) + 0.5) / n[a] for a in range(3))
+    X, Y, Z = np.meshgrid(x, y, z, indexing="ij")
+    W = np.zeros(po.padded_shape(case))
+    I = (slice(3, -3),) * 3
+    W[I + (0,)] = 1.0 + 0.2 * np.sin(X + Y) + 0.1 * np.cos(Z - X)
+    W[I + (1,)] = 1.5 + 0.3 * np.cos(Y + 2 * Z) + 0.1 * np.sin(X)
+    W[I + (2,)] = 0.3 * np.sin(Y) * np.cos(Z) + 0.1 * np.cos(X + Z)
+    W[I + (3,)] = 0.2 * np.cos(X) * np.sin(Z) - 0.1 * np.sin(2 * Y)
+    W[I + (4,)] = 0.25 * np.sin(X - Y) + 0.1 * np.cos(Z)
+    return W
+
+
+def spectral_viscous(case, W):
+    """The viscous and heat-flux terms in conservative form, every derivative spectral (exact
+    for these resolved periodic fields up to round-off): the independent statement of what the
+    corrected operator discretises.  Returns R (interior, stored order) with the reference's
+    sign convention (U_new = U - dt R)."""
+    n = (case["nx"], case["ny"], case["nz"])
+    L = case["L"]
+    g, Pr = case["gamma"], case["Pr"]
+    I = (slice(3, -3),) * 3
+    rho, p = W[I + (0,)], W[I + (1,)]
+    vel = [W[I + (2 + a,)] for a in range(3)]
+    k = [np.fft.fftfreq(n[a], L[a] / n[a]) * 2 * np.pi for a in range(3)]
+    K = np.meshgrid(*k, indexing="ij")
+
+    def d(f, a):
+        return np.real(np.fft.ifftn(1j * K[a] * np.fft.fftn(f)))
+
+    T = p / rho
+    if case.get("viscosity", "constant") == "sutherland":
+        Tt = g * p / rho
+        C = case["suth_C"]
+        mu = case["mu"] * (1 + C) / (Tt + C) * Tt ** 1.5
+    else:
+        mu = case["mu"] * np.ones_like(rho)
+    grad = [[d(vel[a], b) for b in range(3)] for a in range(3)]
+    div = grad[0][0] + grad[1][1] + grad[2][2]
+    tau = [[mu * (grad[a][b] + grad[b][a]) + (a == b) * (-2.0 / 3.0) * mu * div for b in range(3)]
+           for a in range(3)]
+    heat = g / (Pr * (g - 1.0))
+    R = np.zeros(rho.shape + (5,))
+    for a in range(3):
+        R[..., 2 + a] = -sum(d(tau[a][b], b) for b in range(3))
+    work = sum(d(sum(vel[a] * tau[a][b] for a in range(3)), b) for b in range(3))
+    qdiv = sum(d(-heat * mu * d(T, b), b) for b in range(3))
+    R[..., 1] = -work + qdiv
+    return R
+
+
+def viscous_case(n, order, viscosity="constant", physics="corrected_viscous"):
+    C = cases_module()
+    case = dict(C.base_case(*n), L=(1.0, 0.9, 0.8), conv="none", weights="shu",
+                diff_order=order, mu=0.05, physics=physics)
+    if viscosity == "sutherland":
+        case = C.sutherland(case, 0.05, 1.4 * 1.5)
+    return case
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+@pytest.mark.parametrize("viscosity", ["constant", "sutherland"])
+def test_corrected_viscous_converges_to_spectral_operator(oracle, order, viscosity):
+    # per component (energy, three momenta): max error over max value; halving the cells
+    # divides it by 2^order (measured 3.9-4.0, 15.1-15.9, 51-63 for CD2, CD4, CD6)
+    errs = []
+    for n in ((24, 20, 28), (48, 40, 56)):
+        case = viscous_case(n, order, viscosity)
+        W = smooth_periodic_W(case)
+        _, _, R = oracle.rhs(case, oracle.p2c(case, W), which=2)
+        ref = spectral_viscous(case, W)
+        d = np.abs(po.interior(R) - ref)
+        errs.append(np.array([d[..., q].max() / np.abs(ref[..., q]).max() for q in range(1, 5)]))
+    ratio = errs[0] / errs[1]
+    assert (ratio > 0.75 * 2 ** order).all(), (errs, ratio)
+    assert errs[1].max() < {2: 8e-3, 4: 1.5e-4, 6: 5e-6}[order], errs
+    # mass gets no viscous term
+    assert np.abs(po.interior(R)[..., 0]).max() == 0.0
+
+
+def test_reference_viscous_terms_do_not_converge_in_3d(oracle):
+    # the reference's 2-D subset with its quirks misses the 3-D operator by O(1): the test above
+    # has teeth
+    errs = []
+    for n in ((12, 10, 14), (24, 20, 28)):
+        case = viscous_case(n, 4, physics="reference")
+        W = smooth_periodic_W(case)
+        _, _, R = oracle.rhs(case, oracle.p2c(case, W), which=2)
+        ref = spectral_viscous(case, W)
+        errs.append(np.abs(po.interior(R) - ref)[..., 1:].max() / np.abs(ref[..., 1:]).max())
+    assert min(errs) > 0.1, errs
+
+
+def test_corrected_viscous_conserves_momentum_on_periodic_block(oracle):
+    # the momentum terms are divergences of tau: their sum over a periodic block is round-off
+    # for the exactly differentiated part; the expanded discrete form leaves an O(h^order) sum
+    case = viscous_case((24, 20, 28), 4)
+    W = smooth_periodic_W(case)
+    _, _, R = oracle.rhs(case, oracle.p2c(case, W), which=2)
+    Ri = po.interior(R)
+    for a in range(3):
+        assert abs(Ri[..., 2 + a].sum()) <= 1e-4 * np.abs(Ri[..., 2 + a]).sum()
