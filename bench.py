@@ -292,6 +292,10 @@ def main():
     ap.add_argument("--viscosity", choices=["constant", "sutherland"], default="sutherland",
                     help="the Taylor-Green workload's Sutherland law, or constant mu (the only "
                          "law the reference's code path calls)")
+    ap.add_argument("--alpha-mode", choices=["const3", "local_max"], default="const3",
+                    help="Lax-Friedrichs constant: the reference's 3.0, or per-face eigenvalue")
+    ap.add_argument("--physics", choices=["reference", "corrected_convection", "corrected_viscous",
+                                          "corrected"], default="reference")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
@@ -320,6 +324,7 @@ def main():
     C = importlib.import_module("3dnssolver_amd.cases")
     case, W0 = C.config2(args.n, viscosity=args.viscosity)
     case["conv"] = args.conv
+    case["alpha_mode"], case["physics"] = args.alpha_mode, args.physics
     n = args.n
     print(f"[bench] {n}^3 {args.conv} cd4 math={args.math}: setup", file=sys.stderr, flush=True)
     s = ns.Solver(case, math=args.math)
@@ -386,6 +391,7 @@ def main():
                         f"WENO-5 (Shu, LF split) + CD4 viscous ({args.viscosity} mu), SSP-RK3, "
                         f"CFL 0.5",
             "grid": [n, n, n], "math": args.math, "viscosity": args.viscosity,
+            "alpha_mode": args.alpha_mode, "physics": args.physics,
             "state_resident_in_hbm": True,
         },
         "roofline": {
