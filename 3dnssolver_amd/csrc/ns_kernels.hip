@@ -988,7 +988,20 @@ __device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
 // Kernels with x-direction stencils march along i: each thread owns one (j, k) column of a
 // MARCH-plane chunk and keeps the x-window of the values it needs in registers, so every plane is
 // fetched from HBM once instead of once per stencil point.
-constexpr int MARCH = 16;
+#ifndef NS_MARCH
+#define NS_MARCH 16
+#endif
+#ifndef NS_KW_CELL
+#define NS_KW_CELL 64
+#endif
+#ifndef NS_CELL_WAVES
+#define NS_CELL_WAVES 1  // minimum waves per SIMD requested for the marching cell kernels
+#endif
+#ifndef NS_XCD_CELL
+#define NS_XCD_CELL 1
+#endif
+constexpr int MARCH   = NS_MARCH;
+constexpr int KW_CELL = NS_KW_CELL;  // lanes along k of the single-block cell kernels
 
 // first derivatives of rho, p, u, v along x and y on the interior.  256 threads: kw lanes along
 // k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i)
@@ -1027,11 +1040,31 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
     }
 }
 
+// tile of a single-block cell kernel.  With NS_XCD_CELL the workgroups, which the hardware
+// hands round-robin to the 8 XCDs, are renumbered so that XCD x takes the contiguous tile range
+// [x N/8, (x+1) N/8): neighbouring j tiles, which read each other's halo rows, then run on one
+// XCD and share its L2
+__device__ __forceinline__ void cell_tile(int& bx, int& by, int& bz) {
+#if NS_XCD_CELL
+    const unsigned gx = gridDim.x, gy = gridDim.y, n = gx * gy * gridDim.z;
+    unsigned       t  = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (n % 8 == 0) t = (t % 8) * (n / 8) + t / 8;
+    bx = (int)(t % gx);
+    t /= gx;
+    by = (int)(t % gy);
+    bz = (int)(t / gy);
+#else
+    bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+#endif
+}
+
 template <int ORDER>
-__global__ __launch_bounds__(256) void k_deriv1(const double* __restrict__ W,
+__global__ __launch_bounds__(256, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
                                                 Geo g) {
-    deriv1_body<ORDER>(W, Dx, Dy, g, blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
+    int bx, by, bz;
+    cell_tile(bx, by, bz);
+    deriv1_body<ORDER>(W, Dx, Dy, g, bx, by, bz, KW_CELL, MARCH);
 }
 
 // periodic exchange of the derivative halos that the cross derivatives read: the j halos of
@@ -1327,13 +1360,15 @@ __device__ __forceinline__ void residual_body(
 }
 
 template <int ORDER, bool SQ, bool SUTH, bool VC = false>
-__global__ __launch_bounds__(256) void k_residual(
+__global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
     const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
     const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
+    int bx, by, bz;
+    cell_tile(bx, by, bz);
     residual_body<ORDER, SQ, SUTH, VC>(fx, fy, fz, W, Dx, Dy, Rin, Rout, st, dtmin, which, g, ph,
-                                       blockIdx.x, blockIdx.y, blockIdx.z, 64, MARCH);
+                                       bx, by, bz, KW_CELL, MARCH);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1901,7 +1936,8 @@ static void l_face_flux(int dir, const double* U, const double* W, double* f, Ge
 }
 
 static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {
-    dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
+    dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
+             (g.nx + MARCH - 1) / MARCH);
     if (ph.diff_order == 2) k_deriv1<2><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
     else if (ph.diff_order == 4) k_deriv1<4><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
     else k_deriv1<6><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
@@ -1919,7 +1955,8 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
                            const double* Dx, const double* Dy, const double* Rin, double* Rout,
                            StageArgs a, const unsigned long long* dtmin, int which, Geo g,
                            Phys ph, hipStream_t s) {
-    dim3 grd = cell_grid(g.nz, g.ny, (g.nx + MARCH - 1) / MARCH, BLK);
+    dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
+             (g.nx + MARCH - 1) / MARCH);
     if (ORDER > 0 && (ph.physics & 2)) {
         k_residual<ORDER, false, false, true><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
     } else if (ph.suth) {
