@@ -174,6 +174,8 @@ struct Launchers {
     // momentum), which the residual reads in place of Dx
     void (*visc3d)(const double* W, double* Dx, double* Dy, double* Dz, double* Rv, Geo g,
                    Phys ph, Bnd b, hipStream_t s);
+    // U = p2c(W) on the interior and the face halos
+    void (*p2c_faces)(const double* W, double* U, Geo g, Phys ph, hipStream_t s);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

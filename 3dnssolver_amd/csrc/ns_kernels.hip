@@ -948,6 +948,27 @@ __global__ __launch_bounds__(256) void k_c2p_faces(const double* __restrict__ U,
     for (int q = 0; q < 5; q++) W[c + q * g.P] = w[q];
 }
 
+// U = p2c(W) on the interior and the face halos (EqOfState::primitive_to_conservative,
+// eq_of_state.cpp:32-59): the conservative state that goes with a primitive state set from the
+// host
+__global__ __launch_bounds__(256) void k_p2c_faces(const double* __restrict__ W,
+                                                   double* __restrict__ U, Geo g, double gamma) {
+    int k = threadIdx.x + blockIdx.x * blockDim.x;
+    int j = threadIdx.y + blockIdx.y * blockDim.y;
+    int i = blockIdx.z;
+    if (k >= g.NZ || j >= g.NY) return;
+    int outside = (i < NGC || i >= NGC + g.nx) + (j < NGC || j >= NGC + g.ny) +
+                  (k < NGC || k >= NGC + g.nz);
+    if (outside > 1) return;
+    long   c = (long)i * g.sx + (long)j * g.sy + k;
+    double w[5], u[5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) w[q] = W[c + q * g.P];
+    p2c(w, gamma, u);
+#pragma unroll
+    for (int q = 0; q < 5; q++) U[c + q * g.P] = u[q];
+}
+
 // ------------------------------------------------------------------------------------------
 // viscous terms (centered_diffusion.cpp:3-182, cd-n.cpp:91-165, diffusion_scheme.cpp:3-58)
 // ------------------------------------------------------------------------------------------
@@ -1863,6 +1884,10 @@ static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream
     k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b);
 }
 
+static void l_p2c_faces(const double* W, double* U, Geo g, Phys ph, hipStream_t s) {
+    k_p2c_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(W, U, g, ph.gamma);
+}
+
 static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t s) {
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
@@ -2110,7 +2135,7 @@ const Launchers& launchers() {
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
                                 l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
-                                l_mlp_recon,    l_set_dt,     l_visc3d};
+                                l_mlp_recon,    l_set_dt,     l_visc3d,     l_p2c_faces};
     return L;
 }
 

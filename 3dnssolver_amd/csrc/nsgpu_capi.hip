@@ -322,12 +322,18 @@ void* nsgpu_stream(nsgpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
 int nsgpu_set_state(nsgpu_ctx* c, const double* aos, int kind) {
     if (!c || !aos) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
+    // the other form follows on the device, so get_state, nsgpu_reduce and the next step see
+    // one consistent state whichever form was set (the step itself starts from W,
+    // time_scheme.cpp:25)
     if (kind == NSGPU_STATE_PRIMITIVE) {
         int rc = upload_soa(c, aos, c->W);
         if (rc) return rc;
+        c->L->p2c_faces(c->W, c->Ua, c->g, c->ph, c->stream);
+        c->Ulast = c->Ua;
     } else if (kind == NSGPU_STATE_CONSERVATIVE) {
         int rc = upload_soa(c, aos, c->Ua);
         if (rc) return rc;
+        c->L->c2p_faces(c->Ua, c->W, c->g, c->ph, c->stream);
         c->Ulast = c->Ua;
     } else {
         return NSGPU_ERR_INVALID;

@@ -126,7 +126,9 @@ size_t nsgpu_array_len(const nsgpu_ctx* ctx);
 
 /* Upload / download the block state (whole padded array).  PRIMITIVE is the state TimeScheme::step
  * advances (the reference keeps W between steps and re-derives U from it, time_scheme.cpp:25);
- * CONSERVATIVE reads the U of the last step (interior). */
+ * CONSERVATIVE reads the U of the last step (interior).  Setting either form derives the other
+ * on the device (p2c / c2p on the interior and face halos, eq_of_state.cpp:32-87), so
+ * nsgpu_get_state, nsgpu_reduce and the next step all see the state just set. */
 int nsgpu_set_state(nsgpu_ctx* ctx, const double* aos, int kind);
 int nsgpu_get_state(nsgpu_ctx* ctx, double* aos, int kind);
 

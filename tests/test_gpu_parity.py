@@ -330,3 +330,71 @@ def test_thin_two_dimensional_block_bitwise(ns, oracle, nz):
     s.step(5)
     np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(W1))
     s.close()
+
+
+@pytest.mark.parametrize("conv", ["weno5", "weno5cw"])
+def test_minimum_block_bitwise(ns, oracle, conv):
+    # the smallest block the C ABI accepts (3 cells per axis: every stencil reaches the halos
+    # on both sides) with walls on one axis
+    C = cases_module()
+    case = dict(C.base_case(3, 3, 3), conv=conv, diff_order=4, L=(0.3, 0.3, 0.3),
+                bc=["neumann", "neumann", "periodic", "periodic", "periodic", "periodic"])
+    W0 = C.random_smooth_W(case, 51)
+    print(f"[gpu] minimum block 3^3 {conv}", flush=True)
+    Wo, _, dto = oracle.step(case, W0, 3)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    _, dt = s.step(3)
+    assert dt == dto[-1]
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    s.close()
+
+
+def test_full_size_512_isotropic_conserves(ns):
+    # the precision form's full size (512^3, 5.6 GB per state array): size-independent checks
+    # after 5 steps in FP64 and FP32 -- the periodic flux differences telescope, so the mass and
+    # momentum sums change by round-off only; energy changes by the viscous work terms of the
+    # expanded form only (bounded); the state stays healthy
+    C = cases_module()
+    case, W0 = C.config3(512)
+    print("[gpu] full size 512^3 isotropic, 5 steps", flush=True)
+    for math in ("exact", "fp32"):
+        s = ns.Solver(case, math=math)
+        s.set_state(W0, "primitive")
+        s0, _, _, _ = s.reduce()
+        s.step(5)
+        s1, _, flags, rc = s.reduce()
+        assert rc == 0 and flags == 0, (math, rc, flags)
+        vol = (2 * np.pi / 512) ** 3
+        scale = 512 ** 3 * vol  # sum of |rho| ~ rho0 * volume
+        assert abs(s1[0] - s0[0]) < 1e-12 * scale, (math, s1[0] - s0[0])
+        for q in (2, 3, 4):
+            assert abs(s1[q] - s0[q]) < 1e-11 * scale, (math, q, s1[q] - s0[q])
+        assert abs(s1[1] - s0[1]) < 1e-3 * abs(s0[1]), (math, s1[1] - s0[1])
+        s.close()
+
+
+def test_set_state_derives_the_other_form(ns, oracle):
+    # setting W gives U = p2c(W) (get_state, reduce); setting U gives W = c2p(U), and the step
+    # then advances that state, as the reference does from W (time_scheme.cpp:25)
+    C = cases_module()
+    case = dict(C.base_case(12, 10, 14), conv="weno5cw", diff_order=4)
+    W0 = C.random_smooth_W(case, 52)
+    U0 = oracle.p2c(case, W0)
+    print("[gpu] set_state forms", flush=True)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    np.testing.assert_array_equal(po.interior(s.get_state("conservative")), po.interior(U0))
+    sums, _, flags, rc = s.reduce()
+    vol = np.prod(np.array(case["L"]) / np.array([12, 10, 14]))
+    Ui = po.interior(U0).reshape(-1, 5)
+    np.testing.assert_allclose(sums, Ui.sum(axis=0) * vol, rtol=0,
+                               atol=1e-13 * np.abs(Ui).sum() * vol)
+    assert rc == 0 or flags == 4  # only the alpha flag may be raised
+    s.set_state(U0, "conservative")
+    Wc = oracle.c2p(case, U0)
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wc))
+    s.step(2)
+    Wo, _, _ = oracle.step(case, Wc, 2)
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    s.close()
