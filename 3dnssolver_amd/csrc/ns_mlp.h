@@ -43,15 +43,19 @@ struct MlpPacked {
 __device__ __forceinline__ int mlp_perm(int g, int j) { return (j < 4) ? 4 * g + j : 12 + 4 * g + j; }
 
 // GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative; tanh(y) = 1 - 2 / (e^{2y} + 1)
-__device__ __forceinline__ float mlp_tanh(float y) { return 1.0f - 2.0f / (__expf(2.0f * y) + 1.0f); }
-__device__ __forceinline__ float mlp_gelu(float x) {
-    float t = mlp_tanh(0.7978845608028654f * (x + 0.044715f * x * x * x));
-    return 0.5f * x * (1.0f + t);
+// GELU, tanh form: 0.5 x (1 + tanh(y)) = x s, s = 1 / (1 + exp(-2y)), y = sqrt(2/pi) (x +
+// 0.044715 x^3).  exp2 and the reciprocal are the hardware approximations (about 1 ulp of
+// float), far below the bfloat16 rounding the activations get next.
+__device__ __forceinline__ float mlp_sig(float x) {
+    constexpr float K = -2.0f * 0.7978845608028654f * 1.4426950408889634f;  // -2 sqrt(2/pi) log2 e
+    const float     e = __builtin_amdgcn_exp2f(x * fmaf(0.044715f * K, x * x, K));
+    return __builtin_amdgcn_rcpf(1.0f + e);
 }
+__device__ __forceinline__ float mlp_gelu(float x) { return x * mlp_sig(x); }
+// d/dx of x s: s + 2 x s (1 - s) dy/dx
 __device__ __forceinline__ float mlp_gelu_d(float x) {
-    float t = mlp_tanh(0.7978845608028654f * (x + 0.044715f * x * x * x));
-    return 0.5f * (1.0f + t) +
-           0.5f * x * (1.0f - t * t) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+    const float s = mlp_sig(x);
+    return s + 2.0f * x * s * (1.0f - s) * 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
 __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f32x4& c) {
@@ -116,7 +120,7 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpPacked* __restrict__ P
 __device__ __forceinline__ void mlp_softmax(float z0, float z1, float z2, float w[3]) {
     float m  = fmaxf(z0, fmaxf(z1, z2));
     float e0 = __expf(z0 - m), e1 = __expf(z1 - m), e2 = __expf(z2 - m);
-    float r  = 1.0f / (e0 + e1 + e2);
+    float r  = __builtin_amdgcn_rcpf(e0 + e1 + e2);
     w[0] = e0 * r, w[1] = e1 * r, w[2] = e2 * r;
 }
 
@@ -137,8 +141,14 @@ __device__ __forceinline__ int mlp_features(const T v[5], float x[8], float* rat
 #pragma unroll
     for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
     const T r = M - m;
+    if constexpr (sizeof(T) == sizeof(float)) {  // single precision: one reciprocal
+        const float ir = __builtin_amdgcn_rcpf(r);
 #pragma unroll
-    for (int k = 0; k < 5; k++) x[k] = (r > T(0)) ? (float)((v[k] - m) / r) : 0.0f;
+        for (int k = 0; k < 5; k++) x[k] = (r > T(0)) ? (v[k] - m) * ir : 0.0f;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) x[k] = (r > T(0)) ? (float)((v[k] - m) / r) : 0.0f;
+    }
     float t1, t2;
     t1   = x[0] - 2.f * x[1] + x[2];
     t2   = x[0] - 4.f * x[1] + 3.f * x[2];
