@@ -29,6 +29,10 @@ constexpr int MLP_W1 = 0, MLP_B1 = 256, MLP_W2 = 288, MLP_B2 = 1312, MLP_W3 = 13
 constexpr float MLP_EPS    = 1e-6f;
 constexpr float MLP_SMOOTH = 3.0f;
 constexpr float MLP_FLAG   = 1e3f;
+// a stencil whose range is at most MLP_RANGE of its largest magnitude counts as constant (smooth
+// regime, classic weights): its normalised shape is round-off or noise, which the network never
+// saw in training (2^-13, exact in float and double)
+constexpr float MLP_RANGE  = 1.0f / 8192.0f;
 
 // every lane's A fragments (bf16, already in the layouts above) and the FP32 biases
 struct MlpPacked {
@@ -140,14 +144,16 @@ __device__ __forceinline__ int mlp_features(const T v[5], float x[8], float* rat
     T m = v[0], M = v[0];
 #pragma unroll
     for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
-    const T r = M - m;
+    const T    r    = M - m;
+    const T    am   = (m < T(0)) ? -m : m, aM = (M < T(0)) ? -M : M;
+    const bool vary = r > T(MLP_RANGE) * (am > aM ? am : aM);
     if constexpr (sizeof(T) == sizeof(float)) {  // single precision: one reciprocal
         const float ir = __builtin_amdgcn_rcpf(r);
 #pragma unroll
-        for (int k = 0; k < 5; k++) x[k] = (r > T(0)) ? (v[k] - m) * ir : 0.0f;
+        for (int k = 0; k < 5; k++) x[k] = vary ? (v[k] - m) * ir : 0.0f;
     } else {
 #pragma unroll
-        for (int k = 0; k < 5; k++) x[k] = (r > T(0)) ? (float)((v[k] - m) / r) : 0.0f;
+        for (int k = 0; k < 5; k++) x[k] = vary ? (float)((v[k] - m) / r) : 0.0f;
     }
     float t1, t2;
     t1   = x[0] - 2.f * x[1] + x[2];

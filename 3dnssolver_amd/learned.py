@@ -27,6 +27,7 @@ WEIGHTS_PATH = os.path.join(HERE, "data", "weno_mlp.npz")
 NPARAM = 32 * 8 + 32 + 32 * 32 + 32 + 3 * 32 + 3
 ALPHA = 3.0
 EPS, SMOOTH, FLAG = 1e-6, 3.0, 1e3  # regime thresholds of the kernels (ns_mlp.h)
+RANGE = 1.0 / 8192.0  # relative stencil range below which a stencil counts as constant
 
 
 def init_params(seed=0):
@@ -52,8 +53,9 @@ def features(V):
     m = V.min(axis=1)
     M = V.max(axis=1)
     r = M - m
-    rs = np.where(r > 0, r, 1.0)
-    x = np.where((r > 0)[:, None], (V - m[:, None]) / rs[:, None], 0.0)
+    vary = r > RANGE * np.maximum(np.abs(m), np.abs(M))  # else constant up to round-off
+    rs = np.where(vary, r, 1.0)
+    x = np.where(vary[:, None], (V - m[:, None]) / rs[:, None], 0.0)
     a, b, c, d, e = x.T
     b0 = 13.0 / 12.0 * (a - 2 * b + c) ** 2 + 0.25 * (a - 4 * b + 3 * c) ** 2
     b1 = 13.0 / 12.0 * (b - 2 * c + d) ** 2 + 0.25 * (b - d) ** 2

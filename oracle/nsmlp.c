@@ -75,8 +75,10 @@ void nso_mlp_features(const double v[5], double x[8]) {
         m = v[k] < m ? v[k] : m;
         M = v[k] > M ? v[k] : M;
     }
-    double r = M - m;
-    for (int k = 0; k < 5; k++) x[k] = (r > 0.0) ? (v[k] - m) / r : 0.0;
+    double r    = M - m;
+    double am   = fabs(m), aM = fabs(M);
+    int    vary = r > NSO_MLP_RANGE * (am > aM ? am : aM);
+    for (int k = 0; k < 5; k++) x[k] = vary ? (v[k] - m) / r : 0.0;
     betas(x, x + 5);
 }
 

@@ -110,6 +110,8 @@ double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double 
 #define NSO_MLP_EPS 1e-6
 #define NSO_MLP_SMOOTH 3.0
 #define NSO_MLP_FLAG 1e3
+/* a stencil whose range is at most this share of its largest magnitude counts as constant */
+#define NSO_MLP_RANGE (1.0 / 8192.0)
 enum { NSO_REGIME_SMOOTH = 0, NSO_REGIME_MLP = 1, NSO_REGIME_FLAGGED = 2 };
 
 double nso_bf16(double v);

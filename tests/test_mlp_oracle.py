@@ -193,3 +193,30 @@ def test_ideal_weight_network_gives_linear_upwind_scheme(oracle, data):
             assert val == pytest.approx(cl + lam * (lin - cl), rel=1e-13, abs=tol)
             part += 1
     assert full > 20 and part > 5
+
+
+def test_roundoff_level_stencils_stay_classic(oracle):
+    # a stencil that varies by less than 2^-13 of its magnitude (round-off, seeded noise) counts as
+    # constant: smooth regime and the classic reconstruction bitwise, whatever shape the
+    # normalised noise has; just above the floor the shape decides again
+    rng = np.random.default_rng(61)
+    params = random_params(37)
+    oracle.set_mlp(np.asarray(params, dtype=np.float32).astype(np.float64), bf16=True)
+    for scale in (1.0, 3.7, -250.0):
+        for _ in range(50):
+            v = scale * (1.0 + 1e-5 * rng.standard_normal(5))
+            x = oracle.mlp_features(v)
+            assert oracle.mlp_regime(x) == po.REGIMES["smooth"]
+            for side in (0, 1):
+                val, rg = oracle.weno_learned(params, v, side, bf16=True)
+                assert rg == po.REGIMES["smooth"]
+                classic = oracle.weno_left(v, "shu") if side == 0 else oracle.weno_right(v, "shu")
+                assert val == classic
+    # above the floor the normalised shape decides: a network-regime shape riding on a large
+    # constant at 1e-3 relative variation keeps its regime and features
+    shape = np.array([0.0, 0.1, 0.3, 0.35, 1.0])
+    assert oracle.mlp_regime(oracle.mlp_features(shape)) == po.REGIMES["mlp"]
+    v = 2.0 + 2e-3 * shape
+    x = oracle.mlp_features(v)
+    assert oracle.mlp_regime(x) == po.REGIMES["mlp"]
+    np.testing.assert_allclose(x[:5], shape, atol=1e-9)
