@@ -245,7 +245,7 @@ def learned_workload(args):
         s.close()
     errors = {}
     ne = args.eval_n
-    for prob in ("shear", "jump"):
+    for prob in ("shear", "jump") if args.eval_steps > 0 else ():  # --eval-steps 0: rates only
         print(f"[bench] accuracy: {prob} {ne}^3 vs {2 * ne}^3 FP64", file=sys.stderr, flush=True)
         c, w = C.config4(ne, prob)
         fc, fw = C.config4(2 * ne, prob)
