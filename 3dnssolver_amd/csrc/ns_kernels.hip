@@ -574,6 +574,16 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ U,
 // ------------------------------------------------------------------------------------------
 constexpr int TILE_S = 8;    // x/y tiles: faces along the sweep per block (one wave each)
 constexpr int TILE_L = 64;   // lanes along k
+// sweep tile of the single-block x/y kernels: 16 faces (1024-thread blocks) in single precision,
+// where the 32 KB tile of 8 faces leaves room and the 5 extra cells per tile weigh more
+#ifndef NS_TILE_S1
+#if NS_FP32
+#define NS_TILE_S1 16
+#else
+#define NS_TILE_S1 8
+#endif
+#endif
+constexpr int TILE_S1 = NS_TILE_S1;
 #ifndef NS_TILE_WAVES
 #define NS_TILE_WAVES 1      // minimum waves per SIMD requested for the tile kernels
 #endif
@@ -582,27 +592,27 @@ constexpr int TILE_L = 64;   // lanes along k
 // stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
 // values of the 13 cells the tile needs are computed once into LDS.  (bx, by, bz) = (k tile,
 // row tile, sweep tile).
-template <int DIR, int CONV, int WT, int OPT>
+template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
                                                   const MlpPacked* __restrict__ mlp) {
-    constexpr int NC = TILE_S + 5;
+    constexpr int NC = TS + 5;
     constexpr int NV = split_nv<OPT>();
     const int lane = threadIdx.x, w = threadIdx.y;
     const int rows = TILE_L / kw;
     const int k    = NGC + bx * kw + (lane & (kw - 1));
     const int o    = NGC + by * rows + lane / kw;  // j for the x sweep, i for the y sweep
-    const int s0   = bz * TILE_S;                   // first face of the tile (0-based)
+    const int s0   = bz * TS;                       // first face of the tile (0-based)
     const int n    = (DIR == 0) ? g.nx : g.ny;
     const int no   = (DIR == 0) ? g.ny : g.nx;
     const long ss  = (DIR == 0) ? g.sx : g.sy;      // stride along the sweep
     const long so  = (DIR == 0) ? g.sy : g.sx;      // stride of the other index
     const bool kin = (k < NGC + g.nz) && (o < NGC + no);
     // cells s0 .. s0+12 (padded sweep index) feed faces s0+2 .. s0+9
-    for (int t = w; t < NC; t += TILE_S) {
+    for (int t = w; t < NC; t += TS) {
         int cs = s0 + t;
         if (kin && cs <= n + 2 * NGC - 1) {
             real sv[NV];
@@ -635,13 +645,12 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
 }
 
 template <int DIR, int CONV, int WT, int OPT>
-__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_xy(const double* __restrict__ U,
-                                                      const double* __restrict__ W,
-                                                      double* __restrict__ f, Geo g,
-                                                      real gamma, const MlpPacked* mlp) {
-    __shared__ real tile[(TILE_S + 5) * split_nv<OPT>() * TILE_L];
-    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
-                                          TILE_L, tile, mlp);
+__global__ __launch_bounds__(64 * TILE_S1, NS_TILE_WAVES) void k_flux_tile_xy(
+    const double* __restrict__ U, const double* __restrict__ W, double* __restrict__ f, Geo g,
+    real gamma, const MlpPacked* mlp) {
+    __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
+    flux_tile_xy_body<DIR, CONV, WT, OPT, TILE_S1>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
+                                                   blockIdx.z, TILE_L, tile, mlp);
 }
 
 // z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
@@ -1588,12 +1597,28 @@ __global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict_
 // ------------------------------------------------------------------------------------------
 constexpr int RED_THREADS = 256;
 
+// sums of five doubles over a 256-thread workgroup: butterfly over the 64 lanes of each
+// wavefront (__shfl_xor, DPP / ds_swizzle under the hood), then the 4 wave sums in a fixed order;
+// the result is in every lane of wave 0
+__device__ __forceinline__ void block_sum5(double acc[5]) {
+    __shared__ double ws[5][RED_THREADS / 64];
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        for (int o = 32; o > 0; o >>= 1) acc[q] += __shfl_xor(acc[q], o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) ws[q][wid] = acc[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[q] = (ws[q][0] + ws[q][1]) + (ws[q][2] + ws[q][3]);
+}
+
 __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict__ U,
                                                         const double* __restrict__ W,
                                                         double* __restrict__ partial,
                                                         unsigned long long* dtmin,
                                                         unsigned* flags, Geo g, Phys ph) {
-    __shared__ double sh[5][RED_THREADS];
     long     n      = (long)g.nx * g.ny * g.nz;
     double   acc[5] = {0, 0, 0, 0, 0};
     double   dtl    = 1E300;
@@ -1622,33 +1647,23 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce(const double* __restrict
             dtl      = (d < dtl) ? d : dtl;
         }
     }
+    block_sum5(acc);
+    if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 5) partial[blockIdx.x * 5 + threadIdx.x] = sh[threadIdx.x][0];
+        for (int q = 0; q < 5; q++) partial[blockIdx.x * 5 + q] = acc[q];
     if (fl) atomicOr(flags, fl);
     block_min_atomic(dtl, dtmin);
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double* __restrict__ partial,
                                                               int nb, double* __restrict__ sums) {
-    __shared__ double sh[5][RED_THREADS];
-    double            acc[5] = {0, 0, 0, 0, 0};
+    double acc[5] = {0, 0, 0, 0, 0};
     for (int b = threadIdx.x; b < nb; b += RED_THREADS)
         for (int q = 0; q < 5; q++) acc[q] += partial[b * 5 + q];
-    for (int q = 0; q < 5; q++) sh[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s)
-            for (int q = 0; q < 5; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x < 5) sums[threadIdx.x] = sh[threadIdx.x][0];
+    block_sum5(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int q = 0; q < 5; q++) sums[q] = acc[q];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1895,14 +1910,15 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
 template <int DIR, int CONV, int B, int OPT>
 static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
                      const MlpPacked* mlp, hipStream_t s) {
-    dim3 blk(TILE_L, 8, 1);
     if constexpr (DIR == 2) {
+        dim3 blk(TILE_L, 8, 1);
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
         k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     } else {
         int  n = (DIR == 0) ? g.nx : g.ny;
         int  o = (DIR == 0) ? g.ny : g.nx;
-        dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S - 1) / TILE_S);
+        dim3 blk(TILE_L, TILE_S1, 1);
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S1 - 1) / TILE_S1);
         k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     }
     // the last face plane (sweep index NGC + n - 1)

@@ -26,6 +26,7 @@ FLOP_PER_CELL_STEP = {"weno5cw": 12050.0, "weno5": 7860.0}
 FLOP_PER_FACE = {"weno5cw": 1190.0, "weno5": 725.0}
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense, with packed FP32 (which this build leaves off)
 
 
 def cpu_baseline_child(n, steps):
@@ -96,6 +97,11 @@ def precision_workload(args):
         states[math] = (U[..., 0].copy(), U[..., 1].copy())
         _, _, flags, rc = s.reduce()
         out[math + "_health"] = {"status": rc, "flags": flags}
+        # per-kernel times of one more step (HIP events on the context's stream)
+        s.timing(True)
+        s.step(1)
+        out[math + "_timers"] = s.timers()
+        s.timing(False)
         s.close()
         del U
     errs = {}
@@ -105,6 +111,18 @@ def precision_workload(args):
         errs[name] = {"L2_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(b * b))),
                       "Linf_rel": float(np.abs(d).max() / np.abs(b).max()),
                       "L2_abs": float(np.sqrt(np.mean(d * d))), "Linf_abs": float(np.abs(d).max())}
+    # roofline of the FP32 face kernel (the dominant sweep) and of the HBM-bound residual
+    tm = out["fp32_timers"]
+    flux = {k: v for k, v in tm.items() if k.startswith("face_flux")}
+    dom = max(flux, key=lambda k: flux[k][0])
+    dom_ms = flux[dom][0] / flux[dom][1]
+    flop = FLOP_PER_FACE["weno5cw"] * (n + 1) * n * n
+    fp32_tf = flop / (dom_ms * 1e-3) / 1e12
+    res_ms = tm["residual_update"][0] / tm["residual_update"][1]
+    res_gbs = 47 * 8.0 * n ** 3 / (res_ms * 1e-3) / 1e9
+    fp64_dom = max((k for k in out["exact_timers"] if k.startswith("face_flux")),
+                   key=lambda k: out["exact_timers"][k][0])
+    fp64_ms = out["exact_timers"][fp64_dom][0] / out["exact_timers"][fp64_dom][1]
     print(json.dumps({
         "metric": "precision form: Gcell-RK3-steps/s, FP32 reconstruction vs FP64",
         "value": round(out["fp32"], 6), "unit": "Gcell-RK3-steps/s", "n_gpus": 1,
@@ -116,6 +134,17 @@ def precision_workload(args):
         "fp64_value": round(out["exact"], 6), "speedup_vs_fp64": round(out["fp32"] / out["exact"], 4),
         "errors_vs_fp64": errs,
         "health": {"fp32": out["fp32_health"], "fp64": out["exact_health"]},
+        "roofline": {"bound": "valu", "compute": "FP32 vector ALU (packed FP32 disabled in this "
+                                                 "build: 78.6 TFLOP/s attainable)",
+                     "kernel": dom, "achieved": round(fp32_tf, 4), "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(fp32_tf / FP32_PEAK_TFLOPS, 5),
+                     "traffic": None, "flop_per_launch": flop, "avg_launch_ms": round(dom_ms, 5),
+                     "fp64_same_kernel_ms": round(fp64_ms, 5)},
+        "roofline_residual": {"bound": "hbm", "kernel": "residual_update",
+                              "achieved": round(res_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(res_gbs / HBM_PEAK_GBS, 4), "avg_launch_ms": round(res_ms, 5)},
+        "kernel_ms_per_step": {m: {k: round(v[0], 4) for k, v in out[m + "_timers"].items()}
+                               for m in ("exact", "fp32")},
     }))
 
 
