@@ -986,24 +986,24 @@ __global__ __launch_bounds__(256) void k_p2c_faces(const double* __restrict__ W,
 // value at offset t - H, t = 0 .. 2H.  EXACT: sum of f * c_k / d in k order, the zero centre term
 // of the first derivative skipped (an exact +-0 addend); FAST: sum of f * (c_k / d) with the
 // reciprocal of d hoisted.
-template <int ORDER, bool SECOND, class F>
-__device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
+template <int ORDER, bool SECOND, class T, class F>
+__device__ __forceinline__ T fd_sum_t(const F& f, T d, T id) {
     constexpr int H     = ORDER / 2;
     constexpr int OI    = ORDER / 2 - 1;
-    real        s     = RL(0.);
+    T             s     = T(0.);
     bool          first = true;
 #if !NS_FAST
-    const Rcp rd = rcp(d);
+    const auto rd = rcp(d);
 #endif
 #pragma unroll
     for (int t = 0; t <= 2 * H; t++) {
         if (!SECOND && t == H) continue;
-        real co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
+        T co = SECOND ? c_fd2[OI][t] : c_fd1[OI][t];
 #if !NS_FAST
-        real v = dv(f(t) * co, rd);
+        T v = dv(f(t) * co, rd);
         (void)id;
 #else
-        real v = f(t) * (co * id);
+        T v = f(t) * (co * id);
         (void)d;
 #endif
         if (first) {
@@ -1013,6 +1013,10 @@ __device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
         }
     }
     return s;
+}
+template <int ORDER, bool SECOND, class F>
+__device__ __forceinline__ real fd_sum(const F& f, real d, real id) {
+    return fd_sum_t<ORDER, SECOND, real>(f, d, id);
 }
 
 // Kernels with x-direction stencils march along i: each thread owns one (j, k) column of a
@@ -1034,7 +1038,9 @@ constexpr int MARCH   = NS_MARCH;
 constexpr int KW_CELL = NS_KW_CELL;  // lanes along k of the single-block cell kernels
 
 // first derivatives of rho, p, u, v along x and y on the interior.  256 threads: kw lanes along
-// k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i)
+// k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i).  FP64 in
+// every build: the kernel is bound by its HBM traffic, and single-precision arithmetic on FP64
+// planes only adds conversions (it ran 1.5x slower)
 template <int ORDER>
 __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double* __restrict__ Dx,
                                             double* __restrict__ Dy, const Geo& g, int bx, int by,
@@ -1047,7 +1053,7 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
     if (k >= g.nz || j >= g.ny) return;
     const int  i1 = (i0 + march < g.nx) ? i0 + march : g.nx;
     const long c0 = (long)(j + NGC) * g.sy + (k + NGC);
-    real     win[4][2 * H + 1];
+    double   win[4][2 * H + 1];
 #pragma unroll
     for (int t = 0; t < 2 * H; t++)
 #pragma unroll
@@ -1059,8 +1065,9 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const double* fy = W + q * g.P + c;
-            Dx[c + q * g.P] = fd_sum<ORDER, false>([&](int t) { return win[q][t]; }, g.dx, g.idx);
-            Dy[c + q * g.P] = fd_sum<ORDER, false>(
+            Dx[c + q * g.P] =
+                fd_sum_t<ORDER, false, double>([&](int t) { return win[q][t]; }, g.dx, g.idx);
+            Dy[c + q * g.P] = fd_sum_t<ORDER, false, double>(
                 [&](int t) { return fy[(long)(t - H) * g.sy]; }, g.dy, g.idy);
         }
 #pragma unroll

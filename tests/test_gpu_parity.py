@@ -167,6 +167,23 @@ def test_fast_math_rhs_within_1e12(ns, name):
     s.close()
 
 
+@pytest.mark.parametrize("name", GOLDEN_RHS)
+def test_fp32_rhs_against_golden(ns, name):
+    # the precision form on the reference's golden RHS fixtures: single-precision reconstruction
+    # and viscous fluxes stay within float round-off of the reference's FP64 result (max |a-b| /
+    # max |b| per variable; float epsilon is 6e-8, the bound allows the WENO weights' and the
+    # projections' amplification: measured up to 2.1e-5 with the Borges weights in
+    # characteristic space), and really run in single precision
+    case, g = load_golden(name)
+    s = ns.Solver(case, math="fp32")
+    _, _, R = s.rhs(g["U_in"], 3)
+    err = rel_err(po.interior(R), po.interior(g["R"]))
+    print(f"[gpu] fp32 golden {name}: {err.max():.2e}", flush=True)
+    assert err.max() <= 5e-5, err
+    assert err.max() > 1e-12, err
+    s.close()
+
+
 def test_fast_math_100_steps_within_1e10(ns, oracle):
     C = cases_module()
     case, W0 = C.config2(24)
