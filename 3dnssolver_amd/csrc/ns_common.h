@@ -2,10 +2,11 @@
 // (nsgpu_capi.hip) and the kernel translation units (ns_kernels.hip, compiled once per math mode).
 //
 // Device layout: structure of arrays.  A 5-variable field is 5 planes of the padded block, each
-// plane row-major [NX][NY][NZ] with k fastest (the reference Array4d order with q moved out,
-// containers/array_4d.hpp:116-124), plane stride P = NX*NY*NZ, NX = nx + 2*NGC.  Loads along k
-// are unit-stride per variable, so a wavefront reading 64 consecutive k of one variable reads one
-// contiguous 512-byte run.
+// plane row-major [NX][NY][sy] with k fastest (the reference Array4d order with q moved out,
+// containers/array_4d.hpp:116-124), NX = nx + 2*NGC.  Rows are padded to sy = NZ rounded up to
+// KALIGN doubles and every array starts KOFF doubles into its allocation, so the first interior
+// cell of every row (k = NGC) sits on a 128-byte boundary: a wavefront reading 64 consecutive
+// interior k of one variable reads exactly four 128-byte lines.  Plane stride P = NX*NY*sy.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,6 +15,11 @@ namespace nsgpu {
 
 constexpr int NGC = 3;
 constexpr int NQ  = 5;
+#ifndef NS_KALIGN
+#define NS_KALIGN 16
+#endif
+constexpr int KALIGN = NS_KALIGN;                             // row alignment in doubles
+constexpr int KOFF   = (KALIGN - NGC % KALIGN) % KALIGN;      // array start offset in doubles
 
 struct Geo {
     int    nx, ny, nz;  // interior cells

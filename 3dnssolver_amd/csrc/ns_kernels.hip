@@ -1585,17 +1585,27 @@ __global__ void k_time_accum(double* t, const unsigned long long* dtmin, double 
 // ------------------------------------------------------------------------------------------
 // host layout conversion: reference AoS [NX][NY][NZ][5] <-> device SoA planes
 // ------------------------------------------------------------------------------------------
-__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, long P) {
-    long c = threadIdx.x + (long)blockIdx.x * blockDim.x;
-    if (c >= P) return;
-#pragma unroll
-    for (int q = 0; q < 5; q++) soa[c + q * P] = aos[c * 5 + q];
+// cell c of the dense host order (i, j, k) -> its offset in the padded device planes
+__device__ __forceinline__ long dev_offset(long c, const Geo& g) {
+    const int  k = (int)(c % g.NZ);
+    const long t = c / g.NZ;
+    const int  j = (int)(t % g.NY);
+    const int  i = (int)(t / g.NY);
+    return (long)i * g.sx + (long)j * g.sy + k;
 }
-__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, long P) {
+__global__ void k_aos_to_soa(const double* __restrict__ aos, double* __restrict__ soa, Geo g) {
     long c = threadIdx.x + (long)blockIdx.x * blockDim.x;
-    if (c >= P) return;
+    if (c >= (long)g.NX * g.NY * g.NZ) return;
+    const long d = dev_offset(c, g);
 #pragma unroll
-    for (int q = 0; q < 5; q++) aos[c * 5 + q] = soa[c + q * P];
+    for (int q = 0; q < 5; q++) soa[d + q * g.P] = aos[c * 5 + q];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, double* __restrict__ aos, Geo g) {
+    long c = threadIdx.x + (long)blockIdx.x * blockDim.x;
+    if (c >= (long)g.NX * g.NY * g.NZ) return;
+    const long d = dev_offset(c, g);
+#pragma unroll
+    for (int q = 0; q < 5; q++) aos[c * 5 + q] = soa[d + q * g.P];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2054,10 +2064,12 @@ static void l_visc3d(const double* W, double* Dx, double* Dy, double* Dz, double
 }
 
 static void l_aos_to_soa(const double* aos, double* soa, Geo g, hipStream_t s) {
-    k_aos_to_soa<<<(unsigned)((g.P + 255) / 256), 256, 0, s>>>(aos, soa, g.P);
+    const long n = (long)g.NX * g.NY * g.NZ;
+    k_aos_to_soa<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(aos, soa, g);
 }
 static void l_soa_to_aos(const double* soa, double* aos, Geo g, hipStream_t s) {
-    k_soa_to_aos<<<(unsigned)((g.P + 255) / 256), 256, 0, s>>>(soa, aos, g.P);
+    const long n = (long)g.NX * g.NY * g.NZ;
+    k_soa_to_aos<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(soa, aos, g);
 }
 
 static void l_reduce(const double* U, const double* W, double* partial, int nblocks_max,

@@ -146,15 +146,34 @@ int alloc(double** p, size_t bytes, hipStream_t s) {
     return NSGPU_OK;
 }
 
+// a padded-layout array of `bytes`: KOFF doubles into a zero-filled allocation, so the first
+// interior cell of each row is 128-byte aligned (hipMalloc returns at least 256-byte alignment);
+// free with free_array
+int alloc_array(double** p, size_t bytes, hipStream_t s) {
+    double* raw = nullptr;
+    int     rc  = alloc(&raw, bytes + KOFF * sizeof(double), s);
+    if (rc) return rc;
+    *p = raw + KOFF;
+    return NSGPU_OK;
+}
+void free_array(double* p) {
+    if (p) hipFree(p - KOFF);
+}
+
+// doubles of one 5-variable array in the reference's dense host layout
+size_t host_len(const Geo& g) { return (size_t)g.NX * g.NY * g.NZ * 5; }
+
 int upload_soa(nsgpu_ctx* c, const double* aos, double* dst) {
-    HIPCHK(hipMemcpyAsync(c->stage_aos, aos, 5 * c->plane_bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->stage_aos, aos, host_len(c->g) * sizeof(double),
+                          hipMemcpyHostToDevice, c->stream));
     c->L->aos_to_soa(c->stage_aos, dst, c->g, c->stream);
     return NSGPU_OK;
 }
 
 int download_soa(nsgpu_ctx* c, const double* src, double* aos) {
     c->L->soa_to_aos(src, c->stage_aos, c->g, c->stream);
-    HIPCHK(hipMemcpyAsync(aos, c->stage_aos, 5 * c->plane_bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(aos, c->stage_aos, host_len(c->g) * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return NSGPU_OK;
 }
@@ -230,9 +249,9 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     Geo& g    = c->g;
     g.nx = cfg->nx, g.ny = cfg->ny, g.nz = cfg->nz;
     g.NX = g.nx + 2 * NGC, g.NY = g.ny + 2 * NGC, g.NZ = g.nz + 2 * NGC;
-    g.sy = g.NZ;
-    g.sx = g.NY * g.NZ;
-    g.P  = (long)g.NX * g.NY * g.NZ;
+    g.sy = (g.NZ + KALIGN - 1) / KALIGN * KALIGN;
+    g.sx = g.NY * g.sy;
+    g.P  = (long)g.NX * g.sx;
     g.dx = cfg->Lx / cfg->nx;  // Grid::compute_step_sizes (grid.cpp:14-16)
     g.dy = cfg->Ly / cfg->ny;
     g.dz = cfg->Lz / cfg->nz;
@@ -263,21 +282,21 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     size_t five = 5 * c->plane_bytes, four = 4 * c->plane_bytes;
     rc = NSGPU_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
-    if (!rc) rc = alloc(&c->W, five, c->stream);
-    if (!rc) rc = alloc(&c->W2, five, c->stream);
-    for (int t = 0; t < 2 && !rc; t++) rc = alloc(&c->Ws[t], five, c->stream);
-    if (!rc) rc = alloc(&c->Un, five, c->stream);
-    if (!rc) rc = alloc(&c->Ua, five, c->stream);
-    if (!rc) rc = alloc(&c->Ub, five, c->stream);
-    for (int d = 0; d < 3 && !rc; d++) rc = alloc(&c->F[d], five, c->stream);
-    if (!rc) rc = alloc(&c->Dx, four, c->stream);
-    if (!rc) rc = alloc(&c->Dy, four, c->stream);
+    if (!rc) rc = alloc_array(&c->W, five, c->stream);
+    if (!rc) rc = alloc_array(&c->W2, five, c->stream);
+    for (int t = 0; t < 2 && !rc; t++) rc = alloc_array(&c->Ws[t], five, c->stream);
+    if (!rc) rc = alloc_array(&c->Un, five, c->stream);
+    if (!rc) rc = alloc_array(&c->Ua, five, c->stream);
+    if (!rc) rc = alloc_array(&c->Ub, five, c->stream);
+    for (int d = 0; d < 3 && !rc; d++) rc = alloc_array(&c->F[d], five, c->stream);
+    if (!rc) rc = alloc_array(&c->Dx, four, c->stream);
+    if (!rc) rc = alloc_array(&c->Dy, four, c->stream);
     if (!rc && (cfg->physics & NSGPU_PHYS_VISCOUS)) {
-        rc = alloc(&c->Dz, four, c->stream);
-        if (!rc) rc = alloc(&c->Rv, four, c->stream);
+        rc = alloc_array(&c->Dz, four, c->stream);
+        if (!rc) rc = alloc_array(&c->Rv, four, c->stream);
     }
-    if (!rc) rc = alloc(&c->R, five, c->stream);
-    if (!rc) rc = alloc(&c->stage_aos, five, c->stream);
+    if (!rc) rc = alloc_array(&c->R, five, c->stream);
+    if (!rc) rc = alloc_array(&c->stage_aos, five, c->stream);
     c->partial_max = 4096;
     if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max, c->stream);
     if (!rc) rc = alloc((double**)&c->dtmin, 2 * sizeof(unsigned long long), c->stream);
@@ -304,9 +323,12 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->W,  c->W2, c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub, c->F[0], c->F[1], c->F[2], c->Dx, c->Dy,
-                      c->Dz, c->Rv, c->R,  c->stage_aos, c->partial, (double*)c->dtmin, (double*)c->flags,
-                      c->dtime, c->sums, (double*)c->mlp_params, (double*)c->mlp_packed};
+    double* arrays[] = {c->W,     c->W2,   c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub,
+                        c->F[0],  c->F[1], c->F[2],  c->Dx,    c->Dy, c->Dz, c->Rv,
+                        c->R,     c->stage_aos};
+    for (double* p : arrays) free_array(p);
+    double* bufs[] = {c->partial, (double*)c->dtmin, (double*)c->flags, c->dtime, c->sums,
+                      (double*)c->mlp_params, (double*)c->mlp_packed};
     for (double* p : bufs)
         if (p) hipFree(p);
     for (int t = 0; t < 2 * T_COUNT; t++)
@@ -315,7 +337,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     delete c;
 }
 
-size_t nsgpu_array_len(const nsgpu_ctx* c) { return c ? (size_t)c->g.P * 5 : 0; }
+size_t nsgpu_array_len(const nsgpu_ctx* c) { return c ? host_len(c->g) : 0; }
 
 void* nsgpu_stream(nsgpu_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
