@@ -1044,11 +1044,11 @@ constexpr int KW_CELL = NS_KW_CELL;  // lanes along k of the single-block cell k
 template <int ORDER>
 __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double* __restrict__ Dx,
                                             double* __restrict__ Dy, const Geo& g, int bx, int by,
-                                            int bz, int kw, int march) {
+                                            int bz, int kw, int march, int nthreads = 256) {
     constexpr int H   = ORDER / 2;
     const int     tid = threadIdx.x + threadIdx.y * 64;
     const int     k   = (tid & (kw - 1)) + bx * kw;
-    const int     j   = tid / kw + by * (256 / kw);
+    const int     j   = tid / kw + by * (nthreads / kw);
     const int     i0  = bz * march;
     if (k >= g.nz || j >= g.ny) return;
     const int  i1 = (i0 + march < g.nx) ? i0 + march : g.nx;
@@ -1095,13 +1095,17 @@ __device__ __forceinline__ void cell_tile(int& bx, int& by, int& bz) {
 #endif
 }
 
+#ifndef NS_DERIV_ROWS
+#define NS_DERIV_ROWS 4  // rows j per block of the single-block derivative kernel (64 k each)
+#endif
+constexpr int DERIV_ROWS = NS_DERIV_ROWS;
 template <int ORDER>
-__global__ __launch_bounds__(256, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
+__global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
                                                 Geo g) {
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    deriv1_body<ORDER>(W, Dx, Dy, g, bx, by, bz, KW_CELL, MARCH);
+    deriv1_body<ORDER>(W, Dx, Dy, g, bx, by, bz, 64, MARCH, 64 * DERIV_ROWS);
 }
 
 // periodic exchange of the derivative halos that the cross derivatives read: the j halos of
@@ -1994,11 +1998,12 @@ static void l_face_flux(int dir, const double* U, const double* W, double* f, Ge
 }
 
 static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {
-    dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
-             (g.nx + MARCH - 1) / MARCH);
-    if (ph.diff_order == 2) k_deriv1<2><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
-    else if (ph.diff_order == 4) k_deriv1<4><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
-    else k_deriv1<6><<<grd, BLK, 0, s>>>(W, Dx, Dy, g);
+    const dim3 blk(64, DERIV_ROWS, 1);
+    const dim3 grd((g.nz + 63) / 64, (g.ny + DERIV_ROWS - 1) / DERIV_ROWS,
+                   (g.nx + MARCH - 1) / MARCH);
+    if (ph.diff_order == 2) k_deriv1<2><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+    else if (ph.diff_order == 4) k_deriv1<4><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+    else k_deriv1<6><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
 }
 
 static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s) {
