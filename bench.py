@@ -336,8 +336,8 @@ def main():
         cpu_baseline_child(args.cpu_n, args.cpu_steps)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              f"see DESIGN.md for how the block would shard.", file=sys.stderr)
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for {args.gpus}).",
+              file=sys.stderr)
         sys.exit(2)
     if args.workload == "precision":
         precision_workload(args)
