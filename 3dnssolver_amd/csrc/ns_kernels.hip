@@ -1437,8 +1437,8 @@ __global__ __launch_bounds__(256) void k_deriv3d(const double* __restrict__ W,
                                                  double* __restrict__ Dx, double* __restrict__ Dy,
                                                  double* __restrict__ Dz, Geo g) {
     constexpr int H = ORDER / 2;
-    const int     k = threadIdx.x + blockIdx.x * 64, j = threadIdx.y + blockIdx.y * 4;
-    const int     i = blockIdx.z;
+    const int     k = threadIdx.x + blockIdx.y * 64, j = threadIdx.y + blockIdx.z * 4;
+    const int     i = blockIdx.x;  // consecutive workgroups step along i: x stencils meet in L2
     if (k >= g.nz || j >= g.ny) return;
     const long P = g.P;
     const long c = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
@@ -1489,8 +1489,8 @@ __global__ __launch_bounds__(256) void k_visc3d(const double* __restrict__ W,
                                                 const double* __restrict__ Dz,
                                                 double* __restrict__ Rv, Geo g, Phys ph) {
     constexpr int H = ORDER / 2;
-    const int     k = threadIdx.x + blockIdx.x * 64, j = threadIdx.y + blockIdx.y * 4;
-    const int     i = blockIdx.z;
+    const int     k = threadIdx.x + blockIdx.y * 64, j = threadIdx.y + blockIdx.z * 4;
+    const int     i = blockIdx.x;  // consecutive workgroups step along i: x stencils meet in L2
     if (k >= g.nz || j >= g.ny) return;
     const long    P     = g.P;
     const long    c     = (long)(i + NGC) * g.sx + (long)(j + NGC) * g.sy + (k + NGC);
@@ -2052,7 +2052,7 @@ static void l_residual(const double* fx, const double* fy, const double* fz, con
 template <int ORDER>
 static void visc3d_order(const double* W, double* Dx, double* Dy, double* Dz, double* Rv, Geo g,
                          Phys ph, Bnd b, hipStream_t s) {
-    dim3 grd((g.nz + 63) / 64, (g.ny + 3) / 4, g.nx);
+    dim3 grd(g.nx, (g.nz + 63) / 64, (g.ny + 3) / 4);
     k_deriv3d<ORDER><<<grd, BLK, 0, s>>>(W, Dx, Dy, Dz, g);
     int  m2 = g.nz > g.ny ? g.nz : g.ny, m1 = g.ny > g.nx ? g.ny : g.nx;
     dim3 hg((m2 + 63) / 64, (m1 + 3) / 4, 6 * 2 * NGC);
