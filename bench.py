@@ -327,7 +327,7 @@ def main():
                                           "corrected"], default="reference")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
     args = ap.parse_args()
