@@ -357,7 +357,9 @@ def main():
     n = args.n
     print(f"[bench] {n}^3 {args.conv} cd4 math={args.math}: setup", file=sys.stderr, flush=True)
     s = ns.Solver(case, math=args.math)
+    t_up = time.perf_counter()
     s.set_state(W0, "primitive")
+    t_up = time.perf_counter() - t_up  # host array -> device (PCIe) + layout conversion
     if args.warmup:
         s.step(args.warmup)
     print(f"[bench] warmup done, timing {args.steps} steps", file=sys.stderr, flush=True)
@@ -368,6 +370,10 @@ def main():
     cells = n ** 3
     value = cells * args.steps / el / 1e9
     ms_per_step = el / args.steps * 1e3
+    t_down = time.perf_counter()
+    s.get_state("primitive")  # device -> host array
+    t_down = time.perf_counter() - t_down
+    pcie_value = cells * args.steps / (el + t_up + t_down) / 1e9
 
     # per-kernel durations with HIP events on the context's stream (separate pass)
     s.timing(True)
@@ -438,6 +444,10 @@ def main():
         },
         "kernel_ms_per_step": {k: round(v[0] / args.timing_steps, 4) for k, v in timers.items()},
         "health": {"status": rc, "flags": flags, "dt_last": dt},
+        "pcie_inclusive": {"value": round(pcie_value, 6), "upload_ms": round(t_up * 1e3, 2),
+                           "download_ms": round(t_down * 1e3, 2),
+                           "note": "the timed steps plus one upload of the host state before and "
+                                   "one download after"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
