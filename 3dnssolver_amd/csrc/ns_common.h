@@ -147,6 +147,7 @@ struct Launchers {
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
                       hipStream_t s);
+    // Dy null: dW/dx and its halo only (for residual_fd)
     void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
     void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);
     // R (interior) = R_in (or 0) + convection(fx,fy,fz) + diffusion(W, Dx, Dy); then either
@@ -182,6 +183,13 @@ struct Launchers {
                    Phys ph, Bnd b, hipStream_t s);
     // U = p2c(W) on the interior and the face halos
     void (*p2c_faces)(const double* W, double* U, Geo g, Phys ph, hipStream_t s);
+    // residual as above for the 2-D viscous terms (diff_order > 0, physics bit 1 clear) with
+    // their y derivatives formed in the same kernel: Dx from deriv1 / deriv_halo with Dy null
+    void (*residual_fd)(const double* fx, const double* fy, const double* fz, const double* W,
+                        const double* Dx, const double* Rin, double* Rout, const StageArgs* st,
+                        const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
+                        hipStream_t s);
+    int fuse_diff;  // 1: single blocks use residual_fd
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

@@ -1041,7 +1041,7 @@ constexpr int KW_CELL = NS_KW_CELL;  // lanes along k of the single-block cell k
 // k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i).  FP64 in
 // every build: the kernel is bound by its HBM traffic, and single-precision arithmetic on FP64
 // planes only adds conversions (it ran 1.5x slower)
-template <int ORDER>
+template <int ORDER, bool DY = true>
 __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double* __restrict__ Dx,
                                             double* __restrict__ Dy, const Geo& g, int bx, int by,
                                             int bz, int kw, int march, int nthreads = 256) {
@@ -1067,8 +1067,9 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
             const double* fy = W + q * g.P + c;
             Dx[c + q * g.P] =
                 fd_sum_t<ORDER, false, double>([&](int t) { return win[q][t]; }, g.dx, g.idx);
-            Dy[c + q * g.P] = fd_sum_t<ORDER, false, double>(
-                [&](int t) { return fy[(long)(t - H) * g.sy]; }, g.dy, g.idy);
+            if constexpr (DY)
+                Dy[c + q * g.P] = fd_sum_t<ORDER, false, double>(
+                    [&](int t) { return fy[(long)(t - H) * g.sy]; }, g.dy, g.idy);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++)
@@ -1099,13 +1100,14 @@ __device__ __forceinline__ void cell_tile(int& bx, int& by, int& bz) {
 #define NS_DERIV_ROWS 4  // rows j per block of the single-block derivative kernel (64 k each)
 #endif
 constexpr int DERIV_ROWS = NS_DERIV_ROWS;
-template <int ORDER>
+// DY = false: dW/dx only (the residual kernel forms dW/dy itself)
+template <int ORDER, bool DY>
 __global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
                                                 Geo g) {
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    deriv1_body<ORDER>(W, Dx, Dy, g, bx, by, bz, 64, MARCH, 64 * DERIV_ROWS);
+    deriv1_body<ORDER, DY>(W, Dx, Dy, g, bx, by, bz, 64, MARCH, 64 * DERIV_ROWS);
 }
 
 // periodic exchange of the derivative halos that the cross derivatives read: the j halos of
@@ -1151,16 +1153,17 @@ __device__ __forceinline__ real mu_suth(real rho, real p, const Phys& ph) {
     return dv((real)ph.suth_k, T + (real)ph.suth_C) * (T * sqrt(T));
 }
 
+// dxy[a][t]: dW/dx of u (a = 0), v (a = 1) at y offsets t - H (FP64, as the derivative planes
+// hold them); dxr, dxp, dyr, dyp: dW/dx and dW/dy of rho and p at the cell; dmu_dy (Sutherland):
+// the y derivative of the viscosity at the cell
 template <int ORDER, bool SQ, bool SUTH>
-__device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
-                                        const real (&dyx)[2][ORDER + 1],
-                                        const double* __restrict__ W,
-                                        const double* __restrict__ Dx,
-                                        const double* __restrict__ Dy, long c, const Geo& g,
-                                        const Phys& ph, real& ene, real& x_mom,
-                                        real& y_mom) {
+__device__ __forceinline__ void viscous_v(const real (&wx)[4][ORDER + 1],
+                                          const real (&dyx)[2][ORDER + 1],
+                                          const double (&dxy)[2][ORDER + 1], double dxr,
+                                          double dxp, double dyr, double dyp, real dmu_dy_c,
+                                          const Geo& g, const Phys& ph, real& ene, real& x_mom,
+                                          real& y_mom) {
     constexpr int H   = ORDER / 2;
-    const long    P   = g.P;
     const int     rho = 0, p = 1, u = 2, v = 3;
     // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79); when
     // dx*dx == dy*dy the two sums are the same operations on the same operands
@@ -1178,17 +1181,13 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
         dp_dydy   = fd_sum<ORDER, true>([&](int t) { return wx[p][t]; }, g.dyy, g.idyy);
     }
     // cross derivatives: dW_dxdy = df_dy(dW_dx), dW_dydx = df_dx(dW_dy)
-    const double* Dxu     = Dx + u * P + c;
-    const double* Dxv     = Dx + v * P + c;
-    real        du_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxu[(long)(t - H) * g.sy]; },
-                                          g.dy, g.idy);
-    real dv_dxdy = fd_sum<ORDER, false>([&](int t) { return Dxv[(long)(t - H) * g.sy]; }, g.dy,
-                                          g.idy);
+    real du_dxdy = fd_sum<ORDER, false>([&](int t) { return dxy[0][t]; }, g.dy, g.idy);
+    real dv_dxdy = fd_sum<ORDER, false>([&](int t) { return dxy[1][t]; }, g.dy, g.idy);
     real du_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[0][t]; }, g.dx, g.idx);
     real dv_dydx = fd_sum<ORDER, false>([&](int t) { return dyx[1][t]; }, g.dx, g.idx);
 
-    real du_dx = Dxu[0], dv_dx = Dxv[0], drho_dx = Dx[rho * P + c], dp_dx = Dx[p * P + c];
-    real du_dy = dyx[0][H], dv_dy = dyx[1][H], drho_dy = Dy[rho * P + c], dp_dy = Dy[p * P + c];
+    real du_dx = dxy[0][H], dv_dx = dxy[1][H], drho_dx = dxr, dp_dx = dxp;
+    real du_dy = dyx[0][H], dv_dy = dyx[1][H], drho_dy = dyr, dp_dy = dyp;
     real U = wx[u][H], V = wx[v][H], Pp = wx[p][H], RHO = wx[rho][H];
 
     real Dtauxx_dx, Dtauyy_dy, Dtauxy_dx, Dtauyx_dy, Tauxx, Tauxy, mu, dmu_dx = 0, dmu_dy = 0;
@@ -1205,14 +1204,10 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
     } else {
         // viscosity and its first derivatives from the primitive stencils; the full expressions
         // of diffusion_scheme.cpp:3-58 with the dmu / dlamb terms (centered_diffusion.cpp:96-145)
-        const double* Wr = W + rho * P + c;
-        const double* Wp = W + p * P + c;
         mu     = mu_suth(RHO, Pp, ph);
         dmu_dx = fd_sum<ORDER, false>([&](int t) { return mu_suth(wx[rho][t], wx[p][t], ph); },
                                       g.dx, g.idx);
-        dmu_dy = fd_sum<ORDER, false>(
-            [&](int t) { return mu_suth(Wr[(long)(t - H) * g.sy], Wp[(long)(t - H) * g.sy], ph); },
-            g.dy, g.idy);
+        dmu_dy = dmu_dy_c;
         real lamb     = (-RL(2.) / RL(3.)) * mu;
         real dlamb_dx = -(RL(2.) / RL(3.)) * dmu_dx;
         real dlamb_dy = -(RL(2.) / RL(3.)) * dmu_dy;
@@ -1256,6 +1251,48 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
     x_mom = -Dtauxx_dx - Dtauyx_dy;
     y_mom = -Dtauyy_dy - Dtauxy_dx;
     ene   = -Dtauvx_dx - Dtauvy_dy + Dqx_dx + Dqy_dy;
+}
+
+// the y derivative of the Sutherland viscosity at a cell; f(t) returns (rho, p) at y offset t - H
+template <int ORDER, class F>
+__device__ __forceinline__ real dmu_dy_of(const F& f, const Geo& g, const Phys& ph) {
+    return fd_sum<ORDER, false>(
+        [&](int t) {
+            double r, q;
+            f(t, r, q);
+            return mu_suth(r, q, ph);
+        },
+        g.dy, g.idy);
+}
+
+// viscous_v with its inputs read from the derivative planes Dx, Dy (4 planes each: rho, p, u, v)
+// and, for the Sutherland law, the y stencil of rho and p from W
+template <int ORDER, bool SQ, bool SUTH>
+__device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
+                                        const real (&dyx)[2][ORDER + 1],
+                                        const double* __restrict__ W,
+                                        const double* __restrict__ Dx,
+                                        const double* __restrict__ Dy, long c, const Geo& g,
+                                        const Phys& ph, real& ene, real& x_mom,
+                                        real& y_mom) {
+    constexpr int H = ORDER / 2;
+    const long    P = g.P;
+    double        dxy[2][ORDER + 1];
+#pragma unroll
+    for (int t = 0; t <= ORDER; t++) {
+        dxy[0][t] = Dx[2 * P + c + (long)(t - H) * g.sy];
+        dxy[1][t] = Dx[3 * P + c + (long)(t - H) * g.sy];
+    }
+    real dmu_dy = 0;
+    if constexpr (SUTH)
+        dmu_dy = dmu_dy_of<ORDER>(
+            [&](int t, double& r, double& q) {
+                r = W[c + (long)(t - H) * g.sy];
+                q = W[P + c + (long)(t - H) * g.sy];
+            },
+            g, ph);
+    viscous_v<ORDER, SQ, SUTH>(wx, dyx, dxy, Dx[c], Dx[P + c], Dy[c], Dy[P + c], dmu_dy, g, ph,
+                               ene, x_mom, y_mom);
 }
 
 // residual + stage update, marching along i:
@@ -1410,6 +1447,200 @@ __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
     cell_tile(bx, by, bz);
     residual_body<ORDER, SQ, SUTH, VC>(fx, fy, fz, W, Dx, Dy, Rin, Rout, st, dtmin, which, g, ph,
                                        bx, by, bz, KW_CELL, MARCH);
+}
+
+// ------------------------------------------------------------------------------------------
+// residual + stage update of a single block with the y derivatives of the 2-D viscous terms
+// formed in the same kernel.  The separate path writes dW/dx and dW/dy of (rho, p, u, v) to 8
+// derivative planes, fills their halos and reads them back in the residual; with this kernel
+// k_deriv1 writes only the 4 dW/dx planes (an x march, no y stencil) and the residual forms
+// dW/dy itself from the primitives of the neighbouring rows (L1 / L2 hits: the rows belong to the
+// other waves of the block and to the neighbouring tiles of the same XCD): dW/dy of u and v on
+// the plane entering the x window, of rho and p at the cell.  Each is the sum of deriv1_body over
+// the same operands, and on the x-halo planes the window reads, the value the halo exchange of
+// the dW/dy planes puts there (centered_diffusion.cpp:22-24): across a periodic x the
+// derivative at the wrapped interior plane, beyond a wall zero.  The results are bitwise those
+// of k_deriv1 + k_deriv_halo + k_residual.  Offsets within a plane are 32-bit (the host runs
+// this kernel only for planes below 2^31 doubles): uniform plane bases plus one offset register
+// per access.
+// ------------------------------------------------------------------------------------------
+#ifndef NS_FUSED_WAVES
+#define NS_FUSED_WAVES 3  // waves per SIMD the fused kernel is compiled for (CD2 / CD4)
+#endif
+// the sixth-order stencils and the Sutherland terms on unequal dx, dy need more registers than
+// three waves per SIMD leave: two there
+template <int ORDER, bool SQ, bool SUTH>
+constexpr int fused_waves() { return (ORDER >= 6 || (SUTH && !SQ)) ? 2 : NS_FUSED_WAVES; }
+#ifndef NS_FUSE_DIFF
+#define NS_FUSE_DIFF 1  // single blocks run k_residual_fd for the 2-D viscous terms
+#endif
+
+template <int ORDER, bool SQ, bool SUTH>
+__global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_residual_fd(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
+    double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
+    int which, Geo g, Phys ph, Bnd bnd) {
+    constexpr int H = ORDER / 2;
+    int           bx, by, bz;
+    cell_tile(bx, by, bz);
+    const int tid = threadIdx.x + threadIdx.y * 64;
+    const int k   = (tid & (KW_CELL - 1)) + bx * KW_CELL;
+    const int j   = tid / KW_CELL + by * (256 / KW_CELL);
+    const int i0  = bz * MARCH;
+    if (k >= g.nz || j >= g.ny) return;
+    const int       i1   = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const long      P    = g.P;
+    const unsigned  sx   = (unsigned)g.sx, sy = (unsigned)g.sy;
+    const unsigned  row  = (unsigned)(j + NGC) * sy + (unsigned)(k + NGC);
+    const bool      xper = bnd.periodic[0] != 0;
+    const bool      conv = (which & 1) && ph.conv;
+    const double*   Wr = W;
+    const double*   Wp = W + P;
+    const double*   Wu = W + 2 * P;
+    const double*   Wv = W + 3 * P;
+    double wd[4][2 * H + 1];   // primitives of row j at x offsets -H .. H
+    real   dyx[2][2 * H + 1];  // dW/dy of u, v at x offsets -H .. H
+    double fxm[5];
+#pragma unroll
+    for (int t = 0; t < 2 * H; t++) {
+        const int      e = i0 - H + t;
+        const unsigned a = row + (unsigned)(e + NGC) * sx;
+#pragma unroll
+        for (int q = 0; q < 4; q++) wd[q][t] = (W + q * P)[a];
+    }
+    // dW/dy of u (a = 0), v (a = 1) on plane e: zero beyond an x wall, the wrapped plane's
+    // across a periodic x
+    auto dy_uv = [&](int e, int a) -> real {
+        if (!xper && (e < 0 || e >= g.nx)) return real(0.0);
+        const int      ep = (e < 0) ? e + g.nx : (e >= g.nx ? e - g.nx : e);
+        const unsigned b  = row + (unsigned)(ep + NGC) * sx;
+        const double*  f  = a ? Wv : Wu;
+        return (real)fd_sum_t<ORDER, false, double>(
+            [&](int t) { return f[b + (t - H) * (int)sy]; }, g.dy, g.idy);
+    };
+#pragma unroll
+    for (int t = 0; t < 2 * H; t++) {
+        dyx[0][t] = dy_uv(i0 - H + t, 0);
+        dyx[1][t] = dy_uv(i0 - H + t, 1);
+    }
+    if (conv) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) fxm[q] = (fx + q * P)[row + (unsigned)(i0 + NGC - 1) * sx];
+    }
+    double dt = 0, m1 = 0, m2 = 0, m3 = 0;
+    if (st.Uout) {
+        dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
+        if (st.stage == 0) {
+            m1 = 1.0, m2 = 0.0, m3 = -dt;
+        } else if (st.stage == 1) {
+            m1 = 3.0 / 4.0, m2 = 1.0 / 4.0, m3 = -(1. / 4.) * dt;
+        } else {
+            m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
+        }
+    }
+    for (int i = i0; i < i1; i++) {
+        const unsigned cc = row + (unsigned)(i + NGC) * sx;
+        double         R[5];
+        bool           have[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            have[q] = (Rin != nullptr);
+            R[q]    = Rin ? (Rin + q * P)[cc] : 0.0;
+        }
+        if (conv) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                double fc = (fx + q * P)[cc];
+#if !NS_FAST
+                double a = dv(fc - fxm[q], g.dx);
+                double b = dv((fy + q * P)[cc] - (fy + q * P)[cc - sy], g.dy);
+#else
+                double a = (fc - fxm[q]) * g.idx;
+                double b = ((fy + q * P)[cc] - (fy + q * P)[cc - sy]) * g.idy;
+#endif
+                fxm[q] = fc;
+                R[q]   = have[q] ? R[q] + a : a;
+                R[q] += b;
+                if (g.dim == 3) {
+#if !NS_FAST
+                    R[q] += dv((fz + q * P)[cc] - (fz + q * P)[cc - 1], g.dz);
+#else
+                    R[q] += ((fz + q * P)[cc] - (fz + q * P)[cc - 1]) * g.idz;
+#endif
+                }
+                have[q] = true;
+            }
+        }
+        {
+            // the plane entering the windows
+            const unsigned cn = cc + (unsigned)H * sx;
+#pragma unroll
+            for (int q = 0; q < 4; q++) wd[q][2 * H] = (W + q * P)[cn];
+            dyx[0][2 * H] = dy_uv(i + H, 0);
+            dyx[1][2 * H] = dy_uv(i + H, 1);
+            // dW/dx from the derivative planes: rho, p, u, v at the cell, u and v on the rows
+            // j - H .. j + H (their halos filled like the separate path's)
+            double dxy[2][2 * H + 1];
+#pragma unroll
+            for (int t = 0; t <= 2 * H; t++) {
+                dxy[0][t] = (Dx + 2 * P)[cc + (t - H) * (int)sy];
+                dxy[1][t] = (Dx + 3 * P)[cc + (t - H) * (int)sy];
+            }
+            const double dxr = Dx[cc], dxp = (Dx + P)[cc];
+            // dW/dy of rho and p at the cell, and of the viscosity
+            double dyr = fd_sum_t<ORDER, false, double>(
+                [&](int t) { return Wr[cc + (t - H) * (int)sy]; }, g.dy, g.idy);
+            double dyp = fd_sum_t<ORDER, false, double>(
+                [&](int t) { return Wp[cc + (t - H) * (int)sy]; }, g.dy, g.idy);
+            real dmu_dy = 0;
+            if constexpr (SUTH)
+                dmu_dy = dmu_dy_of<ORDER>(
+                    [&](int t, double& r, double& q) {
+                        r = Wr[cc + (t - H) * (int)sy];
+                        q = Wp[cc + (t - H) * (int)sy];
+                    },
+                    g, ph);
+            real wx[4][2 * H + 1];
+#pragma unroll
+            for (int t = 0; t <= 2 * H; t++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) wx[q][t] = (real)wd[q][t];
+            real ene, xm, ym;
+            viscous_v<ORDER, SQ, SUTH>(wx, dyx, dxy, dxr, dxp, dyr, dyp, dmu_dy, g, ph, ene, xm,
+                                       ym);
+            R[1]    = have[1] ? R[1] + ene : ene;
+            R[2]    = have[2] ? R[2] + xm : xm;
+            R[3]    = have[3] ? R[3] + ym : ym;
+            have[1] = have[2] = have[3] = true;
+#pragma unroll
+            for (int t = 0; t < 2 * H; t++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) wd[q][t] = wd[q][t + 1];
+                dyx[0][t] = dyx[0][t + 1];
+                dyx[1][t] = dyx[1][t + 1];
+            }
+        }
+        if (st.Uout == nullptr) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) (Rout + q * P)[cc] = have[q] ? R[q] : 0.0;
+            continue;
+        }
+        double un[5], w[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double r = have[q] ? R[q] : 0.0;
+#if !NS_FAST
+            un[q] = ((st.Un + q * P)[cc] * m1) + ((st.Ucur + q * P)[cc] * m2) + (r * m3);
+#else
+            un[q] = (st.Un + q * P)[cc] * m1 + (st.Ucur + q * P)[cc] * m2 + r * m3;
+#endif
+            (st.Uout + q * P)[cc] = un[q];
+        }
+        c2p(un, ph.gamma, w);
+#pragma unroll
+        for (int q = 0; q < 5; q++) (st.Wout + q * P)[cc] = w[q];
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2001,15 +2232,21 @@ static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hi
     const dim3 blk(64, DERIV_ROWS, 1);
     const dim3 grd((g.nz + 63) / 64, (g.ny + DERIV_ROWS - 1) / DERIV_ROWS,
                    (g.nx + MARCH - 1) / MARCH);
-    if (ph.diff_order == 2) k_deriv1<2><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-    else if (ph.diff_order == 4) k_deriv1<4><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-    else k_deriv1<6><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+    if (Dy) {
+        if (ph.diff_order == 2) k_deriv1<2, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        else if (ph.diff_order == 4) k_deriv1<4, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        else k_deriv1<6, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+    } else {
+        if (ph.diff_order == 2) k_deriv1<2, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        else if (ph.diff_order == 4) k_deriv1<4, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        else k_deriv1<6, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+    }
 }
 
 static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s) {
     int  m1 = g.nx > g.ny ? g.nx : g.ny;
     dim3 blk(64, 4, 1);
-    dim3 grd((g.nz + 63) / 64, (m1 + 3) / 4, 4 * NGC);
+    dim3 grd((g.nz + 63) / 64, (m1 + 3) / 4, (Dy ? 4 : 2) * NGC);  // without Dy: its halo only
     k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b);
 }
 
@@ -2032,6 +2269,41 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
             k_residual<ORDER, true, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
         else
             k_residual<ORDER, false, false><<<grd, BLK, 0, s>>>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph);
+    }
+}
+
+template <int ORDER>
+static void residual_fd_order(const double* fx, const double* fy, const double* fz,
+                              const double* W, const double* Dx, const double* Rin, double* Rout,
+                              StageArgs a,
+                              const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
+                              hipStream_t s) {
+    const dim3 blk(64, 4, 1);
+    const dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
+                   (g.nx + MARCH - 1) / MARCH);
+    if (ph.suth) {
+        if (g.square_xy)
+            k_residual_fd<ORDER, true, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+        else
+            k_residual_fd<ORDER, false, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+    } else {
+        if (g.square_xy)
+            k_residual_fd<ORDER, true, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+        else
+            k_residual_fd<ORDER, false, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+    }
+}
+
+static void l_residual_fd(const double* fx, const double* fy, const double* fz, const double* W,
+                          const double* Dx, const double* Rin, double* Rout, const StageArgs* st,
+                          const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
+                          hipStream_t s) {
+    StageArgs a{};
+    if (st) a = *st;
+    switch (ph.diff_order) {
+    case 2: residual_fd_order<2>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
+    case 4: residual_fd_order<4>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
+    default: residual_fd_order<6>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
     }
 }
 
@@ -2175,7 +2447,8 @@ const Launchers& launchers() {
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
                                 l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
-                                l_mlp_recon,    l_set_dt,     l_visc3d,     l_p2c_faces};
+                                l_mlp_recon,    l_set_dt,     l_visc3d,     l_p2c_faces,
+                                l_residual_fd,  NS_FUSE_DIFF};
     return L;
 }
 

@@ -198,6 +198,12 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W) {
     }
 }
 
+// the residual kernel forms the y derivatives of the 2-D viscous terms itself
+bool fused_diffusion(const nsgpu_ctx* c) {
+    return c->L->fuse_diff && c->ph.diff_order && !(c->ph.physics & NSGPU_PHYS_VISCOUS) &&
+           c->g.P < (1L << 31);
+}
+
 void diffusion_derivs(nsgpu_ctx* c, const double* W) {
     if (!c->ph.diff_order) return;
     Scope s(c, T_DERIV);
@@ -205,14 +211,26 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W) {
         c->L->visc3d(W, c->Dx, c->Dy, c->Dz, c->Rv, c->g, c->ph, c->b, c->stream);
         return;
     }
-    c->L->deriv1(W, c->Dx, c->Dy, c->g, c->ph, c->stream);
-    c->L->deriv_halo(c->Dx, c->Dy, c->g, c->b, c->stream);
+    double* Dy = fused_diffusion(c) ? nullptr : c->Dy;
+    c->L->deriv1(W, c->Dx, Dy, c->g, c->ph, c->stream);
+    c->L->deriv_halo(c->Dx, Dy, c->g, c->b, c->stream);
 }
 
 // what the residual reads as its viscous input: the derivative planes, or the precomputed full
 // 3-D viscous terms
 const double* visc_in(const nsgpu_ctx* c) {
     return (c->ph.physics & NSGPU_PHYS_VISCOUS) ? c->Rv : c->Dx;
+}
+
+// residual (+ stage update) of the single block on the face fluxes F and the primitives W
+void residual(nsgpu_ctx* c, const double* W, const double* Rin, double* Rout, const StageArgs* a,
+              const unsigned long long* dtmin, int which) {
+    if ((which & NSGPU_RHS_DIFFUSION) && fused_diffusion(c))
+        c->L->residual_fd(c->F[0], c->F[1], c->F[2], W, c->Dx, Rin, Rout, a, dtmin, which, c->g,
+                          c->ph, c->b, c->stream);
+    else
+        c->L->residual(c->F[0], c->F[1], c->F[2], W, visc_in(c), c->Dy, Rin, Rout, a, dtmin,
+                       which, c->g, c->ph, c->stream);
 }
 
 }  // namespace
@@ -391,8 +409,7 @@ int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_ao
     convection_fluxes(c, c->Ub, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, c->R, c->R, nullptr,
-                       nullptr, NSGPU_RHS_CONVECTION, c->g, c->ph, c->stream);
+        residual(c, c->W, c->R, c->R, nullptr, nullptr, NSGPU_RHS_CONVECTION);
     }
     return download_soa(c, c->R, R_aos);
 }
@@ -406,8 +423,7 @@ int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
     diffusion_derivs(c, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, c->R, c->R, nullptr,
-                       nullptr, NSGPU_RHS_DIFFUSION, c->g, c->ph, c->stream);
+        residual(c, c->W, c->R, c->R, nullptr, nullptr, NSGPU_RHS_DIFFUSION);
     }
     return download_soa(c, c->R, R_aos);
 }
@@ -429,8 +445,7 @@ int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int whi
     if (which & 2) diffusion_derivs(c, c->W);
     {
         Scope s(c, T_RESIDUAL);
-        c->L->residual(c->F[0], c->F[1], c->F[2], c->W, visc_in(c), c->Dy, nullptr, c->R, nullptr,
-                       nullptr, which, c->g, c->ph, c->stream);
+        residual(c, c->W, nullptr, c->R, nullptr, nullptr, which);
     }
     rc = download_soa(c, c->Ua, U_aos);
     if (!rc && W_out) rc = download_soa(c, c->W, W_out);
@@ -484,8 +499,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st};
             {
                 Scope sc(c, T_RESIDUAL);
-                L.residual(c->F[0], c->F[1], c->F[2], wcur, visc_in(c), c->Dy, nullptr, nullptr, &a,
-                           &c->dtmin[0], NSGPU_RHS_BOTH, c->g, c->ph, c->stream);
+                residual(c, wcur, nullptr, nullptr, &a, &c->dtmin[0], NSGPU_RHS_BOTH);
             }
             double* t = cur;
             cur       = nxt;

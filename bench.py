@@ -25,6 +25,9 @@ sys.path.insert(0, ROOT)
 FLOP_PER_CELL_STEP = {"weno5cw": 12050.0, "weno5": 7860.0}
 FLOP_PER_FACE = {"weno5cw": 1190.0, "weno5": 725.0}
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
+# planes the residual + stage-update kernel moves per launch, each once: 33 read (fx, fy, fz, Un,
+# Ucur: 5 each; W, dW/dx: 4 each; it forms dW/dy itself) and 10 written (Uout, Wout)
+RES_PLANES = 43
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense, with packed FP32 (which this build leaves off)
 
@@ -119,7 +122,7 @@ def precision_workload(args):
     flop = FLOP_PER_FACE["weno5cw"] * (n + 1) * n * n
     fp32_tf = flop / (dom_ms * 1e-3) / 1e12
     res_ms = tm["residual_update"][0] / tm["residual_update"][1]
-    res_gbs = 47 * 8.0 * n ** 3 / (res_ms * 1e-3) / 1e9
+    res_gbs = RES_PLANES * 8.0 * n ** 3 / (res_ms * 1e-3) / 1e9
     fp64_dom = max((k for k in out["exact_timers"] if k.startswith("face_flux")),
                    key=lambda k: out["exact_timers"][k][0])
     fp64_ms = out["exact_timers"][fp64_dom][0] / out["exact_timers"][fp64_dom][1]
@@ -394,10 +397,9 @@ def main():
         tr = json.load(open(tpath))
         traffic = tr.get(f"{dom}_{args.math}_{n}")
     step_frac = value * 1e9 * FLOP_PER_CELL_STEP[args.conv] / (FP64_PEAK_TFLOPS * 1e12)
-    # the residual + stage-update kernel is HBM-bound: 37 planes read (fx, fy, fz, Un, Ucur:
-    # 5 each; W, Dx, Dy: 4 each) and 10 written (Uout, Wout) per launch, each once
+    # the residual + stage-update kernel is HBM-bound
     res_ms, res_n = timers["residual_update"]
-    res_bytes = 47 * 8.0 * cells
+    res_bytes = RES_PLANES * 8.0 * cells
     suffix = "" if args.viscosity == "constant" else "_" + args.viscosity
     res_traffic = tr.get(f"residual_update_{args.math}_{n}{suffix}") if os.path.exists(tpath) else None
     res_achieved = res_bytes / (res_ms / res_n * 1e-3) / 1e9
