@@ -147,7 +147,7 @@ struct Launchers {
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
                       hipStream_t s);
-    // Dy null: dW/dx and its halo only (for residual_fd)
+    // Dy null: dW/dx of u and v (Dx planes 2, 3) and their halo only (for residual_fd)
     void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
     void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);
     // R (interior) = R_in (or 0) + convection(fx,fy,fz) + diffusion(W, Dx, Dy); then either

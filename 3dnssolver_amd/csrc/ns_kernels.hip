@@ -1041,11 +1041,13 @@ constexpr int KW_CELL = NS_KW_CELL;  // lanes along k of the single-block cell k
 // k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of `march` planes i).  FP64 in
 // every build: the kernel is bound by its HBM traffic, and single-precision arithmetic on FP64
 // planes only adds conversions (it ran 1.5x slower)
+// DY = false: dW/dx of u and v only (planes 2, 3 of Dx; the residual kernel forms the rest)
 template <int ORDER, bool DY = true>
 __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double* __restrict__ Dx,
                                             double* __restrict__ Dy, const Geo& g, int bx, int by,
                                             int bz, int kw, int march, int nthreads = 256) {
     constexpr int H   = ORDER / 2;
+    constexpr int Q0  = DY ? 0 : 2;  // first variable formed
     const int     tid = threadIdx.x + threadIdx.y * 64;
     const int     k   = (tid & (kw - 1)) + bx * kw;
     const int     j   = tid / kw + by * (nthreads / kw);
@@ -1057,13 +1059,13 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
 #pragma unroll
     for (int t = 0; t < 2 * H; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) win[q][t] = W[q * g.P + c0 + (long)(i0 + NGC - H + t) * g.sx];
+        for (int q = Q0; q < 4; q++) win[q][t] = W[q * g.P + c0 + (long)(i0 + NGC - H + t) * g.sx];
     for (int i = i0; i < i1; i++) {
         const long c = c0 + (long)(i + NGC) * g.sx;
 #pragma unroll
-        for (int q = 0; q < 4; q++) win[q][2 * H] = W[q * g.P + c + (long)H * g.sx];
+        for (int q = Q0; q < 4; q++) win[q][2 * H] = W[q * g.P + c + (long)H * g.sx];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = Q0; q < 4; q++) {
             const double* fy = W + q * g.P + c;
             Dx[c + q * g.P] =
                 fd_sum_t<ORDER, false, double>([&](int t) { return win[q][t]; }, g.dx, g.idx);
@@ -1072,7 +1074,7 @@ __device__ __forceinline__ void deriv1_body(const double* __restrict__ W, double
                     [&](int t) { return fy[(long)(t - H) * g.sy]; }, g.dy, g.idy);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++)
+        for (int q = Q0; q < 4; q++)
 #pragma unroll
             for (int t = 0; t < 2 * H; t++) win[q][t] = win[q][t + 1];
     }
@@ -1100,7 +1102,7 @@ __device__ __forceinline__ void cell_tile(int& bx, int& by, int& bz) {
 #define NS_DERIV_ROWS 4  // rows j per block of the single-block derivative kernel (64 k each)
 #endif
 constexpr int DERIV_ROWS = NS_DERIV_ROWS;
-// DY = false: dW/dx only (the residual kernel forms dW/dy itself)
+// DY = false: dW/dx of u and v only (the residual kernel forms the rest)
 template <int ORDER, bool DY>
 __global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
@@ -1115,7 +1117,7 @@ __global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const
 // stay zero: the arrays are zeroed once at allocation and never written there.
 // block (64, 4); (bx, by, bz) = (k tile, row tile, (side, low/high, layer))
 __device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Geo& g,
-                                                const Bnd& b, int bx, int by, int bz) {
+                                                const Bnd& b, int bx, int by, int bz, int q0 = 0) {
     int  k    = threadIdx.x + bx * 64;
     int  t1   = threadIdx.y + by * 4;
     int  side = bz / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
@@ -1127,7 +1129,7 @@ __device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Ge
         long base = (long)(t1 + NGC) * g.sx + (k + NGC);
         int  dst  = low ? h : NGC + g.ny + h;
         int  src  = low ? g.ny + h : NGC + h;
-        for (int q = 0; q < 4; q++)
+        for (int q = q0; q < 4; q++)
             Dx[base + (long)dst * g.sy + q * g.P] = Dx[base + (long)src * g.sy + q * g.P];
     } else {
         if (!b.periodic[0] || t1 >= g.ny) return;
@@ -1139,8 +1141,9 @@ __device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Ge
     }
 }
 
+// Dy null: the j halos of dW/dx of u and v only
 __global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
-    deriv_halo_body(Dx, Dy, g, b, blockIdx.x, blockIdx.y, blockIdx.z);
+    deriv_halo_body(Dx, Dy, g, b, blockIdx.x, blockIdx.y, blockIdx.z, Dy ? 0 : 2);
 }
 
 // CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
@@ -1453,9 +1456,10 @@ __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
 // residual + stage update of a single block with the y derivatives of the 2-D viscous terms
 // formed in the same kernel.  The separate path writes dW/dx and dW/dy of (rho, p, u, v) to 8
 // derivative planes, fills their halos and reads them back in the residual; with this kernel
-// k_deriv1 writes only the 4 dW/dx planes (an x march, no y stencil) and the residual forms
-// dW/dy itself from the primitives of the neighbouring rows (L1 / L2 hits: the rows belong to the
-// other waves of the block and to the neighbouring tiles of the same XCD): dW/dy of u and v on
+// k_deriv1 writes only dW/dx of u and v (an x march, no y stencil), which the cross derivatives
+// read on the rows j +- 1 .. H, and the residual forms the rest itself: dW/dx at the cell from its
+// x window, and dW/dy from the primitives of the neighbouring rows (L1 / L2 hits: the rows belong
+// to the other waves of the block and to the neighbouring tiles of the same XCD), of u and v on
 // the plane entering the x window, of rho and p at the cell.  Each is the sum of deriv1_body over
 // the same operands, and on the x-halo planes the window reads, the value the halo exchange of
 // the dW/dy planes puts there (centered_diffusion.cpp:22-24): across a periodic x the
@@ -1467,10 +1471,10 @@ __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
 #ifndef NS_FUSED_WAVES
 #define NS_FUSED_WAVES 3  // waves per SIMD the fused kernel is compiled for (CD2 / CD4)
 #endif
-// the sixth-order stencils and the Sutherland terms on unequal dx, dy need more registers than
+// the sixth-order stencils and the second derivatives on unequal dx, dy need more registers than
 // three waves per SIMD leave: two there
 template <int ORDER, bool SQ, bool SUTH>
-constexpr int fused_waves() { return (ORDER >= 6 || (SUTH && !SQ)) ? 2 : NS_FUSED_WAVES; }
+constexpr int fused_waves() { return (ORDER >= 6 || !SQ) ? 2 : NS_FUSED_WAVES; }
 #ifndef NS_FUSE_DIFF
 #define NS_FUSE_DIFF 1  // single blocks run k_residual_fd for the 2-D viscous terms
 #endif
@@ -1579,15 +1583,23 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
             for (int q = 0; q < 4; q++) wd[q][2 * H] = (W + q * P)[cn];
             dyx[0][2 * H] = dy_uv(i + H, 0);
             dyx[1][2 * H] = dy_uv(i + H, 1);
-            // dW/dx from the derivative planes: rho, p, u, v at the cell, u and v on the rows
-            // j - H .. j + H (their halos filled like the separate path's)
+            // dW/dx: at the cell from the window, of u and v on the rows j +- 1 .. H from the
+            // derivative planes (their halos filled like the separate path's)
+            double dxo[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                dxo[q] = fd_sum_t<ORDER, false, double>([&](int t) { return wd[q][t]; }, g.dx,
+                                                        g.idx);
             double dxy[2][2 * H + 1];
 #pragma unroll
             for (int t = 0; t <= 2 * H; t++) {
+                if (t == H) continue;
                 dxy[0][t] = (Dx + 2 * P)[cc + (t - H) * (int)sy];
                 dxy[1][t] = (Dx + 3 * P)[cc + (t - H) * (int)sy];
             }
-            const double dxr = Dx[cc], dxp = (Dx + P)[cc];
+            dxy[0][H]        = dxo[2];
+            dxy[1][H]        = dxo[3];
+            const double dxr = dxo[0], dxp = dxo[1];
             // dW/dy of rho and p at the cell, and of the viscosity
             double dyr = fd_sum_t<ORDER, false, double>(
                 [&](int t) { return Wr[cc + (t - H) * (int)sy]; }, g.dy, g.idy);

@@ -25,9 +25,10 @@ sys.path.insert(0, ROOT)
 FLOP_PER_CELL_STEP = {"weno5cw": 12050.0, "weno5": 7860.0}
 FLOP_PER_FACE = {"weno5cw": 1190.0, "weno5": 725.0}
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
-# planes the residual + stage-update kernel moves per launch, each once: 33 read (fx, fy, fz, Un,
-# Ucur: 5 each; W, dW/dx: 4 each; it forms dW/dy itself) and 10 written (Uout, Wout)
-RES_PLANES = 43
+# planes the residual + stage-update kernel moves per launch, each once: 31 read (fx, fy, fz, Un,
+# Ucur: 5 each; W: 4; dW/dx of u, v: 2; it forms the other derivatives itself) and 10 written
+# (Uout, Wout)
+RES_PLANES = 41
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense, with packed FP32 (which this build leaves off)
 
