@@ -1475,25 +1475,30 @@ __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
 // three waves per SIMD leave: two there
 template <int ORDER, bool SQ, bool SUTH>
 constexpr int fused_waves() { return (ORDER >= 6 || !SQ) ? 2 : NS_FUSED_WAVES; }
+#ifndef NS_FUSED_MARCH
+#define NS_FUSED_MARCH NS_MARCH  // planes i per block of the fused kernel
+#endif
+constexpr int FUSED_MARCH = NS_FUSED_MARCH;
 #ifndef NS_FUSE_DIFF
 #define NS_FUSE_DIFF 1  // single blocks run k_residual_fd for the 2-D viscous terms
 #endif
 
+// 256 threads: kw lanes along k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of
+// `march` planes i)
 template <int ORDER, bool SQ, bool SUTH>
-__global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_residual_fd(
+__device__ __forceinline__ void residual_fd_body(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
-    double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
-    int which, Geo g, Phys ph, Bnd bnd) {
-    constexpr int H = ORDER / 2;
-    int           bx, by, bz;
-    cell_tile(bx, by, bz);
-    const int tid = threadIdx.x + threadIdx.y * 64;
-    const int k   = (tid & (KW_CELL - 1)) + bx * KW_CELL;
-    const int j   = tid / KW_CELL + by * (256 / KW_CELL);
-    const int i0  = bz * MARCH;
+    double* __restrict__ Rout, const StageArgs& st, const unsigned long long* __restrict__ dtmin,
+    int which, const Geo& g, const Phys& ph, const Bnd& bnd, int bx, int by, int bz, int kw,
+    int march) {
+    constexpr int H   = ORDER / 2;
+    const int     tid = threadIdx.x + threadIdx.y * 64;
+    const int     k   = (tid & (kw - 1)) + bx * kw;
+    const int     j   = tid / kw + by * (256 / kw);
+    const int     i0  = bz * march;
     if (k >= g.nz || j >= g.ny) return;
-    const int       i1   = (i0 + MARCH < g.nx) ? i0 + MARCH : g.nx;
+    const int       i1   = (i0 + march < g.nx) ? i0 + march : g.nx;
     const long      P    = g.P;
     const unsigned  sx   = (unsigned)g.sx, sy = (unsigned)g.sy;
     const unsigned  row  = (unsigned)(j + NGC) * sy + (unsigned)(k + NGC);
@@ -1653,6 +1658,18 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
 #pragma unroll
         for (int q = 0; q < 5; q++) (st.Wout + q * P)[cc] = w[q];
     }
+}
+
+template <int ORDER, bool SQ, bool SUTH>
+__global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_residual_fd(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
+    double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
+    int which, Geo g, Phys ph, Bnd bnd) {
+    int bx, by, bz;
+    cell_tile(bx, by, bz);
+    residual_fd_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin, which, g, ph, bnd,
+                                      bx, by, bz, KW_CELL, FUSED_MARCH);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2068,6 +2085,12 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp);
 }
 
+// ensemble members take the same derivative split as single blocks (k_residual_fd): dW/dx of
+// u and v from the derivative pass, the rest formed in the residual
+__device__ __forceinline__ bool ens_fused(const Geo& g) {
+    return NS_FUSE_DIFF && g.P < (1L << 31);
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(256) void kb_deriv1(EnsTable T, int s) {
     int             bx, by, bz;
@@ -2076,7 +2099,8 @@ __global__ __launch_bounds__(256) void kb_deriv1(EnsTable T, int s) {
     const int       kw = E->t.kw_cell, march = E->t.march;
     const double*   Wp = ens_w(E, s);
     double *Dx = E->Dx, *Dy = E->Dy;
-    deriv1_body<ORDER>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
+    if (ens_fused(g)) deriv1_body<ORDER, false>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
+    else deriv1_body<ORDER>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
 }
 
 __global__ void kb_deriv_halo(EnsTable T) {
@@ -2085,7 +2109,8 @@ __global__ void kb_deriv_halo(EnsTable T) {
     const Geo       g  = E->g;
     const Bnd       bd = E->b;
     double *Dx = E->Dx, *Dy = E->Dy;
-    deriv_halo_body(Dx, Dy, g, bd, bx, by, bz);
+    if (!ens_fused(g)) deriv_halo_body(Dx, Dy, g, bd, bx, by, bz);
+    else if (bz < 2 * NGC) deriv_halo_body(Dx, Dy, g, bd, bx, by, bz, 2);  // j halos of du/dx, dv/dx
 }
 
 // which / Rin / Rout are launch arguments (3, null, null) rather than constants, so that the body
@@ -2103,6 +2128,18 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
     StageArgs     a{E->Un, ens_u(E, s), s == 1 ? E->Ub : E->Ua,
                     s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s};
     const unsigned long long* dtmin = E->dtmin;
+    if constexpr (ORDER > 0) {
+        if (ens_fused(g)) {
+            const Bnd bd = E->b;
+            if (g.square_xy)
+                residual_fd_body<ORDER, true, SUTH>(fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin, which,
+                                                    g, ph, bd, bx, by, bz, kw, march);
+            else
+                residual_fd_body<ORDER, false, SUTH>(fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin,
+                                                     which, g, ph, bd, bx, by, bz, kw, march);
+            return;
+        }
+    }
     if (g.square_xy)
         residual_body<ORDER, true, SUTH>(fx, fy, fz, Wp, Dx, Dy, Rin, Rout, a, dtmin, which, g,
                                          ph, bx, by, bz, kw, march);
@@ -2292,7 +2329,7 @@ static void residual_fd_order(const double* fx, const double* fy, const double* 
                               hipStream_t s) {
     const dim3 blk(64, 4, 1);
     const dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
-                   (g.nx + MARCH - 1) / MARCH);
+                   (g.nx + FUSED_MARCH - 1) / FUSED_MARCH);
     if (ph.suth) {
         if (g.square_xy)
             k_residual_fd<ORDER, true, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
