@@ -464,7 +464,8 @@ template <int DIR, int WT, int Q, bool CORR, class S>
 __device__ __forceinline__ real char_component(const Eig& E, const S& st, const Learn& L) {
     // stencil ii uses fL at cell i-2+ii and fR at cell i-1+ii (weno-5cw.cpp:85-97)
 #if NS_ROW_FENCE
-    __asm__ __volatile__("" ::: "memory");  // re-read the stencil per row: fewer live registers
+    // re-read the stencil per row (per pair of rows with NS_ROW_FENCE 2): fewer live registers
+    if constexpr (NS_ROW_FENCE == 1 || Q % 2 == 0) __asm__ __volatile__("" ::: "memory");
 #endif
     real a[5], b[5];
 #pragma unroll
