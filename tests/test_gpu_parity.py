@@ -90,6 +90,10 @@ def test_steps_match_golden_bitwise(ns):
     ("weno5cw", "borges", 6, ["neumann"] * 6, 3),
     ("weno5", "borges", 2, ["neumann", "neumann", "periodic", "periodic", "neumann", "neumann"], 3),
     ("weno5cw", "shu", 4, ["periodic"] * 6, 2),
+    # periodic x with walls in y (and z): the x-halo planes' dW/dy come from the wrapped plane,
+    # whose y-halo rows are wall copies
+    ("weno5cw", "shu", 4, ["periodic", "periodic", "neumann", "neumann", "periodic", "periodic"], 3),
+    ("weno5", "shu", 6, ["periodic", "periodic", "neumann", "neumann", "neumann", "neumann"], 3),
     ("none", "shu", 4, ["periodic"] * 6, 3),
     ("weno5cw", "shu", 0, ["periodic"] * 6, 3),
 ])
@@ -112,7 +116,9 @@ def test_multi_step_matches_oracle_bitwise(ns, oracle, conv, weights, order, bc,
     s.close()
 
 
-@pytest.mark.parametrize("bc", [["periodic"] * 6, ["neumann"] * 6])
+@pytest.mark.parametrize("bc", [["periodic"] * 6, ["neumann"] * 6,
+                                ["periodic", "periodic", "neumann", "neumann", "periodic", "periodic"],
+                                ["neumann", "neumann", "periodic", "periodic", "periodic", "periodic"]])
 def test_sutherland_matches_oracle_bitwise(ns, oracle, bc):
     # Sutherland viscosity (extension): RHS and steps, characteristic WENO-5 + CD4 and CD6
     C = cases_module()
