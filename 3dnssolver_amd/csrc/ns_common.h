@@ -189,7 +189,7 @@ struct Launchers {
                         const double* Dx, const double* Rin, double* Rout, const StageArgs* st,
                         const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
                         hipStream_t s);
-    int fuse_diff;  // 1: single blocks use residual_fd
+    int fuse_diff;  // 1: single blocks use residual_fd (ensembles take the same split)
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

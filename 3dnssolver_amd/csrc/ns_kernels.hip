@@ -1481,7 +1481,7 @@ constexpr int fused_waves() { return (ORDER >= 6 || !SQ) ? 2 : NS_FUSED_WAVES; }
 #endif
 constexpr int FUSED_MARCH = NS_FUSED_MARCH;
 #ifndef NS_FUSE_DIFF
-#define NS_FUSE_DIFF 1  // single blocks run k_residual_fd for the 2-D viscous terms
+#define NS_FUSE_DIFF 1  // single blocks and ensembles take the derivative split (2-D viscous terms)
 #endif
 
 // 256 threads: kw lanes along k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of
