@@ -145,8 +145,9 @@ struct Launchers {
     // boundary conditions + periodic exchange on nA planes of A and nB planes of B (B may be null)
     void (*halo)(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s);
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
+    // parts: 1 the tile kernel (all faces but the last plane), 2 the last face plane, 3 both
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                      hipStream_t s);
+                      int parts, hipStream_t s);
     // Dy null: dW/dx of u and v (Dx planes 2, 3) and their halo only (for residual_fd)
     void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
     void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);

@@ -2209,10 +2209,12 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
 
+// parts: 1 the tile kernel (faces 0 .. n-1 of the sweep), 2 the last face plane, 3 both
 template <int DIR, int CONV, int B, int OPT>
 static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
-                     const MlpPacked* mlp, hipStream_t s) {
-    if constexpr (DIR == 2) {
+                     const MlpPacked* mlp, int parts, hipStream_t s) {
+    if (!(parts & 1)) {
+    } else if constexpr (DIR == 2) {
         dim3 blk(TILE_L, 8, 1);
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
         k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
@@ -2223,6 +2225,7 @@ static void flux_dir(const double* U, const double* W, double* f, Geo g, double 
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S1 - 1) / TILE_S1);
         k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
     }
+    if (!(parts & 2)) return;
     // the last face plane (sweep index NGC + n - 1)
     int  n  = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
     int  n1 = (DIR == 0) ? g.ny : g.nx;
@@ -2241,41 +2244,41 @@ static int flux_opt(const Phys& ph) {
 
 template <int DIR, int CONV, int B>
 static void flux_dir_opt(const double* U, const double* W, double* f, Geo g, double gamma,
-                         const MlpPacked* m, int opt, hipStream_t s) {
+                         const MlpPacked* m, int opt, int parts, hipStream_t s) {
     if constexpr (B == 2) {
-        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, s);
+        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, s);
     } else if constexpr (CONV == 1) {
-        if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, s);
-        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, s);
+        if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, parts, s);
+        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, s);
     } else {
-        if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, s);
-        else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, s);
-        else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, s);
-        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, s);
+        if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, parts, s);
+        else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, parts, s);
+        else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, parts, s);
+        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, parts, s);
     }
 }
 
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
-                          hipStream_t s) {
+                          int parts, hipStream_t s) {
     const MlpPacked* m   = ph.mlp;
     const int        opt = flux_opt(ph);
     if (ph.conv == 1) {
-        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, s);
-        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, s);
     } else {
-        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, s);
-        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, parts, s);
     }
 }
 
 static void l_face_flux(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                        hipStream_t s) {
-    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, s);
-    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, s);
-    else face_flux_dir<2>(U, W, f, g, ph, s);
+                        int parts, hipStream_t s) {
+    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, parts, s);
+    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, parts, s);
+    else face_flux_dir<2>(U, W, f, g, ph, parts, s);
 }
 
 static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {

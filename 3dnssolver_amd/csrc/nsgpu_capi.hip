@@ -35,6 +35,11 @@ struct nsgpu_ctx {
     nsgpu_config       cfg;
     int                device = 0;
     hipStream_t        stream = nullptr;
+    // second stream of the step: the small launches (last face planes, derivative pass) run on it
+    // beside the face tiles; fork / join events order it against `stream`
+    hipStream_t        aux      = nullptr;
+    hipEvent_t         ev_fork  = nullptr;
+    hipEvent_t         ev_join  = nullptr;
     Geo                g{};
     Bnd                b{};
     Phys               ph{};
@@ -182,19 +187,21 @@ int download_soa(nsgpu_ctx* c, const double* src, double* aos) {
 bool net_ready(const nsgpu_ctx* c) { return c->ph.weights != NSGPU_WEIGHTS_LEARNED || c->ph.mlp; }
 
 // convection face fluxes of the current (U, W); fz stays zero for dim == 2
-void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W) {
+void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts = 3,
+                       hipStream_t st = nullptr) {
     if (!c->ph.conv) return;
+    if (!st) st = c->stream;
     {
         Scope s(c, T_FLUX_X);
-        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, c->stream);
+        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, parts, st);
     }
     {
         Scope s(c, T_FLUX_Y);
-        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, c->stream);
+        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, parts, st);
     }
     if (c->g.dim == 3) {
         Scope s(c, T_FLUX_Z);
-        c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, c->stream);
+        c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, parts, st);
     }
 }
 
@@ -204,16 +211,36 @@ bool fused_diffusion(const nsgpu_ctx* c) {
            c->g.P < (1L << 31);
 }
 
-void diffusion_derivs(nsgpu_ctx* c, const double* W) {
+void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
     if (!c->ph.diff_order) return;
+    if (!st) st = c->stream;
     Scope s(c, T_DERIV);
     if (c->ph.physics & NSGPU_PHYS_VISCOUS) {
-        c->L->visc3d(W, c->Dx, c->Dy, c->Dz, c->Rv, c->g, c->ph, c->b, c->stream);
+        c->L->visc3d(W, c->Dx, c->Dy, c->Dz, c->Rv, c->g, c->ph, c->b, st);
         return;
     }
     double* Dy = fused_diffusion(c) ? nullptr : c->Dy;
-    c->L->deriv1(W, c->Dx, Dy, c->g, c->ph, c->stream);
-    c->L->deriv_halo(c->Dx, Dy, c->g, c->b, c->stream);
+    c->L->deriv1(W, c->Dx, Dy, c->g, c->ph, st);
+    c->L->deriv_halo(c->Dx, Dy, c->g, c->b, st);
+}
+
+// the inputs of a stage's residual: face fluxes and derivatives of (U, W).  The face tiles run on
+// the context's stream, the last face planes and the derivative pass (which write other memory)
+// on the auxiliary stream beside them; the context's stream waits for both before the residual.
+// Timed runs keep everything on one stream so that each kernel family is timed alone.
+void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
+    if (c->timing || !c->aux) {
+        convection_fluxes(c, U, W);
+        diffusion_derivs(c, W);
+        return;
+    }
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->aux, c->ev_fork, 0);
+    convection_fluxes(c, U, W, 2, c->aux);
+    diffusion_derivs(c, W, c->aux);
+    hipEventRecord(c->ev_join, c->aux);
+    convection_fluxes(c, U, W, 1, c->stream);
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
 // what the residual reads as its viscous input: the derivative planes, or the precomputed full
@@ -327,6 +354,12 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     }
     for (int t = 0; t < 2 * T_COUNT && !rc; t++)
         if (hipEventCreate(&c->ev[t]) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc && hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
+    if (!rc && hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        rc = NSGPU_ERR_HIP;
     c->Ulast = c->Ua;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (rc) {
@@ -341,6 +374,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->aux) hipStreamSynchronize(c->aux);
     double* arrays[] = {c->W,     c->W2,   c->Ws[0], c->Ws[1], c->Un, c->Ua, c->Ub,
                         c->F[0],  c->F[1], c->F[2],  c->Dx,    c->Dy, c->Dz, c->Rv,
                         c->R,     c->stage_aos};
@@ -351,6 +385,9 @@ void nsgpu_destroy(nsgpu_ctx* c) {
         if (p) hipFree(p);
     for (int t = 0; t < 2 * T_COUNT; t++)
         if (c->ev[t]) hipEventDestroy(c->ev[t]);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->aux) hipStreamDestroy(c->aux);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -492,8 +529,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
                 Scope sc(c, T_HALO);
                 L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
             }
-            convection_fluxes(c, cur, wcur);
-            diffusion_derivs(c, wcur);
+            stage_inputs(c, cur, wcur);
             // the stage writes U and c2p(U) of the next stage; the last stage writes the
             // primitive state into the other state buffer (the residual reads wcur meanwhile)
             StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st};
