@@ -224,10 +224,12 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
     c->L->deriv_halo(c->Dx, Dy, c->g, c->b, st);
 }
 
-// the inputs of a stage's residual: face fluxes and derivatives of (U, W).  The face tiles run on
-// the context's stream, the last face planes and the derivative pass (which write other memory)
-// on the auxiliary stream beside them; the context's stream waits for both before the residual.
-// Timed runs keep everything on one stream so that each kernel family is timed alone.
+// the inputs of a stage's residual: face fluxes and derivatives of (U, W), which all read U and W
+// and write disjoint memory.  The x and y face tiles run on the context's stream; the z sweep,
+// the last face planes of x and y and the derivative pass run on the auxiliary stream beside
+// them (a z tile block fits beside an x/y tile block on one CU: 44 + 66.5 KB of LDS); the
+// context's stream waits for both before the residual.  Timed runs keep everything on one
+// stream so that each kernel family is timed alone.
 void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
     if (c->timing || !c->aux) {
         convection_fluxes(c, U, W);
@@ -236,10 +238,17 @@ void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
     }
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->aux, c->ev_fork, 0);
-    convection_fluxes(c, U, W, 2, c->aux);
+    if (c->ph.conv && c->g.dim == 3) c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, 3, c->aux);
+    if (c->ph.conv) {
+        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, 2, c->aux);
+        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, 2, c->aux);
+    }
     diffusion_derivs(c, W, c->aux);
     hipEventRecord(c->ev_join, c->aux);
-    convection_fluxes(c, U, W, 1, c->stream);
+    if (c->ph.conv) {
+        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, 1, c->stream);
+        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, 1, c->stream);
+    }
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
