@@ -424,6 +424,44 @@ def test_full_size_512_isotropic_conserves(ns):
         s.close()
 
 
+def test_full_size_256_tgv_200_steps(ns):
+    # the headline workload at its full size and length (Taylor-Green vortex 256^3, Sutherland
+    # viscosity, characteristic WENO-5 + CD4, 200 SSP-RK3 steps, EXACT math): the mass and
+    # momentum sums change by round-off only (periodic flux differences telescope); the state
+    # stays healthy; and the kinetic energy decays by less than 1 % and at no less than half
+    # the vortex's early viscous rate.  The reference's viscous terms act in x and y only
+    # (centered_diffusion.cpp:70-182), so each velocity mode sees |k|^2 = 2 and the energy decays
+    # as exp(-4 nu t) at early times (measured: 5.53e-4 against 4 nu t = 5.58e-4 after 200 steps);
+    # numerical dissipation only adds to it
+    C = cases_module()
+    case, W0 = C.config2(256, viscosity="sutherland")
+    print("[gpu] full size 256^3 TGV, 200 steps", flush=True)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    s0, _, _, _ = s.reduce()
+    t, dt = s.step(200)
+    s1, _, flags, rc = s.reduce()
+    assert rc == 0 and flags == 0, (rc, flags)
+    vol = (2 * np.pi / 256) ** 3
+    scale = 256 ** 3 * vol
+    assert abs(s1[0] - s0[0]) < 1e-12 * scale, s1[0] - s0[0]
+    for q in (2, 3, 4):
+        assert abs(s1[q] - s0[q]) < 1e-12 * scale, (q, s1[q] - s0[q])
+
+    def kinetic(W):
+        w = po.interior(W)
+        return 0.5 * float(np.sum(w[..., 0] * (w[..., 2] ** 2 + w[..., 3] ** 2 + w[..., 4] ** 2)))
+
+    k0 = kinetic(W0)
+    k1 = kinetic(s.get_state("primitive"))
+    nu = 0.1 * C.sound_speed(0.1) / 1600.0  # mu at the initial temperature (M U0 L / Re), rho = 1
+    decay = (k0 - k1) / k0
+    print(f"[gpu] t = {t:.4f}, kinetic energy decay {decay:.3e}, 4 nu t = {4 * nu * t:.3e}",
+          flush=True)
+    assert 0.5 * (4 * nu * t) < decay < 1e-2, (decay, 4 * nu * t)
+    s.close()
+
+
 def test_set_state_derives_the_other_form(ns, oracle):
     # setting W gives U = p2c(W) (get_state, reduce); setting U gives W = c2p(U), and the step
     # then advances that state, as the reference does from W (time_scheme.cpp:25)
