@@ -165,7 +165,9 @@ int nsgpu_reduce(nsgpu_ctx* ctx, double sums[5], double* dt_min, unsigned* flags
  * that validate a new GPU, driver or compiler before trusting bitwise results. */
 int nsgpu_selftest(int device, long n, unsigned long long seed, long* mismatches);
 
-/* The context's HIP stream (a hipStream_t), for callers that time or order work around it. */
+/* The context's HIP stream (a hipStream_t), for callers that time or order work around it.  The
+ * step runs some launches on a second internal stream that joins back into this one within each
+ * stage, so work ordered on this stream after a call sees all of its results. */
 void* nsgpu_stream(nsgpu_ctx* ctx);
 
 /* Per-kernel timing with HIP events on the context's stream: enable, then read the accumulated
