@@ -462,6 +462,29 @@ def test_full_size_256_tgv_200_steps(ns):
     s.close()
 
 
+@pytest.mark.parametrize("math", ["exact", "fp32"])
+def test_step_streams_equal_serial(ns, math):
+    # a step runs the z sweep, the last face planes and the derivative pass on a second stream
+    # beside the x/y face tiles; with per-kernel timing on, everything runs on one stream.  Both
+    # orders give bitwise the same states (walls on x, periodic y and z; Sutherland viscosity)
+    C = cases_module()
+    case = dict(C.base_case(40, 36, 70), conv="weno5cw", diff_order=4, L=(1.0, 0.9, 1.9),
+                bc=["neumann", "neumann", "periodic", "periodic", "periodic", "periodic"])
+    case = C.sutherland(case, 2e-3, 1.0)
+    W0 = C.random_smooth_W(case, 51)
+    print(f"[gpu] two streams vs one, {math}", flush=True)
+    out = []
+    for timed in (False, True):
+        s = ns.Solver(case, math=math)
+        s.timing(timed)
+        s.set_state(W0, "primitive")
+        s.step(4)
+        out.append((s.get_state("primitive"), s.get_state("conservative")))
+        s.close()
+    np.testing.assert_array_equal(po.interior(out[0][0]), po.interior(out[1][0]))
+    np.testing.assert_array_equal(po.interior(out[0][1]), po.interior(out[1][1]))
+
+
 def test_set_state_derives_the_other_form(ns, oracle):
     # setting W gives U = p2c(W) (get_state, reduce); setting U gives W = c2p(U), and the step
     # then advances that state, as the reference does from W (time_scheme.cpp:25)
