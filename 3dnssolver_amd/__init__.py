@@ -231,12 +231,18 @@ class Solver:
         h = ctypes.c_void_p()
         self._check(self.L.nsgpu_create(ctypes.byref(self.cfg), device, ctypes.byref(h)))
         self.h = h
+        self._ensembles = 0  # live Ensembles holding this context
 
     def _check(self, rc):
         if rc != 0:
             raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
 
     def close(self):
+        """Frees the context.  Refused (RuntimeError) while a live Ensemble holds it: the
+        ensemble's block table and step graph point at the context's device buffers."""
+        if getattr(self, "_ensembles", 0) > 0:
+            raise RuntimeError("Solver.close: the context is a member of a live Ensemble; "
+                               "close the ensemble first")
         if getattr(self, "h", None):
             self.L.nsgpu_destroy(self.h)
             self.h = None
@@ -291,7 +297,8 @@ class Solver:
         return U, W, R
 
     def step(self, nsteps=1, time=0.0):
-        """Advances the resident primitive state; returns (time, last dt)."""
+        """Advances the resident primitive state; returns (time, last dt).  Raises NsgpuError
+        (code -4 / -5) when a stage produced a non-finite or non-positive state."""
         t = np.array([time], dtype=np.float64)
         dt = np.zeros(1)
         self._check(self.L.nsgpu_step(self.h, nsteps, _ptr(t), _ptr(dt)))
@@ -348,6 +355,8 @@ class Ensemble:
         if rc != 0:
             raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
         self.h = h
+        for m in self.members:
+            m._ensembles += 1
 
     def _check(self, rc):
         if rc != 0:
@@ -357,6 +366,8 @@ class Ensemble:
         if getattr(self, "h", None):
             self.L.nsgpu_ensemble_destroy(self.h)
             self.h = None
+            for m in self.members:
+                m._ensembles -= 1
 
     def __del__(self):
         try:

@@ -32,7 +32,13 @@
 
 namespace nsgpu_adapters {
 
+// Driver::create_convection_operator (driver.cpp:74-94) builds Weno5 / Weno5cw for "Weno",
+// CenteredConvection for "Centered" and the null scheme for any other name.  The GPU path has no
+// centred convection: such a configuration is refused rather than run without convection.
 inline nsgpu_config make_config(const Params& p, const Array4d& like, int math) {
+    if (p.convection_scheme == "Centered")
+        throw std::invalid_argument("nsgpu: convection_scheme \"Centered\" (CenteredConvection) "
+                                    "has no GPU implementation; use the reference's scheme");
     nsgpu_config c{};
     c.nx = (int)like.nx(), c.ny = (int)like.ny(), c.nz = (int)like.nz(), c.ngc = (int)like.ngc();
     c.dim = p.dim;
@@ -45,7 +51,7 @@ inline nsgpu_config make_config(const Params& p, const Array4d& like, int math) 
     if (p.convection_scheme == "Weno")
         c.conv = p.convection_in_characteristic_space ? NSGPU_CONV_WENO5CW : NSGPU_CONV_WENO5;
     else
-        c.conv = NSGPU_CONV_NONE;
+        c.conv = NSGPU_CONV_NONE;  // the reference's null ConvectionScheme (driver.cpp:91-93)
     c.weights    = (p.weight_type == "Borges") ? NSGPU_WEIGHTS_BORGES : NSGPU_WEIGHTS_SHU;
     c.diff_order = (p.diffusion_scheme == "Centered") ? p.diffusion_scheme_order : 0;
     static const char* names[6] = {"West", "East", "North", "South", "Front", "Back"};

@@ -65,6 +65,8 @@ struct StageArgs {
     double*       Uout;   // stage output (interior)
     double*       Wout;   // c2p(Uout) (interior)
     int           stage;  // 0, 1, 2
+    unsigned*     flags;  // health of the stage output (may be null): bit 0 a non-finite value,
+                          // bit 1 rho <= 0 or p <= 0; bits are only ever set (atomic OR)
 };
 
 // ------------------------------------------------------------------------------------------
@@ -97,6 +99,7 @@ struct EnsBlock {
     double* Dy;
     unsigned long long* dtmin;
     double*  dtime;
+    unsigned* flags;  // health word of the ensemble step (shared by all blocks)
     EnsTiles t;
     int      grid[EF_COUNT][3];  // local grid of each family (x fastest); 0 = no work
 };

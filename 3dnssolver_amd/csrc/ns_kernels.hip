@@ -791,6 +791,24 @@ __device__ __forceinline__ void p2c(const double w[5], double gamma, double u[5]
     u[1]       = dv(w[1], gamma - 1.0) + kin;
 }
 
+// health of a new stage state (the step's failure detection): bit 0 a non-finite conserved or
+// primitive value, bit 1 rho <= 0 or p <= 0 (finite).  The wavefront's verdicts are combined with
+// ballots and its first active lane sets them with one atomic OR, only in a wavefront that holds
+// such a cell, so a healthy stage adds compares and two ballots per cell and no memory traffic.
+__device__ __forceinline__ void stage_health(unsigned* flags, const double u[5], const double w[5]) {
+    bool nf = false;
+#pragma unroll
+    for (int q = 0; q < 5; q++)
+        nf = nf || !(__builtin_fabs(u[q]) <= 1.7976931348623157e308) ||
+             !(__builtin_fabs(w[q]) <= 1.7976931348623157e308);
+    const bool np = !nf && !(w[0] > 0.0 && w[1] > 0.0);
+    const unsigned long long a = __ballot(nf), b = __ballot(np);
+    if ((a | b) == 0ull) return;
+    const unsigned long long act = __ballot(1);
+    if (__lane_id() == (unsigned)(__ffsll((unsigned long long)act) - 1))
+        atomicOr(flags, (a ? 1u : 0u) | (b ? 2u : 0u));
+}
+
 __device__ __forceinline__ double local_dt(const double w[5], double gamma, double dx, double dy,
                                            double dz) {
     double c  = sqrt(dv(gamma * w[1], w[0]));
@@ -1438,6 +1456,7 @@ __device__ __forceinline__ void residual_body(
         c2p(un, ph.gamma, w);
 #pragma unroll
         for (int q = 0; q < 5; q++) st.Wout[c + q * P] = w[q];
+        if (st.flags) stage_health(st.flags, un, w);
     }
 }
 
@@ -1658,6 +1677,7 @@ __device__ __forceinline__ void residual_fd_body(
         c2p(un, ph.gamma, w);
 #pragma unroll
         for (int q = 0; q < 5; q++) (st.Wout + q * P)[cc] = w[q];
+        if (st.flags) stage_health(st.flags, un, w);
     }
 }
 
@@ -2127,7 +2147,7 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
     const double *fx = E->F[0], *fy = E->F[1], *fz = E->F[2], *Dx = E->Dx, *Dy = E->Dy;
     const double* Wp = ens_w(E, s);
     StageArgs     a{E->Un, ens_u(E, s), s == 1 ? E->Ub : E->Ua,
-                    s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s};
+                    s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s, E->flags};
     const unsigned long long* dtmin = E->dtmin;
     if constexpr (ORDER > 0) {
         if (ens_fused(g)) {

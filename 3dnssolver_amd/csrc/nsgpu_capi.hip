@@ -63,9 +63,13 @@ struct nsgpu_ctx {
     int                partial_max = 0;
     unsigned long long* dtmin = nullptr;  // [0] step dt accumulator, [1] reduce accumulator
     unsigned*          flags = nullptr;
+    unsigned*          sflags = nullptr;  // health of the stage outputs of the current step call
     double*            dtime = nullptr;   // device time accumulator
     double*            sums  = nullptr;   // [5]
     double*            Ulast = nullptr;   // buffer holding the last step's U
+    double             dt_scale = 0.0;    // dt = dtmin[0] bits * dt_scale: the CFL, or 1 after a
+                                          // prescribed-dt step (dtmin[0] then holds dt itself)
+    int                enrolled = 0;      // ensembles this context is a member of
     float*             mlp_params = nullptr;  // learned weights: FP32 parameters
     MlpPacked*         mlp_packed = nullptr;  // and their packed BF16 fragments
     bool               timing = false;
@@ -180,6 +184,14 @@ int download_soa(nsgpu_ctx* c, const double* src, double* aos) {
     HIPCHK(hipMemcpyAsync(aos, c->stage_aos, host_len(c->g) * sizeof(double),
                           hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return NSGPU_OK;
+}
+
+// status of a health word (bit 0 non-finite, bit 1 positivity, bit 2 alpha)
+int health_status(unsigned fl) {
+    if (fl & 1u) return NSGPU_ERR_NONFINITE;
+    if (fl & 2u) return NSGPU_ERR_POSITIVITY;
+    if (fl & 4u) return NSGPU_ERR_ALPHA;
     return NSGPU_OK;
 }
 
@@ -355,6 +367,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max, c->stream);
     if (!rc) rc = alloc((double**)&c->dtmin, 2 * sizeof(unsigned long long), c->stream);
     if (!rc) rc = alloc((double**)&c->flags, sizeof(double), c->stream);
+    if (!rc) rc = alloc((double**)&c->sflags, sizeof(double), c->stream);
     if (!rc) rc = alloc(&c->dtime, sizeof(double), c->stream);
     if (!rc) rc = alloc(&c->sums, 5 * sizeof(double), c->stream);
     if (!rc && cfg->weights == NSGPU_WEIGHTS_LEARNED) {
@@ -369,7 +382,8 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
         rc = NSGPU_ERR_HIP;
     if (!rc && hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         rc = NSGPU_ERR_HIP;
-    c->Ulast = c->Ua;
+    c->Ulast    = c->Ua;
+    c->dt_scale = cfg->cfl;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (rc) {
         nsgpu_destroy(c);
@@ -381,6 +395,12 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
 
 void nsgpu_destroy(nsgpu_ctx* c) {
     if (!c) return;
+    if (c->enrolled > 0) {
+        // an ensemble holds this context's device buffers in its block table and its graph
+        fprintf(stderr, "nsgpu: nsgpu_destroy refused: the context is a member of %d live "
+                        "ensemble(s); destroy them first\n", c->enrolled);
+        return;
+    }
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     if (c->aux) hipStreamSynchronize(c->aux);
@@ -388,7 +408,8 @@ void nsgpu_destroy(nsgpu_ctx* c) {
                         c->F[0],  c->F[1], c->F[2],  c->Dx,    c->Dy, c->Dz, c->Rv,
                         c->R,     c->stage_aos};
     for (double* p : arrays) free_array(p);
-    double* bufs[] = {c->partial, (double*)c->dtmin, (double*)c->flags, c->dtime, c->sums,
+    double* bufs[] = {c->partial, (double*)c->dtmin, (double*)c->flags, (double*)c->sflags,
+                      c->dtime, c->sums,
                       (double*)c->mlp_params, (double*)c->mlp_packed};
     for (double* p : bufs)
         if (p) hipFree(p);
@@ -445,17 +466,21 @@ int nsgpu_apply_boundary(nsgpu_ctx* c, double* aos) {
     return download_soa(c, c->R, aos);
 }
 
+// The operator-level calls below work in the stage scratch arrays (Ws[0] for W, Ub for U): a
+// step rewrites both before reading them, and the resident state (W, Ulast = Ua between steps)
+// stays as it was, so stepping, get_state and reduce are unaffected by them.
 int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_aos, double* R_aos) {
     if (!c || !W_aos || !U_aos || !R_aos || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
-    int rc = upload_soa(c, W_aos, c->W);
+    double* Wt = c->Ws[0];
+    int     rc = upload_soa(c, W_aos, Wt);
     if (!rc) rc = upload_soa(c, U_aos, c->Ub);
     if (!rc) rc = upload_soa(c, R_aos, c->R);
     if (rc) return rc;
-    convection_fluxes(c, c->Ub, c->W);
+    convection_fluxes(c, c->Ub, Wt);
     {
         Scope s(c, T_RESIDUAL);
-        residual(c, c->W, c->R, c->R, nullptr, nullptr, NSGPU_RHS_CONVECTION);
+        residual(c, Wt, c->R, c->R, nullptr, nullptr, NSGPU_RHS_CONVECTION);
     }
     return download_soa(c, c->R, R_aos);
 }
@@ -463,13 +488,14 @@ int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_ao
 int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
     if (!c || !W_aos || !R_aos) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
-    int rc = upload_soa(c, W_aos, c->W);
+    double* Wt = c->Ws[0];
+    int     rc = upload_soa(c, W_aos, Wt);
     if (!rc) rc = upload_soa(c, R_aos, c->R);
     if (rc) return rc;
-    diffusion_derivs(c, c->W);
+    diffusion_derivs(c, Wt);
     {
         Scope s(c, T_RESIDUAL);
-        residual(c, c->W, c->R, c->R, nullptr, nullptr, NSGPU_RHS_DIFFUSION);
+        residual(c, Wt, c->R, c->R, nullptr, nullptr, NSGPU_RHS_DIFFUSION);
     }
     return download_soa(c, c->R, R_aos);
 }
@@ -477,26 +503,27 @@ int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
 int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int which) {
     if (!c || !U_aos || !R_out || which < 1 || which > 3 || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
-    int rc = upload_soa(c, U_aos, c->Ua);
+    double* Ut = c->Ub;
+    double* Wt = c->Ws[0];
+    int     rc = upload_soa(c, U_aos, Ut);
     if (rc) return rc;
     {
         Scope s(c, T_HALO);
-        c->L->halo(c->Ua, 5, nullptr, 0, c->g, c->b, c->stream);
+        c->L->halo(Ut, 5, nullptr, 0, c->g, c->b, c->stream);
     }
     {
         Scope s(c, T_C2P);
-        c->L->c2p_faces(c->Ua, c->W, c->g, c->ph, c->stream);
+        c->L->c2p_faces(Ut, Wt, c->g, c->ph, c->stream);
     }
-    if (which & 1) convection_fluxes(c, c->Ua, c->W);
-    if (which & 2) diffusion_derivs(c, c->W);
+    if (which & 1) convection_fluxes(c, Ut, Wt);
+    if (which & 2) diffusion_derivs(c, Wt);
     {
         Scope s(c, T_RESIDUAL);
-        residual(c, c->W, nullptr, c->R, nullptr, nullptr, which);
+        residual(c, Wt, nullptr, c->R, nullptr, nullptr, which);
     }
-    rc = download_soa(c, c->Ua, U_aos);
-    if (!rc && W_out) rc = download_soa(c, c->W, W_out);
+    rc = download_soa(c, Ut, U_aos);
+    if (!rc && W_out) rc = download_soa(c, Wt, W_out);
     if (!rc) rc = download_soa(c, c->R, R_out);
-    c->Ulast = c->Ua;
     return rc;
 }
 
@@ -514,18 +541,30 @@ int nsgpu_step_fixed(nsgpu_ctx* c, int nsteps, double dt, double* time_io) {
 }
 
 namespace {
-// nsteps SSP-RK3 steps; fixed_dt > 0 replaces the CFL time step
+// restores the context's CFL after a prescribed-dt call, on every exit path
+struct CflGuard {
+    nsgpu_ctx* c;
+    double     cfl;
+    ~CflGuard() { c->ph.cfl = cfl; }
+};
+
+// nsteps SSP-RK3 steps; fixed_dt > 0 replaces the CFL time step.  The stages and the time
+// accumulation form dt = dtmin[0] * ph.cfl; a prescribed dt is stored as dtmin[0] = fixed_dt
+// with ph.cfl = 1 for the call, so every stage and the time use fixed_dt itself (bitwise).
 int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double* dt_last) {
     if (!c || nsteps < 0 || !net_ready(c)) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
+    CflGuard guard{c, c->ph.cfl};
+    if (fixed_dt > 0.0) c->ph.cfl = 1.0;
     if (time_io) HIPCHK(hipMemcpyAsync(c->dtime, time_io, sizeof(double), hipMemcpyHostToDevice, c->stream));
     const Launchers& L = *c->L;
+    L.zero_flags(c->sflags, c->stream);
     for (int s = 0; s < nsteps; s++) {
         {
             Scope sc(c, T_STEP_BEGIN);
             L.reset_min(&c->dtmin[0], c->stream);
             L.step_begin(c->W, nullptr, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
-            if (fixed_dt > 0.0) L.set_dt(&c->dtmin[0], fixed_dt / c->ph.cfl, c->stream);
+            if (fixed_dt > 0.0) L.set_dt(&c->dtmin[0], fixed_dt, c->stream);
         }
         // stage inputs Un -> Ua -> Ub: Un (Uprev of time_scheme.cpp:27) is never overwritten
         double* cur  = c->Un;
@@ -541,7 +580,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             stage_inputs(c, cur, wcur);
             // the stage writes U and c2p(U) of the next stage; the last stage writes the
             // primitive state into the other state buffer (the residual reads wcur meanwhile)
-            StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st};
+            StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st, c->sflags};
             {
                 Scope sc(c, T_RESIDUAL);
                 residual(c, wcur, nullptr, nullptr, &a, &c->dtmin[0], NSGPU_RHS_BOTH);
@@ -560,10 +599,16 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
         L.time_accum(c->dtime, &c->dtmin[0], c->ph.cfl, c->stream);
     }
     HIPCHK(hipGetLastError());
+    if (nsteps > 0) c->dt_scale = c->ph.cfl;
+    unsigned fl = 0;
     if (time_io) HIPCHK(hipMemcpyAsync(time_io, c->dtime, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&fl, c->sflags, sizeof fl, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (dt_last) return nsgpu_get_dt(c, dt_last);
-    return NSGPU_OK;
+    if (dt_last) {
+        int rc = nsgpu_get_dt(c, dt_last);
+        if (rc) return rc;
+    }
+    return health_status(fl);
 }
 }  // namespace
 
@@ -575,7 +620,7 @@ int nsgpu_get_dt(nsgpu_ctx* c, double* dt) {
     HIPCHK(hipStreamSynchronize(c->stream));
     double m;
     std::memcpy(&m, &bits, sizeof m);
-    *dt = m * c->ph.cfl;
+    *dt = m * c->dt_scale;
     return NSGPU_OK;
 }
 
@@ -604,10 +649,7 @@ int nsgpu_reduce(nsgpu_ctx* c, double sums[5], double* dt_min, unsigned* flags) 
         *dt_min = m * c->ph.cfl;
     }
     if (flags) *flags = fl;
-    if (fl & 1u) return NSGPU_ERR_NONFINITE;
-    if (fl & 2u) return NSGPU_ERR_POSITIVITY;
-    if (fl & 4u) return NSGPU_ERR_ALPHA;
-    return NSGPU_OK;
+    return health_status(fl);
 }
 
 int nsgpu_selftest(int device, long n, unsigned long long seed, long* mismatches) {
@@ -717,6 +759,7 @@ struct nsgpu_ensemble {
     hipGraph_t              graph  = nullptr;
     hipGraphExec_t          exec   = nullptr;
     hipEvent_t              ev[2]  = {nullptr, nullptr};
+    unsigned*               flags  = nullptr;  // health word of the stage outputs (all blocks)
 };
 
 namespace {
@@ -743,6 +786,7 @@ int ens_build(nsgpu_ensemble* e) {
         k.Dx = c->Dx, k.Dy = c->Dy;
         k.dtmin = &c->dtmin[0];
         k.dtime = c->dtime;
+        k.flags = e->flags;
         k.t     = e->tiles[b];
         ens_grids(k);
     }
@@ -815,50 +859,67 @@ int ens_set_tiles(nsgpu_ensemble* e, const std::vector<EnsTiles>& t) {
 }
 
 // state kept across autotuner trials: W in the member's operator-level array R, the stage state
-// Ua (the conservative state the member reports) in its staging array, and the time on the host
-int ens_save(nsgpu_ensemble* e, std::vector<double>& times) {
-    times.resize(e->m.size());
-    for (size_t b = 0; b < e->m.size(); b++) {
+// Ua (the conservative state the member reports) in its staging array, and on the host the time,
+// the step's dt word (what nsgpu_get_dt reports) and its scale
+struct EnsSaved {
+    std::vector<double>             time, dt_scale;
+    std::vector<unsigned long long> dtbits;
+};
+
+int ens_save(nsgpu_ensemble* e, EnsSaved& sv) {
+    const size_t n = e->m.size();
+    sv.time.resize(n), sv.dt_scale.resize(n), sv.dtbits.resize(n);
+    for (size_t b = 0; b < n; b++) {
         nsgpu_ctx* c = e->m[b];
         HIPCHK(hipMemcpyAsync(c->R, c->W, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
         HIPCHK(hipMemcpyAsync(c->stage_aos, c->Ua, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
                               e->stream));
-        HIPCHK(hipMemcpyAsync(&times[b], c->dtime, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipMemcpyAsync(&sv.time[b], c->dtime, sizeof(double), hipMemcpyDeviceToHost,
+                              e->stream));
+        HIPCHK(hipMemcpyAsync(&sv.dtbits[b], &c->dtmin[0], sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, e->stream));
+        sv.dt_scale[b] = c->dt_scale;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     return NSGPU_OK;
 }
 
-int ens_restore(nsgpu_ensemble* e, const std::vector<double>& times) {
+int ens_restore(nsgpu_ensemble* e, const EnsSaved& sv) {
     for (size_t b = 0; b < e->m.size(); b++) {
         nsgpu_ctx* c = e->m[b];
         HIPCHK(hipMemcpyAsync(c->W, c->R, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
         HIPCHK(hipMemcpyAsync(c->Ua, c->stage_aos, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
                               e->stream));
-        HIPCHK(hipMemcpyAsync(c->dtime, &times[b], sizeof(double), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(c->dtime, &sv.time[b], sizeof(double), hipMemcpyHostToDevice,
+                              e->stream));
+        HIPCHK(hipMemcpyAsync(&c->dtmin[0], &sv.dtbits[b], sizeof(unsigned long long),
+                              hipMemcpyHostToDevice, e->stream));
+        c->dt_scale = sv.dt_scale[b];
+        c->Ulast    = c->Ua;
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     return NSGPU_OK;
 }
 
-// ms per step of `steps` graph replays with tiles t (best of 2), state restored afterwards
-int ens_trial(nsgpu_ensemble* e, const std::vector<EnsTiles>& t, int steps,
-              const std::vector<double>& times, double* ms) {
-    int rc = ens_set_tiles(e, t);
-    if (!rc) rc = ens_run(e, 1, true);
+// ms per step of `steps` graph replays with tiles t (best of 2); the state is restored on every
+// exit path (a failed restore is reported before a failed trial)
+int ens_trial(nsgpu_ensemble* e, const std::vector<EnsTiles>& t, int steps, const EnsSaved& sv,
+              double* ms) {
+    int    rc   = ens_set_tiles(e, t);
     double best = 1e300;
+    if (!rc) rc = ens_run(e, 1, true);
     for (int r = 0; r < 2 && !rc; r++) {
-        HIPCHK(hipEventRecord(e->ev[0], e->stream));
-        rc = ens_run(e, steps, true);
-        HIPCHK(hipEventRecord(e->ev[1], e->stream));
-        HIPCHK(hipEventSynchronize(e->ev[1]));
         float m = 0.f;
-        HIPCHK(hipEventElapsedTime(&m, e->ev[0], e->ev[1]));
-        best = std::min(best, (double)m / steps);
+        if (hipEventRecord(e->ev[0], e->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
+        if (!rc) rc = ens_run(e, steps, true);
+        if (!rc && hipEventRecord(e->ev[1], e->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
+        if (!rc && hipEventSynchronize(e->ev[1]) != hipSuccess) rc = NSGPU_ERR_HIP;
+        if (!rc && hipEventElapsedTime(&m, e->ev[0], e->ev[1]) != hipSuccess) rc = NSGPU_ERR_HIP;
+        if (!rc) best = std::min(best, (double)m / steps);
     }
-    if (!rc) rc = ens_restore(e, times);
+    const int rr = ens_restore(e, sv);
     *ms = best;
-    return rc;
+    return rr ? rr : rc;
 }
 
 }  // namespace
@@ -897,7 +958,10 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
         rc = NSGPU_ERR_HIP;
     for (int t = 0; t < 2 && !rc; t++)
         if (hipEventCreate(&e->ev[t]) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc && hipMalloc((void**)&e->flags, sizeof(unsigned)) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (!rc) rc = ens_build(e);
+    // members are enrolled from here on (nsgpu_ensemble_destroy releases them)
+    for (nsgpu_ctx* c : e->m) c->enrolled++;
     if (rc) {
         nsgpu_ensemble_destroy(e);
         return rc;
@@ -913,6 +977,8 @@ void nsgpu_ensemble_destroy(nsgpu_ensemble* e) {
     ens_drop_graph(e);
     if (e->dblk) hipFree(e->dblk);
     if (e->dstart) hipFree(e->dstart);
+    if (e->flags) hipFree(e->flags);
+    for (nsgpu_ctx* c : e->m) c->enrolled--;
     for (hipEvent_t v : e->ev)
         if (v) hipEventDestroy(v);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -947,11 +1013,17 @@ int nsgpu_ensemble_step(nsgpu_ensemble* e, int nsteps, int use_graph) {
     if (!e || nsteps < 0) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(e->device));
     int rc = ens_refresh(e);
+    if (!rc) e->m[0]->L->zero_flags(e->flags, e->stream);
     if (!rc) rc = ens_run(e, nsteps, use_graph != 0);
+    unsigned fl = 0;
+    HIPCHK(hipMemcpyAsync(&fl, e->flags, sizeof fl, hipMemcpyDeviceToHost, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
     // the last stage leaves the conservative state in Ua and the primitive state in W
-    for (nsgpu_ctx* c : e->m) c->Ulast = c->Ua;
-    return rc;
+    for (nsgpu_ctx* c : e->m) {
+        c->Ulast = c->Ua;
+        if (nsteps > 0) c->dt_scale = c->ph.cfl;
+    }
+    return rc ? rc : health_status(fl);
 }
 
 int nsgpu_ensemble_tune(nsgpu_ensemble* e, int steps, double* ms_default, double* ms_tuned) {
@@ -959,13 +1031,13 @@ int nsgpu_ensemble_tune(nsgpu_ensemble* e, int steps, double* ms_default, double
     HIPCHK(hipSetDevice(e->device));
     int rc = ens_refresh(e);
     if (rc) return rc;
-    std::vector<double> times;
-    rc = ens_save(e, times);
+    EnsSaved saved;
+    rc = ens_save(e, saved);
     if (rc) return rc;
     const size_t n = e->m.size();
     std::vector<EnsTiles> best(n, EnsTiles{64, 64, 16});
     double                tbest = 0;
-    rc = ens_trial(e, best, steps, times, &tbest);
+    rc = ens_trial(e, best, steps, saved, &tbest);
     if (rc) return rc;
     if (ms_default) *ms_default = tbest;
     // classes: members with equal nz share kw_flux / kw_cell, members with equal nx share march
@@ -984,7 +1056,7 @@ int nsgpu_ensemble_tune(nsgpu_ensemble* e, int steps, double* ms_default, double
                 std::vector<EnsTiles> cand = best;
                 for (size_t b : cls) cand[b].*field = vals[v];
                 double t  = 0;
-                int    r2 = ens_trial(e, cand, steps, times, &t);
+                int    r2 = ens_trial(e, cand, steps, saved, &t);
                 if (r2) return r2;
                 if (t < tbest * 0.99) best = cand, tbest = t, improved = true;
             }

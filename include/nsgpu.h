@@ -43,8 +43,8 @@ enum {
     NSGPU_ERR_INVALID     = -1, /* bad configuration or argument */
     NSGPU_ERR_HIP         = -2, /* a HIP runtime call failed */
     NSGPU_ERR_NODEVICE    = -3, /* no GPU / device index out of range */
-    NSGPU_ERR_NONFINITE   = -4, /* state holds NaN / Inf (nsgpu_reduce health check) */
-    NSGPU_ERR_POSITIVITY  = -5, /* rho <= 0 or p <= 0 */
+    NSGPU_ERR_NONFINITE   = -4, /* state holds NaN / Inf (nsgpu_step, nsgpu_reduce) */
+    NSGPU_ERR_POSITIVITY  = -5, /* rho <= 0 or p <= 0 (nsgpu_step, nsgpu_reduce) */
     NSGPU_ERR_ALPHA       = -6, /* max |u|+c exceeds the Lax-Friedrichs constant 3.0 */
     NSGPU_ERR_UNSUPPORTED = -7  /* valid request the library does not implement */
 };
@@ -118,6 +118,9 @@ typedef struct {
 
 /* Create a context on GPU `device` (HIP ordinal).  Allocates the block's device arrays. */
 int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out);
+/* Frees the context.  A context that is a member of a live ensemble is not freed (the ensemble
+ * holds its device buffers): the call prints an error and returns; destroy the ensemble first.
+ * Members must outlive their ensembles. */
 void nsgpu_destroy(nsgpu_ctx* ctx);
 const char* nsgpu_strerror(int status);
 
@@ -132,7 +135,10 @@ size_t nsgpu_array_len(const nsgpu_ctx* ctx);
 int nsgpu_set_state(nsgpu_ctx* ctx, const double* aos, int kind);
 int nsgpu_get_state(nsgpu_ctx* ctx, double* aos, int kind);
 
-/* Boundary conditions + single-block halo exchange on a 5-variable array, in place. */
+/* Boundary conditions + single-block halo exchange on a 5-variable array, in place.
+ * This call and the three operator-level calls below (convection_solve, diffusion_solve, rhs)
+ * work in scratch arrays of the context: the resident state that nsgpu_step advances,
+ * nsgpu_get_state returns and nsgpu_reduce reads is left as it was. */
 int nsgpu_apply_boundary(nsgpu_ctx* ctx, double* aos);
 
 /* ConvectionScheme::solve: R += convective residual of (W, U), halos of W and U as given. */
@@ -146,12 +152,18 @@ int nsgpu_diffusion_solve(nsgpu_ctx* ctx, const double* W_aos, double* R_aos);
 int nsgpu_rhs(nsgpu_ctx* ctx, double* U_aos, double* W_out, double* R_out, int which);
 
 /* nsteps x TimeScheme::step on the resident primitive state.  Adds each step's dt to *time_io
- * (may be NULL) and stores the last dt in *dt_last (may be NULL).  Synchronises once at the end. */
+ * (may be NULL) and stores the last dt in *dt_last (may be NULL).  Synchronises once at the end.
+ * Health: every stage update checks the state it writes; the call returns NSGPU_ERR_NONFINITE
+ * if any stage of any of its steps produced a NaN / Inf, else NSGPU_ERR_POSITIVITY if one
+ * produced rho <= 0 or p <= 0 (the steps still all run; the reference has no such check and
+ * would carry on).  The alpha check stays with nsgpu_reduce. */
 int nsgpu_step(nsgpu_ctx* ctx, int nsteps, double* time_io, double* dt_last);
+/* dt of the last step taken (the prescribed one after nsgpu_step_fixed). */
 int nsgpu_get_dt(nsgpu_ctx* ctx, double* dt);
 /* No reference counterpart (the reference always takes the CFL step, time_scheme.cpp:16):
  * nsteps steps with a prescribed dt, for convergence and resolution studies that compare runs
- * at equal times. */
+ * at equal times.  Every stage and the time use dt itself (the CFL does not enter); health as
+ * nsgpu_step. */
 int nsgpu_step_fixed(nsgpu_ctx* ctx, int nsteps, double dt, double* time_io);
 
 /* Conservation sums over the interior of the resident conservative state (sum of U_q * dx*dy*dz),
@@ -218,6 +230,8 @@ int nsgpu_mlp_recon(int device, int math, const float* params, long n, const dou
  * stay usable (set_state, get_state, reduce, step) between ensemble calls. */
 typedef struct nsgpu_ensemble nsgpu_ensemble;
 
+/* The members stay owned by the caller and must outlive the ensemble: nsgpu_destroy refuses a
+ * context while an ensemble holds it. */
 int  nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out);
 void nsgpu_ensemble_destroy(nsgpu_ensemble* ens);
 
@@ -228,14 +242,16 @@ int nsgpu_ensemble_set_tiles(nsgpu_ensemble* ens, const int* tiles);
 int nsgpu_ensemble_get_tiles(const nsgpu_ensemble* ens, int* tiles);
 
 /* nsteps SSP-RK3 steps of every member; use_graph != 0 replays the captured step graph,
- * otherwise the launches are enqueued directly.  Synchronises once at the end. */
+ * otherwise the launches are enqueued directly.  Synchronises once at the end.  Health as
+ * nsgpu_step, over all members (nsgpu_reduce on each member tells which one failed). */
 int nsgpu_ensemble_step(nsgpu_ensemble* ens, int nsteps, int use_graph);
 
 /* Autotuner: coordinate descent over the tile shapes, kw_flux and kw_cell per class of members
  * with equal nz and march per class of equal nx, timing `steps` graph-replayed steps per trial
- * (best of 2).  The member states are restored after every trial, so tuning leaves them as they
- * were.  Stores the step time of the default tiles and of the chosen tiles (ms, may be NULL)
- * and leaves the chosen tiles set. */
+ * (best of 2).  The member states (primitive and conservative state, time and last dt) are
+ * restored after every trial, also after a failed one, so tuning leaves them as they were.
+ * Stores the step time of the default tiles and of the chosen tiles (ms, may be NULL) and
+ * leaves the chosen tiles set. */
 int nsgpu_ensemble_tune(nsgpu_ensemble* ens, int steps, double* ms_default, double* ms_tuned);
 
 #ifdef __cplusplus
