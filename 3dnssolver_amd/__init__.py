@@ -44,7 +44,7 @@ EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len",
            "nsgpu_timer_read", "nsgpu_ensemble_create", "nsgpu_ensemble_destroy",
            "nsgpu_ensemble_set_tiles", "nsgpu_ensemble_get_tiles", "nsgpu_ensemble_step",
            "nsgpu_ensemble_tune", "nsgpu_set_mlp", "nsgpu_mlp_forward", "nsgpu_mlp_loss_grad",
-           "nsgpu_mlp_train", "nsgpu_mlp_recon", "nsgpu_step_fixed"]
+           "nsgpu_mlp_train", "nsgpu_mlp_recon", "nsgpu_step_fixed", "nsgpu_mlp_loss_grad_noalloc"]
 
 
 class Config(ctypes.Structure):
@@ -121,6 +121,8 @@ def load_library(path=None):
                                   ctypes.c_long, ctypes.c_float, Fp]
     L.nsgpu_mlp_recon.argtypes = [ctypes.c_int, ctypes.c_int, Fp, ctypes.c_long, D, ctypes.c_int, D,
                                   I]
+    # device pointers and a hipStream_t (allocation-free gradient)
+    L.nsgpu_mlp_loss_grad_noalloc.argtypes = [ctypes.c_int, P, ctypes.c_long, P, P, P, P, P]
     _lib = L
     return L
 

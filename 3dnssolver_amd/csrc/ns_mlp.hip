@@ -13,6 +13,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "../../include/nsgpu.h"
@@ -323,6 +324,37 @@ int grad_launch(const MlpPacked* P, long n, const float* X, const float* Q, cons
     return NSGPU_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// allocation-free loss gradient (nsgpu_mlp_loss_grad_noalloc): the packed network and the
+// per-workgroup partial gradients live in one file-scope device array, sized for at most
+// NSGPU_MLP_NOALLOC_MAX_N samples (1024 workgroups of 256 samples: 5.9 MB + the 9.5 KB network).
+// Each GPU holds its own copy until the process ends; its address is looked up once per device.
+// k_mlp_pack rewrites the whole network and k_mlp_grad every entry of the partial rows the
+// reduction reads, so no part of the array has to start at zero.
+// ------------------------------------------------------------------------------------------
+constexpr long   NOALLOC_BLOCKS  = (NSGPU_MLP_NOALLOC_MAX_N + 64L * GRAD_ITERS - 1) / (64L * GRAD_ITERS);
+constexpr size_t NOALLOC_NET     = (sizeof(MlpPacked) + 255) / 256 * 256;
+constexpr size_t NOALLOC_BYTES   = NOALLOC_NET + sizeof(float) * (MLP_NPARAM + 1) * NOALLOC_BLOCKS;
+constexpr int    NOALLOC_DEVICES = 64;
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+std::mutex     g_scratch_mu;
+unsigned char* g_scratch_ptr[NOALLOC_DEVICES] = {};
+
+// the device address of `scratch` on the current device `device` (looked up on first use)
+int scratch_address(int device, unsigned char** out) {
+    if (device >= NOALLOC_DEVICES) return NSGPU_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = (unsigned char*)p;
+    }
+    *out = g_scratch_ptr[device];
+    return NSGPU_OK;
+}
+
 }  // namespace
 
 namespace nsgpu {
@@ -428,6 +460,28 @@ int nsgpu_mlp_train(int device, float* params, long n, const float* X, const flo
     HIPCHK(hipMemcpy(params, dp, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost));
     if (loss_hist && steps > 0)
         HIPCHK(hipMemcpy(loss_hist, dh, sizeof(float) * steps, hipMemcpyDeviceToHost));
+    return NSGPU_OK;
+}
+
+int nsgpu_mlp_loss_grad_noalloc(int device, const float* params, long n, const float* X,
+                                const float* Q, const float* T, float* out, void* stream) {
+    if (!params || !X || !Q || !T || !out || n < 1) return NSGPU_ERR_INVALID;
+    if (n > NSGPU_MLP_NOALLOC_MAX_N) {
+        fprintf(stderr, "nsgpu: nsgpu_mlp_loss_grad_noalloc: n = %ld exceeds the maximum of %ld "
+                        "samples of its fixed device scratch (use nsgpu_mlp_loss_grad)\n",
+                n, (long)NSGPU_MLP_NOALLOC_MAX_N);
+        return NSGPU_ERR_INVALID;
+    }
+    int rc = device_ok(device);
+    if (rc) return rc;
+    unsigned char* base = nullptr;
+    if ((rc = scratch_address(device, &base))) return rc;
+    MlpPacked*        P       = reinterpret_cast<MlpPacked*>(base);
+    float*            partial = reinterpret_cast<float*>(base + NOALLOC_NET);
+    const hipStream_t s       = static_cast<hipStream_t>(stream);
+    k_mlp_pack<<<1, 64, 0, s>>>(params, P);
+    grad_launch(P, n, X, Q, T, partial, out, s);
+    HIPCHK(hipGetLastError());
     return NSGPU_OK;
 }
 

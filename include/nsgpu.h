@@ -218,6 +218,22 @@ int nsgpu_mlp_train(int device, float* params, long n, const float* X, const flo
 int nsgpu_mlp_recon(int device, int math, const float* params, long n, const double* S, int side,
                     double* out, int* regime);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory: code under
+ * HIP graph capture, training loops that forbid allocation between steps.  The mean loss and
+ * its parameter gradient as nsgpu_mlp_loss_grad computes them (same kernels, same summation
+ * order, bitwise the same floats), but every pointer is DEVICE memory of the caller: params
+ * [NSGPU_MLP_NPARAM], X[n][8], Q[n][3], T[n], out[NSGPU_MLP_NPARAM + 1] (gradient, then the loss).
+ * The launches are enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call
+ * returns without waiting: read `out` after synchronising that stream.  It calls no hipMalloc,
+ * hipFree or allocating library routine: the packed network and the per-workgroup partial
+ * gradients live in a fixed device array of the library (6 MB per GPU, held until the process
+ * ends), so n is limited to NSGPU_MLP_NOALLOC_MAX_N samples (NSGPU_ERR_INVALID beyond).  All
+ * calls on one GPU share that array: calls that overlap on one GPU (different streams, or
+ * threads) race and must be serialised by the caller. */
+#define NSGPU_MLP_NOALLOC_MAX_N 262144L
+int nsgpu_mlp_loss_grad_noalloc(int device, const float* params, long n, const float* X,
+                                const float* Q, const float* T, float* out, void* stream);
+
 /* ---- Ensembles: many independent blocks of different sizes on one GPU ----------------------
  * No reference counterpart: the reference advances one block per MPI rank, each with its own
  * driver loop (driver.cpp:45-64 calling TimeScheme::step, time_scheme.cpp:3-70).  An ensemble
