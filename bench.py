@@ -29,6 +29,10 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
 # Ucur: 5 each; W: 4; dW/dx of u, v: 2; it forms the other derivatives itself) and 10 written
 # (Uout, Wout)
 RES_PLANES = 41
+# with the z sweep fused into it, the residual kernel reads U and W of the z stencils instead of
+# fz: 31 read (fx, fy, Un, Ucur: 5 each; W: 5; dW/dx of u, v: 2; the first face of each row: a
+# plane of 1/n) and 10 written
+RES_PLANES_FUSED_Z = 41
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector, dense, with packed FP32 (which this build leaves off)
 
@@ -49,18 +53,31 @@ def cpu_baseline_child(n, steps):
     t0 = time.perf_counter()
     lib.step(case, W, steps)
     el = time.perf_counter() - t0
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     print(json.dumps({"value": n ** 3 * steps / el / 1e9, "unit": "Gcell-RK3-steps/s",
-                      "cores": lib.num_threads(), "kind": kind,
+                      "cores": lib.num_threads(), "kind": kind, "cpu_model": model,
+                      "logical_cpus": os.cpu_count(),
+                      "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND",
+                                                             "OMP_PLACES")},
                       "sample": f"{steps} SSP-RK3 step(s) of the same workload at {n}^3 "
-                                f"({el:.1f} s wall, OpenMP), constant mu: the reference's code "
-                                f"path never calls its Sutherland law"}))
+                                f"({el:.1f} s wall, OpenMP threads = cores), constant mu: the "
+                                f"reference's code path never calls its Sutherland law"}))
 
 
 def run_cpu_baseline(n, steps, timeout):
     try:
+        # one OpenMP thread per core the job may use, spread over the cores
+        env = dict(os.environ, OMP_PROC_BIND="spread", OMP_PLACES="cores")
         r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child",
                             "--cpu-n", str(n), "--cpu-steps", str(steps)],
-                           capture_output=True, text=True, timeout=timeout)
+                           capture_output=True, text=True, timeout=timeout, env=env)
         lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
         if r.returncode == 0 and lines:
             return json.loads(lines[-1])
@@ -334,6 +351,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-baseline-child", action="store_true")
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the FP32 and learned sub-keys of the headline line")
+    ap.add_argument("--learned-steps", type=int, default=10)
     args = ap.parse_args()
 
     if args.cpu_baseline_child:
@@ -359,6 +379,7 @@ def main():
     case["conv"] = args.conv
     case["alpha_mode"], case["physics"] = args.alpha_mode, args.physics
     n = args.n
+    cells = n ** 3
     print(f"[bench] {n}^3 {args.conv} cd4 math={args.math}: setup", file=sys.stderr, flush=True)
     s = ns.Solver(case, math=args.math)
     t_up = time.perf_counter()
@@ -371,39 +392,28 @@ def main():
     _, dt = s.step(args.steps)          # returns after hipStreamSynchronize
     el = time.perf_counter() - t0
     sums, dtmin, flags, rc = s.reduce()
-    cells = n ** 3
     value = cells * args.steps / el / 1e9
     ms_per_step = el / args.steps * 1e3
     t_down = time.perf_counter()
-    s.get_state("primitive")  # device -> host array
+    U_fp64 = s.get_state("conservative")[3:-3, 3:-3, 3:-3]  # device -> host array
     t_down = time.perf_counter() - t_down
     pcie_value = cells * args.steps / (el + t_up + t_down) / 1e9
 
-    # per-kernel durations with HIP events on the context's stream (separate pass)
+    # per-kernel durations with HIP events on the context's stream (separate pass, one stream)
     s.timing(True)
     s.step(args.timing_steps)
     timers = s.timers()
     s.timing(False)
     s.close()
-    flux = {k: v for k, v in timers.items() if k.startswith("face_flux")}
-    dom = max(flux, key=lambda k: flux[k][0]) if flux else max(timers, key=lambda k: timers[k][0])
-    dom_ms, dom_n = timers[dom]
-    faces = (n + 1) * n * n
-    flop_launch = FLOP_PER_FACE[args.conv] * faces
-    achieved = flop_launch / (dom_ms / dom_n * 1e-3) / 1e12
-    traffic = None
-    tr = {}
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        tr = json.load(open(tpath))
-        traffic = tr.get(f"{dom}_{args.math}_{n}")
-    step_frac = value * 1e9 * FLOP_PER_CELL_STEP[args.conv] / (FP64_PEAK_TFLOPS * 1e12)
-    # the residual + stage-update kernel is HBM-bound
-    res_ms, res_n = timers["residual_update"]
-    res_bytes = RES_PLANES * 8.0 * cells
-    suffix = "" if args.viscosity == "constant" else "_" + args.viscosity
-    res_traffic = tr.get(f"residual_update_{args.math}_{n}{suffix}") if os.path.exists(tpath) else None
-    res_achieved = res_bytes / (res_ms / res_n * 1e-3) / 1e9
+    kernels = kernel_rooflines(timers, args, n)
+    step_tflops = value * 1e9 * FLOP_PER_CELL_STEP[args.conv] / 1e12
+    traffic = step_traffic(timers, args, n)
+
+    forms = None
+    if not args.no_forms and args.math == "exact" and args.conv == "weno5cw":
+        forms = {"fp32": fp32_form(ns, case, W0, args, value, U_fp64)}
+        del U_fp64
+        forms["learned"] = learned_form(ns, C, args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -411,7 +421,7 @@ def main():
         cpu = run_cpu_baseline(args.cpu_n, args.cpu_steps, timeout=240)
 
     out = {
-        "metric": "Gcell-RK3-steps/s at 256^3 + % binding roofline",
+        "metric": "Gcell-RK3-steps/s at 256^3 + % binding roofline; learned/FP32 vs plain FP64 at L2",
         "value": round(value, 6),
         "unit": "Gcell-RK3-steps/s",
         "n_gpus": 1,
@@ -432,18 +442,20 @@ def main():
             "alpha_mode": args.alpha_mode, "physics": args.physics,
             "state_resident_in_hbm": True,
         },
+        # the whole SSP-RK3 step against the FP64 vector peak: the algorithmic flop of one
+        # cell-step (12,050 characteristic / 7,860 component-wise) x cells x steps / wall time
         "roofline": {
-            "bound": "mfma", "compute": "FP64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)",
-            "kernel": dom, "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
-            "flop_per_launch": flop_launch, "avg_launch_ms": round(dom_ms / dom_n, 5),
-            "step_frac": round(step_frac, 5),
-        },
-        "roofline_residual": {
-            "bound": "hbm", "kernel": "residual_update", "achieved": round(res_achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(res_achieved / HBM_PEAK_GBS, 4),
-            "traffic": res_traffic, "bytes_per_launch": res_bytes,
-            "avg_launch_ms": round(res_ms / res_n, 5),
+            "bound": "valu", "compute": "FP64 vector ALU (MI355X FP64 peak 78.6 TFLOP/s)",
+            "kernel": "whole SSP-RK3 step (all kernels)",
+            "achieved": round(step_tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(step_tflops / FP64_PEAK_TFLOPS, 5),
+            "flop_per_cell_step": FLOP_PER_CELL_STEP[args.conv],
+            "avg_launch_ms": round(ms_per_step, 4),
+            "traffic": traffic["bytes_per_step"],
+            "traffic_bytes_per_cell_step": traffic["bytes_per_cell_step"],
+            "ideal_bytes_per_cell_step": 320,
+            "traffic_source": traffic["source"],
+            "kernels": kernels,
         },
         "kernel_ms_per_step": {k: round(v[0] / args.timing_steps, 4) for k, v in timers.items()},
         "health": {"status": rc, "flags": flags, "dt_last": dt},
@@ -451,9 +463,134 @@ def main():
                            "download_ms": round(t_down * 1e3, 2),
                            "note": "the timed steps plus one upload of the host state before and "
                                    "one download after"},
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def kernel_rooflines(timers, args, n):
+    """Per-launch rooflines of the dominant face-flux kernel (FP64 VALU) and of the residual +
+    stage-update kernel (HBM), from the HIP-event timing pass."""
+    out = {}
+    flux = {k: v for k, v in timers.items() if k.startswith("face_flux") and v[1]}
+    if flux:
+        dom = max(flux, key=lambda k: flux[k][0] / flux[k][1])
+        ms = flux[dom][0] / flux[dom][1]
+        flop = FLOP_PER_FACE[args.conv] * (n + 1) * n * n
+        tf = flop / (ms * 1e-3) / 1e12
+        out["face_flux"] = {"bound": "valu", "kernel": dom, "achieved": round(tf, 4),
+                            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(tf / FP64_PEAK_TFLOPS, 5), "flop_per_launch": flop,
+                            "avg_launch_ms": round(ms, 5)}
+    res = timers.get("residual_update")
+    if res and res[1]:
+        ms = res[0] / res[1]
+        planes = RES_PLANES_FUSED_Z if timers.get("face_flux_z", (0, 0))[1] == 0 else RES_PLANES
+        b = planes * 8.0 * n ** 3
+        gbs = b / (ms * 1e-3) / 1e9
+        out["residual_update"] = {"bound": "hbm", "kernel": "residual_update",
+                                  "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_launch": b,
+                                  "avg_launch_ms": round(ms, 5)}
+    return out
+
+
+def step_traffic(timers, args, n):
+    """HBM bytes of one step: the launches of one step times each kernel's bytes per launch from
+    the committed counter summary (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload,
+    calibrated on a copy kernel): a lookup, not a measurement of this run."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    none = {"bytes_per_step": None, "bytes_per_cell_step": None, "source": None}
+    if not os.path.exists(path):
+        return none
+    tab = json.load(open(path))
+    key = f"{args.math}_{n}_{args.viscosity}"
+    per = tab.get("per_step", {}).get(key)
+    if per is None:
+        return none
+    return {"bytes_per_step": per, "bytes_per_cell_step": round(per / n ** 3, 1),
+            "source": f"committed lookup profiles/traffic.json[per_step][{key}] (rocprofv3 --pmc "
+                      f"FETCH_SIZE + WRITE_SIZE per kernel, copy-kernel calibrated), not measured "
+                      f"in this run"}
+
+
+def _l2(a, b):
+    import numpy as np
+    d = a - b
+    return float(np.sqrt(np.sum(d * d) / np.sum(b * b)))
+
+
+def fp32_form(ns, case, W0, args, fp64_value, U_fp64):
+    """Precision form on the headline workload: the same steps from the same state in FP32
+    reconstruction + FP64 accumulation; rate, speed-up and the relative L2 / Linf of density and
+    total energy against the FP64 (EXACT) state after the same steps."""
+    import numpy as np
+    n = args.n
+    s = ns.Solver(case, math="fp32")
+    s.set_state(W0, "primitive")
+    if args.warmup:
+        s.step(args.warmup)
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    el = time.perf_counter() - t0
+    U = s.get_state("conservative")[3:-3, 3:-3, 3:-3]
+    rc = s.reduce()[3]
+    s.close()
+    v = n ** 3 * args.steps / el / 1e9
+    err = {}
+    for q, name in ((0, "rho"), (1, "rhoE")):
+        a, b = U[..., q], U_fp64[..., q]
+        err[name] = {"L2_rel": _l2(a, b), "Linf_rel": float(np.abs(a - b).max() / np.abs(b).max())}
+    print(f"[bench] fp32 form: {v:.3f} Gcell-steps/s", file=sys.stderr, flush=True)
+    return {"workload": "the headline workload, FP32 reconstruction and viscous fluxes, FP64 "
+                        "state, stage accumulation and reductions",
+            "value": round(v, 6), "ms_per_step": round(el / args.steps * 1e3, 4),
+            "speedup_vs_fp64": round(v / fp64_value, 4),
+            "errors_vs_fp64": err, "steps_compared": args.warmup + args.steps, "health": rc}
+
+
+def learned_form(ns, C, args):
+    """Learned form: the 256^3 shear layer (component-wise WENO-5 + CD4) stepped with the plain
+    FP64 method (Shu weights, EXACT) and with FP32 reconstruction + the learned weights; rates
+    and speed-up, and the relative L2 errors of both against a 2x-resolution FP64 run on the
+    shear layer and the Sod-like jump (args.eval_n^3 against (2 eval_n)^3, prescribed dt)."""
+    lr = importlib.import_module("3dnssolver_amd.learned")
+    params = lr.load()
+    n = args.n
+    case, W = C.config4(n, "shear")
+    rates = {}
+    for math, weights in (("exact", "shu"), ("fp32", "learned")):
+        s = ns.Solver(dict(case, weights=weights), math=math)
+        if weights == "learned":
+            s.set_mlp(params)
+        s.set_state(W, "primitive")
+        s.step(2)
+        t0 = time.perf_counter()
+        s.step(args.learned_steps)
+        el = time.perf_counter() - t0
+        rates[f"{math}_{weights}"] = {"value": round(n ** 3 * args.learned_steps / el / 1e9, 6),
+                                      "ms_per_step": round(el / args.learned_steps * 1e3, 4),
+                                      "health": s.reduce()[3]}
+        s.close()
+    errors = {}
+    ne = args.eval_n
+    for prob in ("shear", "jump"):
+        c, w = C.config4(ne, prob)
+        fc, fw = C.config4(2 * ne, prob)
+        dtp, t_end, ref = lr.reference(c, w, fc, fw, args.eval_steps)
+        e = {"t_end": t_end, "steps": args.eval_steps}
+        for math, weights in (("exact", "shu"), ("fp32", "learned")):
+            e[f"{math}_{weights}"] = lr.coarse_error(c, w, dtp, args.eval_steps, ref, math,
+                                                     weights, params)
+        errors[prob] = e
+    sp = rates["fp32_learned"]["value"] / rates["exact_shu"]["value"]
+    print(f"[bench] learned form: {sp:.3f}x plain FP64", file=sys.stderr, flush=True)
+    return {"workload": f"double shear layer {n}^3 on [0,1]^3, Mach 0.3, Re 1000, component-wise "
+                        f"WENO-5 (LF) + CD4, SSP-RK3 CFL 0.5; accuracy on the shear layer and the "
+                        f"Sod-like planar jump (8:1, 10:1) at {ne}^3 against a {2 * ne}^3 FP64 run",
+            "rates": rates, "speedup_vs_fp64_plain": round(sp, 4),
+            "l2_rel_vs_2x_fp64": errors}
 
 
 if __name__ == "__main__":
