@@ -44,7 +44,9 @@ EXPORTS = ["nsgpu_create", "nsgpu_destroy", "nsgpu_strerror", "nsgpu_array_len",
            "nsgpu_timer_read", "nsgpu_ensemble_create", "nsgpu_ensemble_destroy",
            "nsgpu_ensemble_set_tiles", "nsgpu_ensemble_get_tiles", "nsgpu_ensemble_step",
            "nsgpu_ensemble_tune", "nsgpu_set_mlp", "nsgpu_mlp_forward", "nsgpu_mlp_loss_grad",
-           "nsgpu_mlp_train", "nsgpu_mlp_recon", "nsgpu_step_fixed", "nsgpu_mlp_loss_grad_noalloc"]
+           "nsgpu_mlp_train", "nsgpu_mlp_recon", "nsgpu_step_fixed", "nsgpu_mlp_loss_grad_noalloc",
+           "nsgpu_domain_create", "nsgpu_domain_destroy", "nsgpu_domain_size", "nsgpu_domain_block",
+           "nsgpu_domain_step"]
 
 
 class Config(ctypes.Structure):
@@ -123,6 +125,14 @@ def load_library(path=None):
                                   I]
     # device pointers and a hipStream_t (allocation-free gradient)
     L.nsgpu_mlp_loss_grad_noalloc.argtypes = [ctypes.c_int, P, ctypes.c_long, P, P, P, P, P]
+    I3 = ctypes.c_int * 3
+    L.nsgpu_domain_create.argtypes = [ctypes.POINTER(Config), I3, ctypes.c_int, ctypes.POINTER(P)]
+    L.nsgpu_domain_destroy.argtypes = [P]
+    L.nsgpu_domain_destroy.restype = None
+    L.nsgpu_domain_size.argtypes = [P]
+    L.nsgpu_domain_block.argtypes = [P, ctypes.c_int, I3, I3]
+    L.nsgpu_domain_block.restype = P
+    L.nsgpu_domain_step.argtypes = [P, ctypes.c_int, D, D]
     _lib = L
     return L
 
@@ -398,3 +408,98 @@ class Ensemble:
         b = np.zeros(1)
         self._check(self.L.nsgpu_ensemble_tune(self.h, steps, _ptr(a), _ptr(b)))
         return float(a[0]), float(b[0])
+
+
+class _Block:
+    """A block of a Domain: the state calls of a Solver on a context the domain owns."""
+
+    def __init__(self, L, h, shape, start, count):
+        self.L, self.h, self.shape, self.start, self.count = L, h, shape, start, count
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+
+    set_state = Solver.set_state
+    get_state = Solver.get_state
+    reduce = Solver.reduce
+    _arr = Solver._arr
+
+
+class Domain:
+    """One block split into nproc = (px, py, pz) linked blocks on one GPU, the reference's
+    decomposition of a multi-rank run (decomposition.cpp:26-57, communicator.cpp:6-144).  Global
+    arrays are the padded (nx+6, ny+6, nz+6, 5) layout of the undivided block; set_state scatters
+    their interiors to the blocks, get_state gathers them back (halos zero).  Each block's own
+    padded arrays are available through .blocks (what one rank of the reference holds)."""
+
+    def __init__(self, case, nproc, math="exact", device=0):
+        self.L = load_library()
+        self.case = dict(case)
+        self.cfg = make_config(case, math)
+        self.nproc = tuple(int(p) for p in nproc)
+        self.shape = (case["nx"] + 6, case["ny"] + 6, case["nz"] + 6, 5)
+        h = ctypes.c_void_p()
+        rc = self.L.nsgpu_domain_create(ctypes.byref(self.cfg), (ctypes.c_int * 3)(*self.nproc),
+                                        device, ctypes.byref(h))
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+        self.h = h
+        self.blocks = []
+        for r in range(self.L.nsgpu_domain_size(h)):
+            st, ct = (ctypes.c_int * 3)(), (ctypes.c_int * 3)()
+            bh = self.L.nsgpu_domain_block(h, r, st, ct)
+            st, ct = tuple(st), tuple(ct)
+            self.blocks.append(_Block(self.L, ctypes.c_void_p(bh),
+                                      (ct[0] + 6, ct[1] + 6, ct[2] + 6, 5), st, ct))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.nsgpu_domain_destroy(self.h)
+            self.h = None
+            self.blocks = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _slices(self, b):
+        return tuple(slice(3 + s, 3 + s + c) for s, c in zip(b.start, b.count))
+
+    def set_state(self, A, kind="primitive"):
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        if A.shape != self.shape:
+            raise ValueError(f"array shape {A.shape} != {self.shape}")
+        for b in self.blocks:
+            local = np.zeros(b.shape)
+            local[3:-3, 3:-3, 3:-3] = A[self._slices(b)]
+            b.set_state(local, kind)
+
+    def get_state(self, kind="primitive"):
+        out = np.zeros(self.shape)
+        for b in self.blocks:
+            out[self._slices(b)] = b.get_state(kind)[3:-3, 3:-3, 3:-3]
+        return out
+
+    def step(self, nsteps=1, time=0.0):
+        """Advances every block; returns (time, last dt); raises on a failed health check."""
+        t = np.array([time], dtype=np.float64)
+        dt = np.zeros(1)
+        rc = self.L.nsgpu_domain_step(self.h, nsteps, _ptr(t), _ptr(dt))
+        if rc != 0:
+            raise NsgpuError(rc, self.L.nsgpu_strerror(rc).decode())
+        return float(t[0]), float(dt[0])
+
+    def reduce(self):
+        """Conservation sums over all blocks, the CFL dt of the whole domain, the OR of the
+        blocks' health flags and the worst status."""
+        sums, dts, flags = np.zeros(5), [], 0
+        for b in self.blocks:
+            s, dt, fl, _ = b.reduce()
+            sums += s
+            dts.append(dt)
+            flags |= fl
+        status = -4 if flags & 1 else (-5 if flags & 2 else (-6 if flags & 4 else 0))
+        return sums, min(dts), flags, status

@@ -34,9 +34,23 @@ struct Geo {
 };
 
 // per-face boundary action (resolved on the host from nsgpu_config::bc)
+constexpr int FACE_PERIODIC = 0;  // the block's own opposite interior planes (single-rank wrap)
+constexpr int FACE_NEUMANN  = 1;  // the first interior plane (a physical wall)
+constexpr int FACE_LINK     = 2;  // the neighbouring block of a decomposition (its interior planes)
 struct Bnd {
-    int periodic[3];
-    int face_kind[6];  // W,E,N,S,F,B: 0 periodic, 1 Neumann
+    int periodic[3];   // the axis wraps onto the block itself
+    int face_kind[6];  // W,E,N,S,F,B: FACE_*
+};
+
+// source of a linked face: the neighbour's arrays, its extent along the face normal and its
+// strides (a neighbour has the block's tangential extents, but its rows and planes are as long as
+// its own extents make them); unused for the other kinds
+struct HaloLink {
+    const double* A;
+    const double* B;
+    int           n;
+    int           sx, sy;
+    long          P;
 };
 
 struct MlpPacked;  // ns_mlp.h
@@ -100,6 +114,8 @@ struct EnsBlock {
     unsigned long long* dtmin;
     double*  dtime;
     unsigned* flags;  // health word of the ensemble step (shared by all blocks)
+    int      nbr[6];  // member index of the neighbour behind each linked face, -1 elsewhere
+    int      linked;  // the block has linked faces: the separate derivative path, exchanges
     EnsTiles t;
     int      grid[EF_COUNT][3];  // local grid of each family (x fastest); 0 = no work
 };

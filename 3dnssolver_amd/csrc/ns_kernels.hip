@@ -905,10 +905,16 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
 // copy applied to the primitive planes gives c2p(U) on the halo bit for bit (c2p is per cell).
 // ------------------------------------------------------------------------------------------
 // block (64, 4); (bx, by, bz): bz = face, by = (row tile, layer), bx = lane tile
+// A linked face (FACE_LINK) copies the neighbour's interior planes next to it (`lk`: its arrays and
+// extent along the normal; same tangential layout), as Communicator::communicate does between
+// ranks (communicator.cpp:16-144): faces only, like every other face.
 __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, const Geo& g,
-                                          const Bnd& b, int bx, int by, int bz) {
+                                          const Bnd& b, int bx, int by, int bz,
+                                          const HaloLink* lk = nullptr) {
     int face = bz;
     int axis = face >> 1;
+    const bool link = b.face_kind[face] == FACE_LINK;
+    if (link && !(lk && lk->A)) return;  // a linked face is filled by its decomposition only
     // a 2-D block thinner than the halo (nz < 3): its z halos are never read (no z sweep, 2-D
     // viscous terms) and their periodic copy would read halo cells being written: left as is
     if (axis == 2 && g.nz < NGC) return;
@@ -933,24 +939,31 @@ __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, 
     }
     int dst = low ? h : NGC + n + h;
     int src;
-    if (b.periodic[axis]) src = low ? n + h : NGC + h;
-    else src = low ? NGC : NGC + n - 1;
-    long ct, cd, cs;
-    if (axis == 0) {
-        ct = (long)(t1 + NGC) * g.sy + (t2 + NGC);
-        cd = ct + (long)dst * g.sx;
-        cs = ct + (long)src * g.sx;
-    } else if (axis == 1) {
-        ct = (long)(t1 + NGC) * g.sx + (t2 + NGC);
-        cd = ct + (long)dst * g.sy;
-        cs = ct + (long)src * g.sy;
+    const double* sA = A;
+    const double* sB = B;
+    int           sx = g.sx, sy = g.sy;  // strides of the source
+    long          sP = g.P;
+    if (link) {
+        src = low ? lk->n + h : NGC + h, sA = lk->A, sB = lk->B;
+        sx = lk->sx, sy = lk->sy, sP = lk->P;
+    } else if (b.periodic[axis]) {
+        src = low ? n + h : NGC + h;
     } else {
-        ct = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy;
-        cd = ct + dst;
-        cs = ct + src;
+        src = low ? NGC : NGC + n - 1;
     }
-    for (int q = 0; q < nA; q++) A[cd + q * g.P] = A[cs + q * g.P];
-    for (int q = 0; q < nB; q++) B[cd + q * g.P] = B[cs + q * g.P];
+    long cd, cs;
+    if (axis == 0) {
+        cd = (long)dst * g.sx + (long)(t1 + NGC) * g.sy + (t2 + NGC);
+        cs = (long)src * sx + (long)(t1 + NGC) * sy + (t2 + NGC);
+    } else if (axis == 1) {
+        cd = (long)(t1 + NGC) * g.sx + (long)dst * g.sy + (t2 + NGC);
+        cs = (long)(t1 + NGC) * sx + (long)src * sy + (t2 + NGC);
+    } else {
+        cd = (long)(t1 + NGC) * g.sx + (long)(t2 + NGC) * g.sy + dst;
+        cs = (long)(t1 + NGC) * sx + (long)(t2 + NGC) * sy + src;
+    }
+    for (int q = 0; q < nA; q++) A[cd + q * g.P] = sA[cs + q * sP];
+    for (int q = 0; q < nB; q++) B[cd + q * g.P] = sB[cs + q * sP];
 }
 
 __global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
@@ -1135,28 +1148,39 @@ __global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const
 // dW/dx and the i halos of dW/dy (centered_diffusion.cpp:22-24).  At a physical wall the halos
 // stay zero: the arrays are zeroed once at allocation and never written there.
 // block (64, 4); (bx, by, bz) = (k tile, row tile, (side, low/high, layer))
+// A linked face copies the neighbour's derivative planes (`lk` of that face: its Dx in A, its Dy
+// in B, its extent along the normal), as the derivative exchange between ranks does.
 __device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Geo& g,
-                                                const Bnd& b, int bx, int by, int bz, int q0 = 0) {
+                                                const Bnd& b, int bx, int by, int bz, int q0 = 0,
+                                                const HaloLink* lk = nullptr) {
     int  k    = threadIdx.x + bx * 64;
     int  t1   = threadIdx.y + by * 4;
     int  side = bz / (2 * NGC);  // 0: Dx j-halo, 1: Dy i-halo
     int  h    = bz % NGC;
     bool low  = ((bz / NGC) & 1) == 0;
     if (k >= g.nz) return;
+    // faces N (2), S (3) for the j halos, W (0), E (1) for the i halos
+    const int  face = side == 0 ? (low ? 2 : 3) : (low ? 0 : 1);
+    const bool link = b.face_kind[face] == FACE_LINK && lk && lk->A;
+    // strides of the source (the neighbour's own, see HaloLink)
+    const int  sx = link ? lk->sx : g.sx, sy = link ? lk->sy : g.sy;
+    const long sP = link ? lk->P : g.P;
     if (side == 0) {
-        if (!b.periodic[1] || t1 >= g.nx) return;
-        long base = (long)(t1 + NGC) * g.sx + (k + NGC);
-        int  dst  = low ? h : NGC + g.ny + h;
-        int  src  = low ? g.ny + h : NGC + h;
-        for (int q = q0; q < 4; q++)
-            Dx[base + (long)dst * g.sy + q * g.P] = Dx[base + (long)src * g.sy + q * g.P];
+        if (!(b.periodic[1] || link) || t1 >= g.nx) return;
+        int  dst = low ? h : NGC + g.ny + h;
+        int  src = low ? (link ? lk->n : g.ny) + h : NGC + h;
+        long cd  = (long)(t1 + NGC) * g.sx + (long)dst * g.sy + (k + NGC);
+        long cs  = (long)(t1 + NGC) * sx + (long)src * sy + (k + NGC);
+        const double* S = link ? lk->A : Dx;
+        for (int q = q0; q < 4; q++) Dx[cd + q * g.P] = S[cs + q * sP];
     } else {
-        if (!b.periodic[0] || t1 >= g.ny) return;
-        long base = (long)(t1 + NGC) * g.sy + (k + NGC);
-        int  dst  = low ? h : NGC + g.nx + h;
-        int  src  = low ? g.nx + h : NGC + h;
-        for (int q = 0; q < 4; q++)
-            Dy[base + (long)dst * g.sx + q * g.P] = Dy[base + (long)src * g.sx + q * g.P];
+        if (!(b.periodic[0] || link) || t1 >= g.ny) return;
+        int  dst = low ? h : NGC + g.nx + h;
+        int  src = low ? (link ? lk->n : g.nx) + h : NGC + h;
+        long cd  = (long)dst * g.sx + (long)(t1 + NGC) * g.sy + (k + NGC);
+        long cs  = (long)src * sx + (long)(t1 + NGC) * sy + (k + NGC);
+        const double* S = link ? lk->B : Dy;
+        for (int q = 0; q < 4; q++) Dy[cd + q * g.P] = S[cs + q * sP];
     }
 }
 
@@ -2056,13 +2080,28 @@ __global__ __launch_bounds__(256) void kb_step_begin(EnsTable T, double* Ua) {
     step_begin_body(W, Ua, Un, Ws, dtmin, g, gam, bx, nwg);
 }
 
+// the link of face `f` of block E: the neighbour's stage arrays and its extent along the normal
+__device__ __forceinline__ HaloLink ens_link(const EnsTable& T, const EnsBlock* E, int f, int s) {
+    const int nb = E->nbr[f];
+    if (nb < 0) return HaloLink{nullptr, nullptr, 0, 0, 0, 0};
+    const EnsBlock* N = T.blk + nb;
+    const int       a = f >> 1;
+    return HaloLink{ens_u(N, s), ens_w(N, s), a == 0 ? N->g.nx : (a == 1 ? N->g.ny : N->g.nz),
+                    N->g.sx, N->g.sy, N->g.P};
+}
+
 __global__ void kb_halo(EnsTable T, int s) {
     int             bx, by, bz;
     const EnsBlock* E  = T.blk + ens_locate(T, EF_HALO, bx, by, bz);
     const Geo       g  = E->g;
     const Bnd       bd = E->b;
     double *U = ens_u(E, s), *Wp = ens_w(E, s);
-    halo_body(U, 5, Wp, 5, g, bd, bx, by, bz);
+    if (E->linked) {
+        const HaloLink lk = ens_link(T, E, bz, s);  // bz is the face
+        halo_body(U, 5, Wp, 5, g, bd, bx, by, bz, &lk);
+    } else {
+        halo_body(U, 5, Wp, 5, g, bd, bx, by, bz);
+    }
 }
 
 template <int DIR, int CONV, int WT, int OPT>
@@ -2108,8 +2147,9 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
 
 // ensemble members take the same derivative split as single blocks (k_residual_fd): dW/dx of
 // u and v from the derivative pass, the rest formed in the residual
-__device__ __forceinline__ bool ens_fused(const Geo& g) {
-    return NS_FUSE_DIFF && g.P < (1L << 31);
+// (blocks of a decomposition take the separate path: their derivative halos are exchanged)
+__device__ __forceinline__ bool ens_fused(const Geo& g, int linked = 0) {
+    return NS_FUSE_DIFF && g.P < (1L << 31) && !linked;
 }
 
 template <int ORDER>
@@ -2120,7 +2160,7 @@ __global__ __launch_bounds__(256) void kb_deriv1(EnsTable T, int s) {
     const int       kw = E->t.kw_cell, march = E->t.march;
     const double*   Wp = ens_w(E, s);
     double *Dx = E->Dx, *Dy = E->Dy;
-    if (ens_fused(g)) deriv1_body<ORDER, false>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
+    if (ens_fused(g, E->linked)) deriv1_body<ORDER, false>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
     else deriv1_body<ORDER>(Wp, Dx, Dy, g, bx, by, bz, kw, march);
 }
 
@@ -2130,8 +2170,22 @@ __global__ void kb_deriv_halo(EnsTable T) {
     const Geo       g  = E->g;
     const Bnd       bd = E->b;
     double *Dx = E->Dx, *Dy = E->Dy;
-    if (!ens_fused(g)) deriv_halo_body(Dx, Dy, g, bd, bx, by, bz);
-    else if (bz < 2 * NGC) deriv_halo_body(Dx, Dy, g, bd, bx, by, bz, 2);  // j halos of du/dx, dv/dx
+    if (E->linked) {
+        // the face of this (side, low/high) slice, and its neighbour's derivative planes
+        const int side = bz / (2 * NGC), low = ((bz / NGC) & 1) == 0;
+        const int f    = side == 0 ? (low ? 2 : 3) : (low ? 0 : 1);
+        const int nb   = E->nbr[f];
+        HaloLink  lk{nullptr, nullptr, 0, 0, 0, 0};
+        if (nb >= 0) {
+            const EnsBlock* N = T.blk + nb;
+            lk = HaloLink{N->Dx, N->Dy, side == 0 ? N->g.ny : N->g.nx, N->g.sx, N->g.sy, N->g.P};
+        }
+        deriv_halo_body(Dx, Dy, g, bd, bx, by, bz, 0, &lk);
+    } else if (!ens_fused(g)) {
+        deriv_halo_body(Dx, Dy, g, bd, bx, by, bz);
+    } else if (bz < 2 * NGC) {
+        deriv_halo_body(Dx, Dy, g, bd, bx, by, bz, 2);  // j halos of du/dx, dv/dx
+    }
 }
 
 // which / Rin / Rout are launch arguments (3, null, null) rather than constants, so that the body
@@ -2150,7 +2204,7 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
                     s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s, E->flags};
     const unsigned long long* dtmin = E->dtmin;
     if constexpr (ORDER > 0) {
-        if (ens_fused(g)) {
+        if (ens_fused(g, E->linked)) {
             const Bnd bd = E->b;
             if (g.square_xy)
                 residual_fd_body<ORDER, true, SUTH>(fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin, which,

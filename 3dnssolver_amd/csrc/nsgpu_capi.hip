@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -70,6 +71,8 @@ struct nsgpu_ctx {
     double             dt_scale = 0.0;    // dt = dtmin[0] bits * dt_scale: the CFL, or 1 after a
                                           // prescribed-dt step (dtmin[0] then holds dt itself)
     int                enrolled = 0;      // ensembles this context is a member of
+    int                linked   = 0;      // a block of a decomposition (nsgpu_domain): faces
+                                          // linked to its neighbours; stepped by its domain only
     float*             mlp_params = nullptr;  // learned weights: FP32 parameters
     MlpPacked*         mlp_packed = nullptr;  // and their packed BF16 fragments
     bool               timing = false;
@@ -187,6 +190,16 @@ int download_soa(nsgpu_ctx* c, const double* src, double* aos) {
     return NSGPU_OK;
 }
 
+// grid spacings and the quantities derived from them
+void set_spacing(Geo& g, double dx, double dy, double dz) {
+    g.dx = dx, g.dy = dy, g.dz = dz;
+    g.dxx = g.dx * g.dx;  // CDN::ddf_dxdx / ddf_dydy divisors (cd-n.cpp:67-79)
+    g.dyy = g.dy * g.dy;
+    g.idx = 1.0 / g.dx, g.idy = 1.0 / g.dy, g.idz = 1.0 / g.dz;
+    g.idxx = 1.0 / g.dxx, g.idyy = 1.0 / g.dyy;
+    g.square_xy = (g.dxx == g.dyy);
+}
+
 // status of a health word (bit 0 non-finite, bit 1 positivity, bit 2 alpha)
 int health_status(unsigned fl) {
     if (fl & 1u) return NSGPU_ERR_NONFINITE;
@@ -220,7 +233,7 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts
 // the residual kernel forms the y derivatives of the 2-D viscous terms itself
 bool fused_diffusion(const nsgpu_ctx* c) {
     return c->L->fuse_diff && c->ph.diff_order && !(c->ph.physics & NSGPU_PHYS_VISCOUS) &&
-           c->g.P < (1L << 31);
+           c->g.P < (1L << 31) && !c->linked;
 }
 
 void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
@@ -318,15 +331,9 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     g.sy = (g.NZ + KALIGN - 1) / KALIGN * KALIGN;
     g.sx = g.NY * g.sy;
     g.P  = (long)g.NX * g.sx;
-    g.dx = cfg->Lx / cfg->nx;  // Grid::compute_step_sizes (grid.cpp:14-16)
-    g.dy = cfg->Ly / cfg->ny;
-    g.dz = cfg->Lz / cfg->nz;
-    g.dxx = g.dx * g.dx;  // CDN::ddf_dxdx / ddf_dydy divisors (cd-n.cpp:67-79)
-    g.dyy = g.dy * g.dy;
-    g.idx = 1.0 / g.dx, g.idy = 1.0 / g.dy, g.idz = 1.0 / g.dz;
-    g.idxx = 1.0 / g.dxx, g.idyy = 1.0 / g.dyy;
-    g.dim       = cfg->dim;
-    g.square_xy = (g.dxx == g.dyy);
+    // Grid::compute_step_sizes (grid.cpp:14-16)
+    set_spacing(g, cfg->Lx / cfg->nx, cfg->Ly / cfg->ny, cfg->Lz / cfg->nz);
+    g.dim = cfg->dim;
     static const int lead[3] = {0, 3, 4};
     for (int a = 0; a < 3; a++) c->b.periodic[a] = (cfg->bc[lead[a]] == NSGPU_BC_PERIODIC);
     for (int f = 0; f < 6; f++) c->b.face_kind[f] = cfg->bc[f];
@@ -459,6 +466,7 @@ int nsgpu_get_state(nsgpu_ctx* c, double* aos, int kind) {
 
 int nsgpu_apply_boundary(nsgpu_ctx* c, double* aos) {
     if (!c || !aos) return NSGPU_ERR_INVALID;
+    if (c->linked) return NSGPU_ERR_INVALID;  // a decomposition's block: through its domain
     HIPCHK(hipSetDevice(c->device));
     int rc = upload_soa(c, aos, c->R);
     if (rc) return rc;
@@ -471,6 +479,7 @@ int nsgpu_apply_boundary(nsgpu_ctx* c, double* aos) {
 // stays as it was, so stepping, get_state and reduce are unaffected by them.
 int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_aos, double* R_aos) {
     if (!c || !W_aos || !U_aos || !R_aos || !net_ready(c)) return NSGPU_ERR_INVALID;
+    if (c->linked) return NSGPU_ERR_INVALID;  // a decomposition's block: through its domain
     HIPCHK(hipSetDevice(c->device));
     double* Wt = c->Ws[0];
     int     rc = upload_soa(c, W_aos, Wt);
@@ -487,6 +496,7 @@ int nsgpu_convection_solve(nsgpu_ctx* c, const double* W_aos, const double* U_ao
 
 int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
     if (!c || !W_aos || !R_aos) return NSGPU_ERR_INVALID;
+    if (c->linked) return NSGPU_ERR_INVALID;  // a decomposition's block: through its domain
     HIPCHK(hipSetDevice(c->device));
     double* Wt = c->Ws[0];
     int     rc = upload_soa(c, W_aos, Wt);
@@ -502,6 +512,7 @@ int nsgpu_diffusion_solve(nsgpu_ctx* c, const double* W_aos, double* R_aos) {
 
 int nsgpu_rhs(nsgpu_ctx* c, double* U_aos, double* W_out, double* R_out, int which) {
     if (!c || !U_aos || !R_out || which < 1 || which > 3 || !net_ready(c)) return NSGPU_ERR_INVALID;
+    if (c->linked) return NSGPU_ERR_INVALID;  // a decomposition's block: through its domain
     HIPCHK(hipSetDevice(c->device));
     double* Ut = c->Ub;
     double* Wt = c->Ws[0];
@@ -552,7 +563,7 @@ struct CflGuard {
 // accumulation form dt = dtmin[0] * ph.cfl; a prescribed dt is stored as dtmin[0] = fixed_dt
 // with ph.cfl = 1 for the call, so every stage and the time use fixed_dt itself (bitwise).
 int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double* dt_last) {
-    if (!c || nsteps < 0 || !net_ready(c)) return NSGPU_ERR_INVALID;
+    if (!c || nsteps < 0 || !net_ready(c) || c->linked) return NSGPU_ERR_INVALID;
     HIPCHK(hipSetDevice(c->device));
     CflGuard guard{c, c->ph.cfl};
     if (fixed_dt > 0.0) c->ph.cfl = 1.0;
@@ -760,6 +771,9 @@ struct nsgpu_ensemble {
     hipGraphExec_t          exec   = nullptr;
     hipEvent_t              ev[2]  = {nullptr, nullptr};
     unsigned*               flags  = nullptr;  // health word of the stage outputs (all blocks)
+    // decompositions (nsgpu_domain): neighbour member per face and the shared CFL minimum
+    std::vector<std::array<int, 6>> nbr;
+    unsigned long long*             shared_dt = nullptr;
 };
 
 namespace {
@@ -784,9 +798,11 @@ int ens_build(nsgpu_ensemble* e) {
         k.Ws[0] = c->Ws[0], k.Ws[1] = c->Ws[1];
         for (int d = 0; d < 3; d++) k.F[d] = c->F[d];
         k.Dx = c->Dx, k.Dy = c->Dy;
-        k.dtmin = &c->dtmin[0];
+        k.dtmin = e->shared_dt ? e->shared_dt : &c->dtmin[0];
         k.dtime = c->dtime;
         k.flags = e->flags;
+        k.linked = c->linked;
+        for (int f = 0; f < 6; f++) k.nbr[f] = e->nbr.empty() ? -1 : e->nbr[b][f];
         k.t     = e->tiles[b];
         ens_grids(k);
     }
@@ -922,16 +938,17 @@ int ens_trial(nsgpu_ensemble* e, const std::vector<EnsTiles>& t, int steps, cons
     return rr ? rr : rc;
 }
 
-}  // namespace
-
-extern "C" {
-
-int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out) {
+// ensemble creation; `nbr` (per member, per face: the neighbour member or -1) and `shared_dt`
+// link the members into one decomposed block (nsgpu_domain), else both are empty / null
+int ens_create(nsgpu_ctx* const* members, int n, const std::vector<std::array<int, 6>>& nbr,
+               unsigned long long* shared_dt, nsgpu_ensemble** out) {
     if (!out) return NSGPU_ERR_INVALID;
     *out = nullptr;
     if (!members || n < 1 || n > 65536) return NSGPU_ERR_INVALID;
     const nsgpu_ctx* c0 = members[0];
     if (!c0) return NSGPU_ERR_INVALID;
+    for (int b = 0; b < n; b++)
+        if (!members[b] || (members[b]->linked != 0) != !nbr.empty()) return NSGPU_ERR_INVALID;
     for (int b = 0; b < n; b++) {
         const nsgpu_ctx* c = members[b];
         if (!c || c->device != c0->device || c->L != c0->L || c->ph.conv != c0->ph.conv ||
@@ -949,7 +966,9 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
     nsgpu_ensemble* e = new (std::nothrow) nsgpu_ensemble();
     if (!e) return NSGPU_ERR_INVALID;
     e->m.assign(members, members + n);
-    e->device = c0->device;
+    e->device    = c0->device;
+    e->nbr       = nbr;
+    e->shared_dt = shared_dt;
     e->tiles.assign(n, EnsTiles{64, 64, 16});
     int rc = NSGPU_OK;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
@@ -968,6 +987,14 @@ int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out
     }
     *out = e;
     return NSGPU_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int nsgpu_ensemble_create(nsgpu_ctx* const* members, int n, nsgpu_ensemble** out) {
+    // blocks of a decomposition belong to their domain's own ensemble
+    return ens_create(members, n, {}, nullptr, out);
 }
 
 void nsgpu_ensemble_destroy(nsgpu_ensemble* e) {
@@ -1072,6 +1099,175 @@ int nsgpu_ensemble_tune(nsgpu_ensemble* e, int steps, double* ms_default, double
     }
     rc = ens_set_tiles(e, best);
     if (ms_tuned) *ms_tuned = tbest;
+    return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// decomposed block on one GPU (include/nsgpu.h): the reference's Cartesian decomposition
+// (decomposition.cpp:26-57) into member contexts whose faces are linked to their neighbours, and
+// the exchange of Communicator::communicate (communicator.cpp:16-144) as halo kernels that read
+// the neighbouring member's interior planes; dt is one minimum over all blocks
+// (time_scheme.cpp:131-135, MPI_Allreduce MIN).  The members are advanced by a linked ensemble:
+// one launch per kernel family over every block, one hipGraph per step.
+// ------------------------------------------------------------------------------------------
+struct nsgpu_domain {
+    nsgpu_config                    global{};
+    int                             p[3]   = {1, 1, 1};
+    int                             device = 0;
+    std::vector<nsgpu_ctx*>         blocks;  // rank = (bk * py + bj) * px + bi
+    std::vector<std::array<int, 3>> start, count;
+    nsgpu_ensemble*                 ens = nullptr;
+    unsigned long long*             dt  = nullptr;  // shared CFL minimum (device word)
+};
+
+namespace {
+
+// block sizes of one axis: n / p each, the remainder to block 0 (decomposition.cpp:41-52), and
+// the start indices (decomposition.cpp:84-90)
+void split_axis(int n, int p, int b, int* start, int* count) {
+    const int local = n / p, rem = n % p;
+    *count = local + (b == 0 ? rem : 0);
+    *start = b * local + (b > 0 ? rem : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int nsgpu_domain_create(const nsgpu_config* cfg, const int nproc[3], int device,
+                        nsgpu_domain** out) {
+    if (!out) return NSGPU_ERR_INVALID;
+    *out = nullptr;
+    if (!cfg || !nproc) return NSGPU_ERR_INVALID;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    const int n[3] = {cfg->nx, cfg->ny, cfg->nz};
+    long      nb   = 1;
+    for (int a = 0; a < 3; a++) {
+        if (nproc[a] < 1 || nproc[a] > 1024) return NSGPU_ERR_INVALID;
+        nb *= nproc[a];
+        // every block at least as thick as the halo it hands to its neighbour
+        if (nproc[a] > 1 && n[a] / nproc[a] < NGC) return NSGPU_ERR_INVALID;
+    }
+    if (nb > 65536 || (cfg->dim == 2 && nproc[2] > 1)) return NSGPU_ERR_INVALID;
+    if (cfg->physics & NSGPU_PHYS_VISCOUS) return NSGPU_ERR_UNSUPPORTED;  // ensembles carry the
+                                                                        // reference's terms only
+    nsgpu_domain* d = new (std::nothrow) nsgpu_domain();
+    if (!d) return NSGPU_ERR_INVALID;
+    d->global = *cfg;
+    d->device = device;
+    for (int a = 0; a < 3; a++) d->p[a] = nproc[a];
+    static const int lead[3] = {0, 3, 4};
+    bool periodic[3];
+    for (int a = 0; a < 3; a++) periodic[a] = cfg->bc[lead[a]] == NSGPU_BC_PERIODIC;
+    // grid spacing of every block: the global one (grid.cpp:14-16)
+    const double dx = cfg->Lx / cfg->nx, dy = cfg->Ly / cfg->ny, dz = cfg->Lz / cfg->nz;
+    std::vector<std::array<int, 6>> nbr;
+    auto rank_of = [&](int i, int j, int k) { return (k * nproc[1] + j) * nproc[0] + i; };
+    for (int k = 0; k < nproc[2] && !rc; k++)
+        for (int j = 0; j < nproc[1] && !rc; j++)
+            for (int i = 0; i < nproc[0] && !rc; i++) {
+                const int           bc[3] = {i, j, k};
+                std::array<int, 3> st, ct;
+                for (int a = 0; a < 3; a++) split_axis(n[a], nproc[a], bc[a], &st[a], &ct[a]);
+                nsgpu_config m = *cfg;
+                m.nx = ct[0], m.ny = ct[1], m.nz = ct[2];
+                // the block's length at the global spacing; the spacing is set exactly below
+                m.Lx = dx * ct[0], m.Ly = dy * ct[1], m.Lz = dz * ct[2];
+                nsgpu_ctx* c = nullptr;
+                rc = nsgpu_create(&m, device, &c);
+                if (rc) break;
+                d->blocks.push_back(c);
+                d->start.push_back(st);
+                d->count.push_back(ct);
+                set_spacing(c->g, dx, dy, dz);
+                // faces W, E (axis 0), N, S (1), F (high), B (low) (2): linked where a neighbouring
+                // block lies behind (including the wrap of a periodic axis over several
+                // blocks), the block's own wrap for a periodic axis of one block, walls otherwise
+                std::array<int, 6> nb6;
+                nb6.fill(-1);
+                bool linked = false;
+                for (int a = 0; a < 3; a++) {
+                    const int faces[2] = {a == 2 ? 5 : 2 * a, a == 2 ? 4 : 2 * a + 1};  // low, high
+                    c->b.periodic[a]   = periodic[a] && nproc[a] == 1;
+                    for (int side = 0; side < 2; side++) {
+                        int       o    = bc[a] + (side ? 1 : -1);
+                        const int f    = faces[side];
+                        bool      edge = (o < 0 || o >= nproc[a]);
+                        if (edge && periodic[a] && nproc[a] > 1) o = (o + nproc[a]) % nproc[a], edge = false;
+                        if (nproc[a] == 1 || edge) {
+                            c->b.face_kind[f] = periodic[a] ? FACE_PERIODIC : FACE_NEUMANN;
+                        } else {
+                            int nbc[3] = {i, j, k};
+                            nbc[a]     = o;
+                            nb6[f]     = rank_of(nbc[0], nbc[1], nbc[2]);
+                            c->b.face_kind[f] = FACE_LINK;
+                            linked            = true;
+                        }
+                    }
+                }
+                c->linked = linked ? 1 : 0;
+                nbr.push_back(nb6);
+            }
+    if (!rc && hipSetDevice(device) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc && hipMalloc((void**)&d->dt, sizeof(unsigned long long)) != hipSuccess) rc = NSGPU_ERR_HIP;
+    if (!rc) {
+        // a decomposition of one block has no links: an ordinary one-member ensemble
+        const bool any = std::any_of(d->blocks.begin(), d->blocks.end(),
+                                     [](const nsgpu_ctx* c) { return c->linked != 0; });
+        rc = ens_create(d->blocks.data(), (int)d->blocks.size(),
+                        any ? nbr : std::vector<std::array<int, 6>>{}, d->dt, &d->ens);
+    }
+    if (rc) {
+        nsgpu_domain_destroy(d);
+        return rc;
+    }
+    *out = d;
+    return NSGPU_OK;
+}
+
+void nsgpu_domain_destroy(nsgpu_domain* d) {
+    if (!d) return;
+    if (d->ens) nsgpu_ensemble_destroy(d->ens);
+    for (nsgpu_ctx* c : d->blocks) nsgpu_destroy(c);
+    if (d->dt) {
+        hipSetDevice(d->device);
+        hipFree(d->dt);
+    }
+    delete d;
+}
+
+int nsgpu_domain_size(const nsgpu_domain* d) { return d ? (int)d->blocks.size() : 0; }
+
+nsgpu_ctx* nsgpu_domain_block(nsgpu_domain* d, int rank, int start[3], int count[3]) {
+    if (!d || rank < 0 || rank >= (int)d->blocks.size()) return nullptr;
+    for (int a = 0; a < 3; a++) {
+        if (start) start[a] = d->start[rank][a];
+        if (count) count[a] = d->count[rank][a];
+    }
+    return d->blocks[rank];
+}
+
+int nsgpu_domain_step(nsgpu_domain* d, int nsteps, double* time_io, double* dt_last) {
+    if (!d || nsteps < 0) return NSGPU_ERR_INVALID;
+    HIPCHK(hipSetDevice(d->device));
+    if (time_io)
+        for (nsgpu_ctx* c : d->blocks) {
+            HIPCHK(hipStreamSynchronize(c->stream));
+            HIPCHK(hipMemcpy(c->dtime, time_io, sizeof(double), hipMemcpyHostToDevice));
+        }
+    const int rc = nsgpu_ensemble_step(d->ens, nsteps, 1);
+    if (rc != NSGPU_OK && rc != NSGPU_ERR_NONFINITE && rc != NSGPU_ERR_POSITIVITY) return rc;
+    if (time_io) HIPCHK(hipMemcpy(time_io, d->blocks[0]->dtime, sizeof(double), hipMemcpyDeviceToHost));
+    if (dt_last) {
+        unsigned long long bits = 0;
+        HIPCHK(hipMemcpy(&bits, d->dt, sizeof bits, hipMemcpyDeviceToHost));
+        double m;
+        std::memcpy(&m, &bits, sizeof m);
+        *dt_last = m * d->global.cfl;
+    }
     return rc;
 }
 

@@ -270,6 +270,35 @@ int nsgpu_ensemble_step(nsgpu_ensemble* ens, int nsteps, int use_graph);
  * leaves the chosen tiles set. */
 int nsgpu_ensemble_tune(nsgpu_ensemble* ens, int steps, double* ms_default, double* ms_tuned);
 
+/* ---- Decomposed block: several linked blocks on one GPU --------------------------------------
+ * Replaces the reference's multi-rank run of one block per MPI rank: Decomposition
+ * (decomposition.cpp:26-57: n / p cells per block, the remainder to block 0; neighbours from the
+ * Cartesian topology, periodic where West / South / Front say so, :116-161), Grid's global spacing
+ * (grid.cpp:14-16), Communicator::communicate (communicator.cpp:6-144: the 3-layer face halos of
+ * each block from its neighbours, faces only) and the dt minimum over ranks (time_scheme.cpp:131-135,
+ * MPI_Allreduce MIN).  The blocks live on one GPU: halo kernels copy the neighbouring block's
+ * interior planes, the derivative halos of the viscous terms are exchanged the same way
+ * (centered_diffusion.cpp:22-24), and one SSP-RK3 step of all blocks is one launch per kernel
+ * family, replayed as one hipGraph.  The result of every block is bitwise that of the same cells
+ * advanced as one block.  nproc[3] = blocks per axis (x, y, z); dim = 2 needs nproc[2] = 1;
+ * every block at least 3 cells thick along a split axis; the full 3-D viscous option is not
+ * supported (NSGPU_ERR_UNSUPPORTED). */
+typedef struct nsgpu_domain nsgpu_domain;
+
+int  nsgpu_domain_create(const nsgpu_config* global, const int nproc[3], int device,
+                         nsgpu_domain** out);
+void nsgpu_domain_destroy(nsgpu_domain* dom);
+/* number of blocks (nproc[0] * nproc[1] * nproc[2]) */
+int  nsgpu_domain_size(const nsgpu_domain* dom);
+/* Block `rank` (rank = (k * nproc[1] + j) * nproc[0] + i for block coordinates i, j, k, the
+ * order of the reference's Cartesian communicator): its first global cell index and its cell
+ * counts per axis, and its context, owned by the domain.  Use nsgpu_set_state / nsgpu_get_state /
+ * nsgpu_reduce on it with the block's own padded arrays (Domain::get_NS_primitive_array of that
+ * rank); the operator-level calls and nsgpu_step refuse it (NSGPU_ERR_INVALID). */
+nsgpu_ctx* nsgpu_domain_block(nsgpu_domain* dom, int rank, int start[3], int count[3]);
+/* nsteps SSP-RK3 steps of all blocks, as nsgpu_step (time, last dt, health over all blocks). */
+int nsgpu_domain_step(nsgpu_domain* dom, int nsteps, double* time_io, double* dt_last);
+
 #ifdef __cplusplus
 }
 #endif
