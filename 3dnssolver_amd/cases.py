@@ -340,3 +340,117 @@ def riemann_case(name, n=64, nz=4):
         case = dict(base_case(nz, n, n), L=(nz / n, 1.0, 1.0), conv="weno5", diff_order=0,
                     mu=0.0, bc=["periodic", "periodic", "neumann", "neumann", "neumann", "neumann"])
     return case, riemann_W(case, name)
+
+
+# ---- the reference's shear layers (testcases.cpp:250-344) ---------------------------------------
+# They draw their perturbations from the C library's rand() (unseeded, so seed 1, RAND_MAX =
+# 2^31 - 1) once or twice per cell in i, j, k order over each rank's block: every rank of a
+# decomposed run starts the sequence afresh.  CRand restates that generator (the additive
+# feedback generator of the GNU C library, its default TYPE_3 state: r[0] = seed,
+# r[i] = 16807 r[i-1] mod (2^31 - 1) for i < 31, r[i] = r[i-31] for i = 31..33,
+# r[i] = r[i-3] + r[i-31] mod 2^32 after that, outputs r[i] >> 1 from i = 344 on).
+RAND_MAX = 2147483647
+
+
+class CRand:
+    """The C library's rand() sequence from srand(seed) (pure Python: ~0.1 us per value)."""
+
+    def __init__(self, seed=1):
+        seed = seed or 1
+        r = [seed]
+        for i in range(1, 31):
+            hi, lo = divmod(r[i - 1], 127773)
+            word = 16807 * lo - 2836 * hi
+            if word < 0:
+                word += 2147483647
+            r.append(word)
+        r += r[:3]
+        for i in range(34, 344):
+            r.append((r[i - 31] + r[i - 3]) & 0xFFFFFFFF)
+        self.r = r[-31:]  # the last 31 values: the lags the recurrence reads
+
+    def take(self, n):
+        """The next n outputs as a float64 array divided by RAND_MAX, as (double)rand() /
+        (RAND_MAX) evaluates them."""
+        r = self.r
+        out = np.empty(n, dtype=np.float64)
+        for k in range(n):
+            v = (r[-31] + r[-3]) & 0xFFFFFFFF
+            r.append(v)
+            out[k] = (v >> 1)
+        self.r = r[-31:]
+        return out / float(RAND_MAX)
+
+
+def block_coords(case, start=(0, 0, 0), count=None):
+    """Cell centres of a block as Grid computes them (grid.cpp:24-57): x = (x0 + dx * 0.5) +
+    i * dx with x0 = dx * start_i and dx = L / n of the whole domain."""
+    count = count or (case["nx"], case["ny"], case["nz"])
+    out = []
+    for n, L, s, c in zip((case["nx"], case["ny"], case["nz"]), case["L"], start, count):
+        d = L / n
+        x0 = d * s
+        out.append((x0 + d * 0.5) + np.arange(c) * d)
+    return np.meshgrid(*out, indexing="ij")
+
+
+def shear_layer_W(case, name, start=(0, 0, 0), count=None, seed=1):
+    """The reference's "shear_xy" (testcases.cpp:250-296) or "shear_3d" (:298-344) primitive state
+    on the block at `start` with `count` cells (padded array of the block), the perturbations from
+    a rand() sequence started at `seed` (the reference: a fresh sequence per rank)."""
+    count = tuple(count or (case["nx"], case["ny"], case["nz"]))
+    X, Y, _ = block_coords(case, start, count)
+    Lx, Ly = case["L"][0], case["L"][1]
+    x = X - 0.5 * Lx
+    y = Y - 0.5 * Ly
+    pi = 3.14159265358979323846
+    # sin of the C library, once per distinct x
+    xs = np.unique(x)
+    sx = dict(zip(xs.tolist(), (math.sin(2.0 * pi * v / Lx) for v in xs.tolist())))
+    s = np.vectorize(sx.__getitem__, otypes=[np.float64])(x)
+    ncell = int(np.prod(count))
+    g = CRand(seed)
+    W = np.zeros((count[0] + 6, count[1] + 6, count[2] + 6, 5))
+    I = (slice(NGC, -NGC),) * 3
+    if name == "shear_xy":
+        ypert = (0.01 * g.take(ncell)).reshape(count)
+        inside = np.abs(y) <= 0.1 * Ly
+        W[I + (0,)] = 1.0
+        W[I + (1,)] = 2.5
+        W[I + (2,)] = np.where(inside, -0.5, 0.5)
+        W[I + (3,)] = np.where(inside, ypert, 0.0 * s)
+        W[I + (4,)] = 0.0
+    elif name == "shear_3d":
+        pert = (0.02 * g.take(2 * ncell)).reshape(count + (2,))  # z then y, per cell
+        zp, yp = pert[..., 0], pert[..., 1]
+        inside = np.abs(y) <= 0.25 * Ly
+        W[I + (0,)] = np.where(inside, 2.0, 1.0)
+        W[I + (1,)] = 2.5
+        W[I + (2,)] = np.where(inside, -0.5, 0.5)
+        W[I + (3,)] = np.where(inside, 0.01 * s + yp, 0.01 * s)
+        W[I + (4,)] = zp
+    else:
+        raise ValueError(name)
+    return W
+
+
+def reference_initial_condition(case, name, start=(0, 0, 0), count=None):
+    """The state Driver::set_initial_condition (driver.cpp:107-136) sets on one rank's block:
+    the Riemann configurations and the two shear layers.  Other names leave the reference's
+    state zero (no initial condition): ValueError."""
+    count = tuple(count or (case["nx"], case["ny"], case["nz"]))
+    if name in ("shear_xy", "shear_3d"):
+        return shear_layer_W(case, name, start, count)
+    if name in RIEMANN:
+        NW, NE, SW, SE, plane = RIEMANN[name]
+        X, Y, Z = block_coords(case, start, count)
+        Lx, Ly, Lz = case["L"]
+        a, La = (X, Lx) if plane == "xy" else (Z, Lz)
+        lo_a, lo_y = a <= La / 2.0, Y <= Ly / 2.0
+        W = np.zeros((count[0] + 6, count[1] + 6, count[2] + 6, 5))
+        I = (slice(NGC, -NGC),) * 3
+        for q in range(5):
+            W[I + (q,)] = np.where(lo_a & lo_y, NW[q], np.where(~lo_a & lo_y, NE[q],
+                                                             np.where(lo_a & ~lo_y, SW[q], SE[q])))
+        return W
+    raise ValueError(f"test case {name!r} sets no initial condition in the reference")

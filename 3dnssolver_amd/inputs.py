@@ -4,8 +4,9 @@ Reads the document Reader::read_input_file parses (io/reader.cpp:3-172; example
 tutorials/shearLayer/input.xml) with the same element names, whitespace collapsing of the texts
 (tinyxml2 COLLAPSE_WHITESPACE) and the checks of Params::sanity_check (params.cpp:3-120), and maps
 it onto the fields this package's Solver takes.  Choices the reference accepts but this path does
-not implement (the Centered convection scheme, the Shen diffusion scheme, Inflow boundaries, more
-than one rank) raise UnsupportedInput; invalid values raise InputFileError as the reference exits.
+not implement (the Centered convection scheme, the Shen diffusion scheme, Inflow boundaries) raise
+UnsupportedInput; invalid values raise InputFileError as the reference exits.  The rank counts
+(DomainDecomposition) come back in `extra` and drive a decomposed run on one GPU (driver.py).
 """
 import xml.etree.ElementTree as ET
 

@@ -10,6 +10,7 @@
 // and calls the scheme objects that Driver::create_convection_operator /
 // create_diffusion_operator (driver.cpp:74-105) would create.  MPI is the image's MPICH, run as a
 // single self-initialised rank.
+#include <cstdlib>
 #include <cstring>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -156,9 +157,10 @@ int ref_step(const nso_cfg* c, double* W_io, double* U_out, int nsteps, double* 
     return 0;
 }
 
-// The reference's initial conditions (testcases.cpp:5-180) on the block of `c`: id 0..2 Sod along
-// x, y, z; 3..8 the 2-D Riemann configurations 3, 4, 6, 12, 15, 17.  W_out gets the whole padded
-// primitive array (halos as a fresh Domain leaves them: zero).
+// The reference's initial conditions (testcases.cpp:5-344) on the block of `c`: id 0..2 Sod along
+// x, y, z; 3..8 the 2-D Riemann configurations 3, 4, 6, 12, 15, 17; 9 shear_layer_xy, 10
+// shear_layer_3d (their rand() draws from the sequence of a fresh process: srand(1) first).
+// W_out gets the whole padded primitive array (halos as a fresh Domain leaves them: zero).
 int ref_testcase(const nso_cfg* c, int id, double* W_out) {
     ensure_mpi();
     Params        p = make_params(c);
@@ -168,8 +170,10 @@ int ref_testcase(const nso_cfg* c, int id, double* W_out) {
     typedef void (*ic_fn)(Domain&, const Grid&, const Params&);
     static const ic_fn fns[] = {testcases::sod_xdir, testcases::sod_ydir, testcases::sod_zdir,
                                 testcases::case3,    testcases::case4,    testcases::case6,
-                                testcases::case12,   testcases::case15,   testcases::case17};
+                                testcases::case12,   testcases::case15,   testcases::case17,
+                                testcases::shear_layer_xy, testcases::shear_layer_3d};
     if (id < 0 || id >= (int)(sizeof(fns) / sizeof(fns[0]))) return -1;
+    srand(1);
     fns[id](domain, grid, p);
     Array4d& W = domain.get_NS_primitive_array();
     std::memcpy(W_out, &W(0, 0, 0, 0), len(W) * sizeof(double));

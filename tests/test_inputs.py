@@ -118,3 +118,12 @@ def test_binary_output_matches_the_reference_writer(tmp_path):
         assert (ref_dir / "results" / name).read_bytes() == (our_dir / "results" / name).read_bytes()
     np.testing.assert_array_equal(out.read_fields(str(our_dir / "results" / "fields_7_p_0.dat"), case),
                                   po.interior(W))
+
+
+@pytest.mark.skipif(not os.path.exists(TUTORIAL), reason="reference tree absent")
+def test_tutorial_document_is_the_reference_tutorial(tmp_path):
+    # the GPU driver test's document holds the tutorial's values
+    import importlib
+    gd = importlib.import_module("test_gpu_driver")
+    mine = inputs.read_input_xml(gd.write_xml(tmp_path))
+    assert mine == inputs.read_input_xml(TUTORIAL)

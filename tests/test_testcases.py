@@ -69,3 +69,46 @@ def test_sutherland_case_hits_reference_viscosity():
         assert law(r * T_ref) / law(T_ref) == pytest.approx(air, rel=1e-13)
     with pytest.raises(ValueError):
         C.config2(8, viscosity="power")
+
+
+def test_crand_is_the_c_library_sequence():
+    # known answer of srand(1): the first draws of the C library's rand(), and the library itself
+    import ctypes
+    C = cases_module()
+    g = C.CRand(1)
+    first = (g.take(5) * C.RAND_MAX).astype(np.int64).tolist()
+    assert first == [1804289383, 846930886, 1681692777, 1714636915, 1957747793]
+    libc = ctypes.CDLL("libc.so.6")
+    libc.srand(1)
+    ref = np.array([libc.rand() for _ in range(20000)], dtype=np.float64) / C.RAND_MAX
+    np.testing.assert_array_equal(C.CRand(1).take(20000), ref)
+
+
+@pytest.mark.skipif(not po.reference_available("strict"), reason="reference build absent")
+@pytest.mark.parametrize("name,ident,shape,L", [
+    ("shear_xy", 9, (20, 16, 1), (1.0, 1.0, 1.0)),
+    ("shear_xy", 9, (12, 10, 6), (2.0, 1.5, 0.5)),
+    ("shear_3d", 10, (12, 10, 6), (1.0, 1.0, 1.0)),
+])
+def test_shear_layers_match_reference(name, ident, shape, L):
+    # the reference's shear layers (testcases.cpp:250-344, rand() perturbations) bitwise
+    C = cases_module()
+    case = dict(C.base_case(*shape), L=L, dim=3 if shape[2] > 1 else 2)
+    W = C.reference_initial_condition(case, name)
+    Wr = po.Reference("strict").testcase(case, ident)
+    np.testing.assert_array_equal(W, Wr)
+    assert np.array_equal(np.signbit(W), np.signbit(Wr))  # the -0.0 of 0.0 * sin(x < 0) too
+
+
+def test_block_initial_conditions_tile_the_domain():
+    # a decomposed run's blocks: coordinates from the block's start (grid.cpp:24-57), so the
+    # Riemann states of the blocks tile the single block's exactly
+    C = cases_module()
+    case = dict(C.base_case(20, 16, 4), L=(1.0, 1.0, 0.2))
+    whole = C.reference_initial_condition(case, "case6")
+    for start, count in (((0, 0, 0), (10, 8, 4)), ((10, 8, 0), (10, 8, 4)), ((10, 0, 0), (10, 8, 4))):
+        blk = C.reference_initial_condition(case, "case6", start, count)
+        sl = tuple(slice(3 + s, 3 + s + c) for s, c in zip(start, count))
+        np.testing.assert_array_equal(blk[3:-3, 3:-3, 3:-3], whole[sl])
+    with pytest.raises(ValueError):
+        C.reference_initial_condition(case, "jet")
