@@ -164,12 +164,17 @@ struct Launchers {
     // boundary conditions + periodic exchange on nA planes of A and nB planes of B (B may be null)
     void (*halo)(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s);
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
-    // parts: 1 the tile kernel (all faces but the last plane), 2 the last face plane, 3 both
+    // parts: 1 the tile kernel (all faces but the last plane), 2 the last face plane, 3 both;
+    // [lo, hi): the faces of the x sweep (the whole block: [0, nx + 1)), the planes i of the y
+    // and z sweeps (the whole block: [0, nx); lo a multiple of 4)
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                      int parts, hipStream_t s);
-    // Dy null: dW/dx of u and v (Dx planes 2, 3) and their halo only (for residual_fd)
-    void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s);
-    void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s);
+                      int parts, int lo, int hi, hipStream_t s);
+    // planes i in [lo, hi) (lo a multiple of the march length, 16).  Dy null: dW/dx of u and v
+    // (Dx planes 2, 3) only (for residual_fd)
+    void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, int lo, int hi,
+                   hipStream_t s);
+    // Dy null: the j halos of dW/dx of u and v on planes [lo, hi); else all halos (lo, hi unused)
+    void (*deriv_halo)(double* Dx, double* Dy, Geo g, Bnd b, int lo, int hi, hipStream_t s);
     // R (interior) = R_in (or 0) + convection(fx,fy,fz) + diffusion(W, Dx, Dy); then either
     // stores R (st == nullptr) or applies the SSP-RK3 stage update.  With the full 3-D viscous
     // terms (physics bit 1) Dx is the 4-plane output of visc3d and Dy is unused.
@@ -205,11 +210,13 @@ struct Launchers {
     void (*p2c_faces)(const double* W, double* U, Geo g, Phys ph, hipStream_t s);
     // residual as above for the 2-D viscous terms (diff_order > 0, physics bit 1 clear) with
     // their y derivatives formed in the same kernel: Dx from deriv1 / deriv_halo with Dy null
+    // on planes [lo, hi) (lo a multiple of 16)
     void (*residual_fd)(const double* fx, const double* fy, const double* fz, const double* W,
                         const double* Dx, const double* Rin, double* Rout, const StageArgs* st,
                         const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
-                        hipStream_t s);
+                        int lo, int hi, hipStream_t s);
     int fuse_diff;  // 1: single blocks use residual_fd (ensembles take the same split)
+    int slab_align; // slabs of a pipelined stage start on multiples of this many planes i
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

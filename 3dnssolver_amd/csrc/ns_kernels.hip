@@ -588,34 +588,42 @@ constexpr int TILE_S1 = NS_TILE_S1;
 #ifndef NS_TILE_WAVES
 #define NS_TILE_WAVES 1      // minimum waves per SIMD requested for the tile kernels
 #endif
+#ifndef NS_TILE1_WAVES
+#define NS_TILE1_WAVES NS_TILE_WAVES  // the same for the single-block x/y tile kernel
+#endif
 
 // x or y sweep: 8 waves at 8 consecutive sweep positions; a wave covers kw consecutive k (unit
 // stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
 // values of the 13 cells the tile needs are computed once into LDS.  (bx, by, bz) = (k tile,
-// row tile, sweep tile).
+// row tile, sweep tile).  [lo, hi): for the x sweep the faces of the launch (face f between cells
+// f - 1 and f; hi <= nx), for the y sweep its planes i (a slab of the block); the full range is
+// [0, n) of the sweep resp. of the other index.
 template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
                                                   double* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
-                                                  const MlpPacked* __restrict__ mlp) {
+                                                  const MlpPacked* __restrict__ mlp, int lo,
+                                                  int hi) {
     constexpr int NC = TS + 5;
     constexpr int NV = split_nv<OPT>();
     const int lane = threadIdx.x, w = threadIdx.y;
     const int rows = TILE_L / kw;
     const int k    = NGC + bx * kw + (lane & (kw - 1));
-    const int o    = NGC + by * rows + lane / kw;  // j for the x sweep, i for the y sweep
-    const int s0   = bz * TS;                       // first face of the tile (0-based)
-    const int n    = (DIR == 0) ? g.nx : g.ny;
-    const int no   = (DIR == 0) ? g.ny : g.nx;
+    // j for the x sweep, i for the y sweep
+    const int o    = NGC + (DIR == 1 ? lo : 0) + by * rows + lane / kw;
+    const int s0   = (DIR == 0 ? lo : 0) + bz * TS;  // first face of the tile (0-based)
+    const int n    = (DIR == 0) ? hi : g.ny;          // end of the faces of this launch
+    const int no   = (DIR == 0) ? g.ny : hi;          // end of the other index
+    const int  nsw = (DIR == 0) ? g.nx : g.ny;      // cells along the sweep
     const long ss  = (DIR == 0) ? g.sx : g.sy;      // stride along the sweep
     const long so  = (DIR == 0) ? g.sy : g.sx;      // stride of the other index
     const bool kin = (k < NGC + g.nz) && (o < NGC + no);
     // cells s0 .. s0+12 (padded sweep index) feed faces s0+2 .. s0+9
     for (int t = w; t < NC; t += TS) {
         int cs = s0 + t;
-        if (kin && cs <= n + 2 * NGC - 1) {
+        if (kin && cs <= nsw + 2 * NGC - 1) {
             real sv[NV];
             stage_cell<DIR, OPT>(U, W, (long)cs * ss + (long)o * so + k, g.P, gamma, sv);
 #pragma unroll
@@ -630,8 +638,9 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     if constexpr (WT != 2) {
         if (!valid) return;
     }
-    const int  i = (valid ? fidx : 0) + NGC - 1;  // face i+1/2
-    const long c = (long)i * ss + (long)(valid ? o : NGC) * so + (valid ? k : NGC);
+    const int  i = (valid ? fidx : (DIR == 0 ? lo : 0)) + NGC - 1;  // face i+1/2
+    const long c = (long)i * ss + (long)(valid ? o : NGC + (DIR == 1 ? lo : 0)) * so +
+                   (valid ? k : NGC);
     real     Wl[5], Wr[5];
     if constexpr (CONV == 2) {
 #pragma unroll
@@ -646,12 +655,12 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
 }
 
 template <int DIR, int CONV, int WT, int OPT>
-__global__ __launch_bounds__(64 * TILE_S1, NS_TILE_WAVES) void k_flux_tile_xy(
+__global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy(
     const double* __restrict__ U, const double* __restrict__ W, double* __restrict__ f, Geo g,
-    real gamma, const MlpPacked* mlp) {
+    real gamma, const MlpPacked* mlp, int lo, int hi) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
     flux_tile_xy_body<DIR, CONV, WT, OPT, TILE_S1>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
-                                                   blockIdx.z, TILE_L, tile, mlp);
+                                                   blockIdx.z, TILE_L, tile, mlp, lo, hi);
 }
 
 // z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
@@ -712,13 +721,14 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
+// planes i = lo + blockIdx.z
 template <int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      double* __restrict__ f, Geo g,
-                                                     real gamma, const MlpPacked* mlp) {
+                                                     real gamma, const MlpPacked* mlp, int lo) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<OPT>()];
-    flux_tile_z_body<CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, blockIdx.z,
+    flux_tile_z_body<CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
                                     TILE_L, tile, mlp);
 }
 
@@ -761,12 +771,13 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
+// by0: first row tile (of 4 rows of the slow tangential index) of the launch
 template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
                                                     const double* __restrict__ W,
                                                     double* __restrict__ f, Geo g, real gamma,
-                                                    int i, const MlpPacked* mlp) {
-    flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, blockIdx.y, mlp);
+                                                    int i, const MlpPacked* mlp, int by0) {
+    flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, by0 + blockIdx.y, mlp);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1138,10 +1149,10 @@ constexpr int DERIV_ROWS = NS_DERIV_ROWS;
 template <int ORDER, bool DY>
 __global__ __launch_bounds__(64 * DERIV_ROWS, NS_CELL_WAVES) void k_deriv1(const double* __restrict__ W,
                                                 double* __restrict__ Dx, double* __restrict__ Dy,
-                                                Geo g) {
+                                                Geo g, int z0) {
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    deriv1_body<ORDER, DY>(W, Dx, Dy, g, bx, by, bz, 64, MARCH, 64 * DERIV_ROWS);
+    deriv1_body<ORDER, DY>(W, Dx, Dy, g, bx, by, z0 + bz, 64, MARCH, 64 * DERIV_ROWS);
 }
 
 // periodic exchange of the derivative halos that the cross derivatives read: the j halos of
@@ -1184,9 +1195,9 @@ __device__ __forceinline__ void deriv_halo_body(double* Dx, double* Dy, const Ge
     }
 }
 
-// Dy null: the j halos of dW/dx of u and v only
-__global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b) {
-    deriv_halo_body(Dx, Dy, g, b, blockIdx.x, blockIdx.y, blockIdx.z, Dy ? 0 : 2);
+// Dy null: the j halos of dW/dx of u and v only; by0: the first row tile (4 planes i) of the launch
+__global__ void k_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, int by0) {
+    deriv_halo_body(Dx, Dy, g, b, blockIdx.x, by0 + blockIdx.y, blockIdx.z, Dy ? 0 : 2);
 }
 
 // CenteredDiffusion::compute_residual (centered_diffusion.cpp:70-182) at one cell, with the
@@ -1710,11 +1721,11 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
-    int which, Geo g, Phys ph, Bnd bnd) {
+    int which, Geo g, Phys ph, Bnd bnd, int z0) {
     int bx, by, bz;
     cell_tile(bx, by, bz);
     residual_fd_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin, which, g, ph, bnd,
-                                      bx, by, bz, KW_CELL, FUSED_MARCH);
+                                      bx, by, z0 + bz, KW_CELL, FUSED_MARCH);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2115,7 +2126,8 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
+    // the whole block: faces 0 .. nx-1 of the x sweep, planes 0 .. nx-1 of the y sweep
+    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx);
 }
 
 template <int CONV, int WT, int OPT>
@@ -2283,30 +2295,45 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
     k_c2p_faces<<<cell_grid(g.NZ, g.NY, g.NX, BLK), BLK, 0, s>>>(U, W, g, ph.gamma);
 }
 
-// parts: 1 the tile kernel (faces 0 .. n-1 of the sweep), 2 the last face plane, 3 both
+// parts: 1 the tile kernel (faces 0 .. n-1 of the sweep), 2 the last face plane, 3 both.
+// [lo, hi): for the x sweep the faces f of the launch (f between cells f-1 and f, 0 <= f <= nx:
+// the tile kernel takes f < nx, the last plane f = nx), for the y and z sweeps the planes i (both
+// kernels); the whole block is [0, nx + 1) resp. [0, nx).  lo is a multiple of 4 for y and z.
 template <int DIR, int CONV, int B, int OPT>
 static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
-                     const MlpPacked* mlp, int parts, hipStream_t s) {
+                     const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s) {
     if (!(parts & 1)) {
     } else if constexpr (DIR == 2) {
         dim3 blk(TILE_L, 8, 1);
-        dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, g.nx);
-        k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, hi - lo);
+        if (hi > lo) k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo);
+    } else if constexpr (DIR == 0) {
+        const int fe = hi < g.nx ? hi : g.nx;  // faces of the tile kernel: [lo, fe)
+        dim3      blk(TILE_L, TILE_S1, 1);
+        dim3      grd((g.nz + TILE_L - 1) / TILE_L, g.ny, (fe - lo + TILE_S1 - 1) / TILE_S1);
+        if (fe > lo)
+            k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, fe);
     } else {
-        int  n = (DIR == 0) ? g.nx : g.ny;
-        int  o = (DIR == 0) ? g.ny : g.nx;
         dim3 blk(TILE_L, TILE_S1, 1);
-        dim3 grd((g.nz + TILE_L - 1) / TILE_L, o, (n + TILE_S1 - 1) / TILE_S1);
-        k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp);
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, hi - lo, (g.ny + TILE_S1 - 1) / TILE_S1);
+        if (hi > lo)
+            k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, hi);
     }
     if (!(parts & 2)) return;
     // the last face plane (sweep index NGC + n - 1)
     int  n  = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
-    int  n1 = (DIR == 0) ? g.ny : g.nx;
     int  n2 = (DIR == 2) ? g.ny : g.nz;
     dim3 pb(64, 4, 1);
-    dim3 pg((n2 + 63) / 64, (n1 + 3) / 4, 1);
-    k_flux_plane<DIR, CONV, B, OPT><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp);
+    if constexpr (DIR == 0) {
+        if (hi <= g.nx) return;  // face nx is not in this launch
+        dim3 pg((n2 + 63) / 64, (g.ny + 3) / 4, 1);
+        k_flux_plane<DIR, CONV, B, OPT><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp, 0);
+    } else {
+        if (hi <= lo) return;
+        dim3 pg((n2 + 63) / 64, (hi - lo + 3) / 4, 1);
+        k_flux_plane<DIR, CONV, B, OPT><<<pg, pb, 0, s>>>(U, W, f, g, gamma, NGC + n - 1, mlp,
+                                                          lo / 4);
+    }
 }
 
 // kernel options of a configuration: bit 0 per-face alpha, bit 1 corrected characteristic
@@ -2318,63 +2345,74 @@ static int flux_opt(const Phys& ph) {
 
 template <int DIR, int CONV, int B>
 static void flux_dir_opt(const double* U, const double* W, double* f, Geo g, double gamma,
-                         const MlpPacked* m, int opt, int parts, hipStream_t s) {
+                         const MlpPacked* m, int opt, int parts, int lo, int hi, hipStream_t s) {
     if constexpr (B == 2) {
-        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, s);
+        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
     } else if constexpr (CONV == 1) {
-        if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, parts, s);
-        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, s);
+        if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
     } else {
-        if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, parts, s);
-        else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, parts, s);
-        else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, parts, s);
-        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, parts, s);
+        if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
     }
 }
 
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
-                          int parts, hipStream_t s) {
+                          int parts, int lo, int hi, hipStream_t s) {
     const MlpPacked* m   = ph.mlp;
     const int        opt = flux_opt(ph);
     if (ph.conv == 1) {
-        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, s);
-        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
     } else {
-        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, parts, s);
-        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, parts, s);
+        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
     }
 }
 
 static void l_face_flux(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                        int parts, hipStream_t s) {
-    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, parts, s);
-    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, parts, s);
-    else face_flux_dir<2>(U, W, f, g, ph, parts, s);
+                        int parts, int lo, int hi, hipStream_t s) {
+    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, parts, lo, hi, s);
+    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, parts, lo, hi, s);
+    else face_flux_dir<2>(U, W, f, g, ph, parts, lo, hi, s);
 }
 
-static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, hipStream_t s) {
+// planes [lo, hi) (lo a multiple of MARCH)
+static void l_deriv1(const double* W, double* Dx, double* Dy, Geo g, Phys ph, int lo, int hi,
+                     hipStream_t s) {
+    if (hi <= lo) return;
     const dim3 blk(64, DERIV_ROWS, 1);
     const dim3 grd((g.nz + 63) / 64, (g.ny + DERIV_ROWS - 1) / DERIV_ROWS,
-                   (g.nx + MARCH - 1) / MARCH);
+                   (hi - lo + MARCH - 1) / MARCH);
+    const int  z0 = lo / MARCH;
     if (Dy) {
-        if (ph.diff_order == 2) k_deriv1<2, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-        else if (ph.diff_order == 4) k_deriv1<4, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-        else k_deriv1<6, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        if (ph.diff_order == 2) k_deriv1<2, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
+        else if (ph.diff_order == 4) k_deriv1<4, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
+        else k_deriv1<6, true><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
     } else {
-        if (ph.diff_order == 2) k_deriv1<2, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-        else if (ph.diff_order == 4) k_deriv1<4, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
-        else k_deriv1<6, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g);
+        if (ph.diff_order == 2) k_deriv1<2, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
+        else if (ph.diff_order == 4) k_deriv1<4, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
+        else k_deriv1<6, false><<<grd, blk, 0, s>>>(W, Dx, Dy, g, z0);
     }
 }
 
-static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, hipStream_t s) {
+// without Dy: the j halos of dW/dx of u and v on planes [lo, hi) (lo a multiple of 4); with Dy
+// both halo families of the whole block (lo, hi ignored)
+static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, int lo, int hi, hipStream_t s) {
     int  m1 = g.nx > g.ny ? g.nx : g.ny;
     dim3 blk(64, 4, 1);
-    dim3 grd((g.nz + 63) / 64, (m1 + 3) / 4, (Dy ? 4 : 2) * NGC);  // without Dy: its halo only
-    k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b);
+    if (Dy) {
+        dim3 grd((g.nz + 63) / 64, (m1 + 3) / 4, 4 * NGC);
+        k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b, 0);
+    } else if (hi > lo) {
+        dim3 grd((g.nz + 63) / 64, (hi - lo + 3) / 4, 2 * NGC);
+        k_deriv_halo<<<grd, blk, 0, s>>>(Dx, Dy, g, b, lo / 4);
+    }
 }
 
 template <int ORDER>
@@ -2399,38 +2437,41 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
     }
 }
 
+// planes [lo, hi) (lo a multiple of FUSED_MARCH)
 template <int ORDER>
 static void residual_fd_order(const double* fx, const double* fy, const double* fz,
                               const double* W, const double* Dx, const double* Rin, double* Rout,
                               StageArgs a,
                               const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
-                              hipStream_t s) {
+                              int lo, int hi, hipStream_t s) {
+    if (hi <= lo) return;
     const dim3 blk(64, 4, 1);
     const dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
-                   (g.nx + FUSED_MARCH - 1) / FUSED_MARCH);
+                   (hi - lo + FUSED_MARCH - 1) / FUSED_MARCH);
+    const int  z0 = lo / FUSED_MARCH;
     if (ph.suth) {
         if (g.square_xy)
-            k_residual_fd<ORDER, true, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+            k_residual_fd<ORDER, true, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
         else
-            k_residual_fd<ORDER, false, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+            k_residual_fd<ORDER, false, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
     } else {
         if (g.square_xy)
-            k_residual_fd<ORDER, true, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+            k_residual_fd<ORDER, true, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
         else
-            k_residual_fd<ORDER, false, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b);
+            k_residual_fd<ORDER, false, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
     }
 }
 
 static void l_residual_fd(const double* fx, const double* fy, const double* fz, const double* W,
                           const double* Dx, const double* Rin, double* Rout, const StageArgs* st,
                           const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
-                          hipStream_t s) {
+                          int lo, int hi, hipStream_t s) {
     StageArgs a{};
     if (st) a = *st;
     switch (ph.diff_order) {
-    case 2: residual_fd_order<2>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
-    case 4: residual_fd_order<4>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
-    default: residual_fd_order<6>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, s); break;
+    case 2: residual_fd_order<2>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
+    case 4: residual_fd_order<4>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
+    default: residual_fd_order<6>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
     }
 }
 
@@ -2575,7 +2616,9 @@ const Launchers& launchers() {
                                 l_aos_to_soa,   l_soa_to_aos, l_reduce,     l_reduce_final,
                                 l_time_accum,   l_zero_flags, l_selftest_div, l_ens_step,
                                 l_mlp_recon,    l_set_dt,     l_visc3d,     l_p2c_faces,
-                                l_residual_fd,  NS_FUSE_DIFF};
+                                l_residual_fd,  NS_FUSE_DIFF,
+                                // plane ranges of the cell kernels start on a march boundary
+                                (MARCH > FUSED_MARCH ? MARCH : FUSED_MARCH)};
     return L;
 }
 

@@ -509,3 +509,25 @@ def test_set_state_derives_the_other_form(ns, oracle):
     Wo, _, _ = oracle.step(case, Wc, 2)
     np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
     s.close()
+
+
+@pytest.mark.parametrize("slabs,bc", [(5, "periodic"), (3, "walls"), (2, "mixed")])
+def test_pipelined_stage_slabs_bitwise(ns, oracle, slabs, bc, monkeypatch):
+    # the stage pipelined in slabs of planes i on two streams (the residual of slab s beside the
+    # sweeps of slab s + 1; an option, $NSGPU_STAGE_SLABS) is bitwise the oracle: the same kernels
+    # on plane ranges, x faces computed once each
+    C = cases_module()
+    bcs = {"periodic": ["periodic"] * 6, "walls": ["neumann"] * 6,
+           "mixed": ["neumann", "neumann", "periodic", "periodic", "neumann", "neumann"]}[bc]
+    case = dict(C.base_case(80, 20, 24), conv="weno5cw", bc=bcs, mu=2e-3, L=(2.0, 1.0, 1.0))
+    C.sutherland(case, case["mu"], 2.2 ** 2)
+    W0 = C.random_smooth_W(case, seed=9, c0=2.2)
+    monkeypatch.setenv("NSGPU_STAGE_SLABS", str(slabs))
+    print(f"[gpu] pipelined stage, {slabs} slabs, {bc}", flush=True)
+    s = ns.Solver(case, math="exact")
+    s.set_state(W0, "primitive")
+    t, dt = s.step(3)
+    Wo, _, dto = oracle.step(case, W0, 3)
+    np.testing.assert_array_equal(po.interior(s.get_state("primitive")), po.interior(Wo))
+    assert dt == dto[-1]
+    s.close()
