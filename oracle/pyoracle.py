@@ -288,6 +288,44 @@ class Reference(_Lib):
         return W
 
 
+class Adapters:
+    """The reference's time loop driving the GPU adapters (oracle/_ref/libnsref_adapters.so,
+    adapter_harness.cpp): with GpuConvection / GpuDiffusion as its schemes, or GpuTimeScheme."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libnsref_adapters.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle adapters`")
+        self.lib = ctypes.CDLL(path)
+        self.lib.adapt_make_config.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+
+    def _step(self, fn, case, W, nsteps):
+        cfg = make_cfg(case)
+        W = np.ascontiguousarray(W, dtype=np.float64).copy()
+        dts = np.zeros(max(nsteps, 1))
+        rc = fn(ctypes.byref(cfg), _ptr(W), ctypes.c_int(nsteps), _ptr(dts))
+        if rc != 0:
+            raise RuntimeError(f"adapter run failed ({rc})")
+        return W, dts[:nsteps]
+
+    def step_schemes(self, case, W, nsteps):
+        """(W, dts) after nsteps of TimeScheme::step with GpuConvection / GpuDiffusion."""
+        return self._step(self.lib.adapt_step_schemes, case, W, nsteps)
+
+    def step_timescheme(self, case, W, nsteps):
+        """(W, dts) after nsteps of GpuTimeScheme::step."""
+        return self._step(self.lib.adapt_step_timescheme, case, W, nsteps)
+
+    def make_config(self, case, scheme):
+        """0: make_config accepts convection_scheme `scheme`; 1: it throws invalid_argument."""
+        cfg = make_cfg(case)
+        return self.lib.adapt_make_config(ctypes.byref(cfg), scheme.encode())
+
+
+def adapters_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libnsref_adapters.so"))
+
+
 def reference_available(flavour="strict"):
     return os.path.exists(os.path.join(HERE, "_ref", f"libnsref_{flavour}.so"))
 

@@ -34,8 +34,9 @@
 #include "time_scheme.hpp"
 
 #include "nsoracle.h"
+#include "ref_harness.hpp"
 
-namespace {
+namespace ref_harness {
 
 void ensure_mpi() {
     int flag = 0;
@@ -98,7 +99,9 @@ bool reference_semantics(const nso_cfg* c) {
            c->weights != NSO_W_LEARNED;
 }
 
-} // namespace
+} // namespace ref_harness
+
+using namespace ref_harness;
 
 extern "C" {
 

@@ -98,3 +98,19 @@ def test_make_config_refuses_centered_convection(tmp_path):
                os.path.join(os.path.dirname(MPI_INC), "lib"))
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60, env=env)
     assert r.returncode == 0 and "fails=0" in r.stdout, (r.stdout, r.stderr)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libnsref_adapters.so")),
+                    reason="adapter harness not built (needs the reference tree)")
+def test_adapter_library_make_config():
+    # the adapter harness (the reference's TUs + gpu_schemes.hpp, linked to libnsgpu.so) loads
+    # without a GPU, and its make_config refuses the Centered convection scheme
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    C = __import__("importlib").import_module("3dnssolver_amd.cases")
+    a = pyoracle.Adapters()
+    case = C.base_case(8, 8, 8)
+    assert a.make_config(case, "Weno") == 0
+    assert a.make_config(case, "none") == 0
+    assert a.make_config(case, "Centered") == 1
