@@ -190,16 +190,36 @@ __device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, rea
 // Every lane of the wavefront must call this (the network uses matrix instructions); `active`
 // marks the lanes whose face is real.
 // ------------------------------------------------------------------------------------------
+// q: this wavefront's queue of network samples in LDS (LearnQueue), or null: the face kernels
+// batch the network samples of all reconstructions of a face (face_flux_learned_cw)
 struct Learn {
     const MlpPacked* P;
     bool             active;
+    real*            q = nullptr;
 };
+
+// the classic Shu reconstruction of the stencil v in left-biased order (side 1: mirrored)
+__device__ __forceinline__ real learned_classic(const real v[5], int side) {
+    return side ? weno_right_c<false>(v[4], v[3], v[2], v[1], v[0])
+                : weno_left_c<false>(v[0], v[1], v[2], v[3], v[4]);
+}
+
+// the blend of the network's reconstruction (weights w of the three ENO candidates of v) with
+// the classic one
+__device__ __forceinline__ real learned_finish(const real v[5], const float w[3], real classic,
+                                               float ratio) {
+    const real f0 = (RL(2.) * v[0] - RL(7.) * v[1] + RL(11.) * v[2]) * ONEPERSIX;
+    const real f1 = (-v[1] + RL(5.) * v[2] + RL(2.) * v[3]) * ONEPERSIX;
+    const real f2 = (RL(2.) * v[2] + RL(5.) * v[3] - v[4]) * ONEPERSIX;
+    const real net = (real)w[0] * f0 + (real)w[1] * f1 + (real)w[2] * f2;
+    return classic + (real)mlp_blend(ratio) * (net - classic);
+}
 
 template <int SIDE>
 __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, real e,
                                              const Learn& L, int* regime = nullptr) {
-    const real classic = SIDE ? weno_right_c<false>(a, b, c, d, e) : weno_left_c<false>(a, b, c, d, e);
     const real v[5]    = {SIDE ? e : a, SIDE ? d : b, c, SIDE ? b : d, SIDE ? a : e};
+    const real classic = learned_classic(v, SIDE);
     float      x[8], ratio;
     const int  rg   = mlp_features(v, x, &ratio);
     const bool need = L.active && rg == 1;
@@ -210,11 +230,97 @@ __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, rea
     for (int k = 0; k < 8; k++) xs[k] = need ? x[k] : 0.0f;
     mlp_weights_lane(L.P, xs, w);
     if (!need) return classic;
-    const real f0 = (RL(2.) * v[0] - RL(7.) * v[1] + RL(11.) * v[2]) * ONEPERSIX;
-    const real f1 = (-v[1] + RL(5.) * v[2] + RL(2.) * v[3]) * ONEPERSIX;
-    const real f2 = (RL(2.) * v[2] + RL(5.) * v[3] - v[4]) * ONEPERSIX;
-    const real net = (real)w[0] * f0 + (real)w[1] * f1 + (real)w[2] * f2;
-    return classic + (real)mlp_blend(ratio) * (net - classic);
+    return learned_finish(v, w, classic, ratio);
+}
+
+// ---- batched network samples of one face (component-wise sweeps) ---------------------------
+// A wavefront's queue holds up to 64 stencils that need the network: their five values
+// (left-biased order) and side, as planes [6][64] of `real` in LDS.  The reconstructions of a
+// face enqueue their network stencils; when the next one would overflow the queue, and after the
+// last, the wavefront evaluates the queued samples in one 64-sample network call (lane l takes
+// sample l), writes each result over the sample's first value, and every lane takes back the
+// results of its own samples.  A sample's arithmetic (features, classic reconstruction, matrix
+// columns, blend) is that of weno_learned, so the results are bitwise the same; only the number
+// of network calls changes (one per 64 samples instead of one per reconstruction site).
+constexpr int LQ_REALS = 6 * 64;  // per wavefront
+
+// LDS writes of one lane become visible to the other lanes of its wavefront
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// evaluates the n <= 64 queued samples; sample s's result replaces its value 0 (q[s])
+__device__ __forceinline__ void learn_flush(const MlpPacked* P, real* q, int n) {
+    const int  l   = threadIdx.x & 63;
+    const bool has = l < n;
+    real       v[5];
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < 5; k++) v[k] = has ? q[64 * k + l] : RL(0.0);
+    const int side = has ? (int)q[64 * 5 + l] : 0;
+    float     x[8], xs[8], ratio, w[3];
+    mlp_features(v, x, &ratio);
+#pragma unroll
+    for (int k = 0; k < 8; k++) xs[k] = has ? x[k] : 0.0f;
+    mlp_weights_lane(P, xs, w);
+    if (has) q[l] = learned_finish(v, w, learned_classic(v, side), ratio);
+    wave_sync();
+}
+
+// component-wise face flux with the learned weights: out[q] = left_q + right_q as in face_flux,
+// each reconstruction as weno_learned computes it, the network samples batched per face
+template <class S>
+__device__ __forceinline__ void face_flux_learned_cw(const S& st, real out[5], const Learn& L) {
+    real val[10];
+    int  slot[10];
+    int  base = 0;  // queued samples (wavefront-uniform)
+    const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const int qv = r >> 1, side = r & 1;
+        real      v[5];
+        if (side == 0) {
+#pragma unroll
+            for (int k = 0; k < 5; k++) v[k] = st(k, qv);  // WENO5_left(fL of cells i-2 .. i+2)
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; k++) v[k] = st(5 - k, 5 + qv);  // WENO5_right mirrored
+        }
+        const real classic = learned_classic(v, side);
+        float      x[8], ratio;
+        const bool need = L.active && mlp_features(v, x, &ratio) == 1;
+        const unsigned long long m = __ballot(need);
+        const int cnt = __popcll(m);
+        if (base + cnt > 64) {  // wavefront-uniform
+            learn_flush(L.P, L.q, base);
+#pragma unroll
+            for (int t = 0; t < r; t++)
+                if (slot[t] >= 0) val[t] = L.q[slot[t]], slot[t] = -1;
+            wave_sync();  // results read before the queue is refilled
+            base = 0;
+        }
+        val[r]  = classic;
+        slot[r] = -1;
+        if (need) {
+            const int s = base + __popcll(m & below);
+#pragma unroll
+            for (int k = 0; k < 5; k++) L.q[64 * k + s] = v[k];
+            L.q[64 * 5 + s] = (real)side;
+            slot[r]         = s;
+        }
+        base += cnt;
+    }
+    if (base > 0) {
+        learn_flush(L.P, L.q, base);
+#pragma unroll
+        for (int t = 0; t < 10; t++)
+            if (slot[t] >= 0) val[t] = L.q[slot[t]];
+        wave_sync();
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) out[q] = val[2 * q] + val[2 * q + 1];
 }
 
 // weights WT: 0 Shu, 1 Borges, 2 learned
@@ -515,6 +621,12 @@ __device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const 
     const S& st = raw;
     constexpr bool CORR = (OPT & 2) != 0;
     if constexpr (CONV == 1) {
+        if constexpr (WT == 2) {
+            if (L.q) {
+                face_flux_learned_cw(st, out, L);
+                return;
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 5; q++)
             out[q] = weno_left<WT>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q), L) +
@@ -605,7 +717,7 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
                                                   const MlpPacked* __restrict__ mlp, int lo,
-                                                  int hi) {
+                                                  int hi, real* lq = nullptr) {
     constexpr int NC = TS + 5;
     constexpr int NV = split_nv<OPT>();
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -648,7 +760,8 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     }
     LdsSplit<TILE_L, NV> st{&tile[(w * NV) * TILE_L + lane]};
     real               out[5];
-    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
+                                  Learn{mlp, valid, lq ? lq + w * LQ_REALS : nullptr});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -659,8 +772,10 @@ __global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy(
     const double* __restrict__ U, const double* __restrict__ W, double* __restrict__ f, Geo g,
     real gamma, const MlpPacked* mlp, int lo, int hi) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
+    __shared__ real lq[WT == 2 ? TILE_S1 * LQ_REALS : 1];  // network queues (learned weights)
     flux_tile_xy_body<DIR, CONV, WT, OPT, TILE_S1>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
-                                                   blockIdx.z, TILE_L, tile, mlp, lo, hi);
+                                                   blockIdx.z, TILE_L, tile, mlp, lo, hi,
+                                                   WT == 2 ? lq : nullptr);
 }
 
 // z sweep: lanes along k = the sweep.  A wave covers 64/kw rows (j) with a segment of kw faces
@@ -674,7 +789,8 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
                                                  double* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
                                                  real* __restrict__ lds,
-                                                 const MlpPacked* __restrict__ mlp) {
+                                                 const MlpPacked* __restrict__ mlp,
+                                                 real* lq = nullptr) {
     constexpr int NV = split_nv<OPT>();
     const int NC   = kw + 5;
     const int sub  = TILE_L / kw;                   // rows per wave
@@ -715,7 +831,8 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
         __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
     } st{&T[kk], NC};
     real out[5];
-    face_flux<2, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    face_flux<2, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
+                                Learn{mlp, valid, lq ? lq + r * LQ_REALS : nullptr});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -728,8 +845,9 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
                                                      double* __restrict__ f, Geo g,
                                                      real gamma, const MlpPacked* mlp, int lo) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<OPT>()];
+    __shared__ real lq[WT == 2 ? 8 * LQ_REALS : 1];
     flux_tile_z_body<CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
-                                    TILE_L, tile, mlp);
+                                    TILE_L, tile, mlp, WT == 2 ? lq : nullptr);
 }
 
 // one plane of faces (sweep index `i`), one thread per face, split values in registers;
@@ -739,7 +857,8 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
                                                 const double* __restrict__ W,
                                                 double* __restrict__ f, const Geo& g, real gamma,
                                                 int i, int bx, int by,
-                                                const MlpPacked* __restrict__ mlp) {
+                                                const MlpPacked* __restrict__ mlp,
+                                                real* lq = nullptr) {
     // tangential indices (t1 slow, t2 fast) over the interior
     int        t2    = threadIdx.x + bx * 64;
     int        t1    = threadIdx.y + by * 4;
@@ -765,7 +884,8 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
     }
     RegSplit<NV> st{sv};
     real         out[5];
-    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out, Learn{mlp, valid});
+    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
+                                  Learn{mlp, valid, lq ? lq + threadIdx.y * LQ_REALS : nullptr});
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -777,7 +897,9 @@ __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U
                                                     const double* __restrict__ W,
                                                     double* __restrict__ f, Geo g, real gamma,
                                                     int i, const MlpPacked* mlp, int by0) {
-    flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, by0 + blockIdx.y, mlp);
+    __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
+    flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, by0 + blockIdx.y, mlp,
+                                        WT == 2 ? lq : nullptr);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2118,6 +2240,7 @@ __global__ void kb_halo(EnsTable T, int s) {
 template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T, int s) {
     __shared__ real tile[(TILE_S + 5) * split_nv<OPT>() * TILE_L];
+    __shared__ real lq[WT == 2 ? TILE_S * LQ_REALS : 1];
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, DIR == 0 ? EF_FLUX_X : EF_FLUX_Y, bx, by, bz);
     const Geo       g   = E->g;
@@ -2127,12 +2250,14 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     double*       f = E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
     // the whole block: faces 0 .. nx-1 of the x sweep, planes 0 .. nx-1 of the y sweep
-    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx);
+    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx,
+                                          WT == 2 ? lq : nullptr);
 }
 
 template <int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T, int s) {
     __shared__ real tile[8 * zlds_wave_max<OPT>()];
+    __shared__ real lq[WT == 2 ? 8 * LQ_REALS : 1];
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_FLUX_Z, bx, by, bz);
     const Geo       g   = E->g;
@@ -2141,7 +2266,8 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T,
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[2];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp);
+    flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp,
+                                    WT == 2 ? lq : nullptr);
 }
 
 template <int DIR, int CONV, int WT, int OPT>
@@ -2154,7 +2280,9 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     double*       f = E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp);
+    __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
+    flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp,
+                                        WT == 2 ? lq : nullptr);
 }
 
 // ensemble members take the same derivative split as single blocks (k_residual_fd): dW/dx of
