@@ -217,6 +217,15 @@ struct Launchers {
                         int lo, int hi, hipStream_t s);
     int fuse_diff;  // 1: single blocks use residual_fd (ensembles take the same split)
     int slab_align; // slabs of a pipelined stage start on multiples of this many planes i
+    // the stage's residual with the z face fluxes formed in the kernel (fz never materialised
+    // but for the first face of each row segment): the segment width it runs with, 0 where it
+    // does not cover the configuration (then the z sweep + residual_fd run instead)
+    int (*zfd_kw)(Geo g, Phys ph);
+    // residual_fd + stage update of planes [lo, hi) for the stage st (its Ucur and the
+    // primitives W are the input state), writing only the first face of each segment into fz
+    void (*residual_zfd)(const double* fx, const double* fy, double* fz, const double* W,
+                         const double* Dx, const StageArgs* st, const unsigned long long* dtmin,
+                         Geo g, Phys ph, Bnd b, int lo, int hi, hipStream_t s);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

@@ -1660,21 +1660,37 @@ constexpr int FUSED_MARCH = NS_FUSED_MARCH;
 #define NS_FUSE_DIFF 1  // single blocks and ensembles take the derivative split (2-D viscous terms)
 #endif
 
+// z faces formed in the residual kernel (ZC = convection type of the sweep, Shu weights, no
+// kernel options): the LDS of a block, sized for every row segment kw in {64, 128, 256}
+// (256 / kw rows): the split values of the kw + 5 cells of each row, and the kw + 1 face fluxes
+template <int ZC> constexpr int zfd_split_reals() { return ZC ? 4 * (64 + 5) * split_nv<0>() : 1; }
+template <int ZC> constexpr int zfd_face_reals() { return ZC ? 4 * 5 * (64 + 1) : 1; }
+
 // 256 threads: kw lanes along k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of
-// `march` planes i)
-template <int ORDER, bool SQ, bool SUTH>
+// `march` planes i).  ZC > 0: the z face fluxes of the plane are formed here (convection type
+// ZC, Shu weights) instead of read from fz: the lanes of a row stage the split values of their
+// cells, and of the two cells before and three after the row segment, in LDS (zt); lane k forms
+// face k + 1/2 into zf, and takes face k - 1/2 from lane k - 1, the first lane of the segment
+// from fz, where the plane launch of that face stored it.  The same face_flux on the same split
+// values as the z sweep: the residual is bitwise the one the sweep's faces give.  Every thread
+// of the block reaches the two barriers of each plane; threads without a cell skip the work.
+template <int ORDER, bool SQ, bool SUTH, int ZC = 0>
 __device__ __forceinline__ void residual_fd_body(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, const StageArgs& st, const unsigned long long* __restrict__ dtmin,
     int which, const Geo& g, const Phys& ph, const Bnd& bnd, int bx, int by, int bz, int kw,
-    int march) {
+    int march, real* __restrict__ zt = nullptr, real* __restrict__ zf = nullptr) {
     constexpr int H   = ORDER / 2;
     const int     tid = threadIdx.x + threadIdx.y * 64;
-    const int     k   = (tid & (kw - 1)) + bx * kw;
+    const int     kr  = tid & (kw - 1);  // lane within the row segment
+    const int     k   = kr + bx * kw;
     const int     j   = tid / kw + by * (256 / kw);
     const int     i0  = bz * march;
-    if (k >= g.nz || j >= g.ny) return;
+    const bool    live = k < g.nz && j < g.ny;
+    if constexpr (ZC == 0) {
+        if (!live) return;
+    }
     const int       i1   = (i0 + march < g.nx) ? i0 + march : g.nx;
     const long      P    = g.P;
     const unsigned  sx   = (unsigned)g.sx, sy = (unsigned)g.sy;
@@ -1688,13 +1704,6 @@ __device__ __forceinline__ void residual_fd_body(
     double wd[4][2 * H + 1];   // primitives of row j at x offsets -H .. H
     real   dyx[2][2 * H + 1];  // dW/dy of u, v at x offsets -H .. H
     double fxm[5];
-#pragma unroll
-    for (int t = 0; t < 2 * H; t++) {
-        const int      e = i0 - H + t;
-        const unsigned a = row + (unsigned)(e + NGC) * sx;
-#pragma unroll
-        for (int q = 0; q < 4; q++) wd[q][t] = (W + q * P)[a];
-    }
     // dW/dy of u (a = 0), v (a = 1) on plane e: zero beyond an x wall, the wrapped plane's
     // across a periodic x
     auto dy_uv = [&](int e, int a) -> real {
@@ -1705,6 +1714,14 @@ __device__ __forceinline__ void residual_fd_body(
         return (real)fd_sum_t<ORDER, false, double>(
             [&](int t) { return f[b + (t - H) * (int)sy]; }, g.dy, g.idy);
     };
+    if (live) {
+#pragma unroll
+    for (int t = 0; t < 2 * H; t++) {
+        const int      e = i0 - H + t;
+        const unsigned a = row + (unsigned)(e + NGC) * sx;
+#pragma unroll
+        for (int q = 0; q < 4; q++) wd[q][t] = (W + q * P)[a];
+    }
 #pragma unroll
     for (int t = 0; t < 2 * H; t++) {
         dyx[0][t] = dy_uv(i0 - H + t, 0);
@@ -1714,6 +1731,14 @@ __device__ __forceinline__ void residual_fd_body(
 #pragma unroll
         for (int q = 0; q < 5; q++) fxm[q] = (fx + q * P)[row + (unsigned)(i0 + NGC - 1) * sx];
     }
+    }  // live
+    // z faces (ZC): the row's split tile and face row in LDS, the cells valid in the segment
+    constexpr int NVZ  = split_nv<0>();
+    const int     ncz  = kw + 5;
+    real*         T    = zt + (tid / kw) * (NVZ * ncz);
+    real*         Fz   = zf + (tid / kw) * (5 * (kw + 1));
+    const bool    rowz = j < g.ny;
+    const int     mz   = (g.nz - bx * kw < kw) ? g.nz - bx * kw : kw;
     double dt = 0, m1 = 0, m2 = 0, m3 = 0;
     if (st.Uout) {
         dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
@@ -1727,6 +1752,49 @@ __device__ __forceinline__ void residual_fd_body(
     }
     for (int i = i0; i < i1; i++) {
         const unsigned cc = row + (unsigned)(i + NGC) * sx;
+        real           zp[5], zm[5];  // z faces k + 1/2 and k - 1/2 (ZC)
+        if constexpr (ZC > 0) {
+            const unsigned rb  = (unsigned)(j + NGC) * sy + (unsigned)(i + NGC) * sx;
+            const real     gam = (real)ph.gamma;
+            if (live) {
+                real sv[NVZ];
+                stage_cell<2, 0>(st.Ucur, W, (long)cc, P, gam, sv);
+#pragma unroll
+                for (int v = 0; v < NVZ; v++) T[v * ncz + kr + 2] = sv[v];
+            }
+            if (rowz && kr < 5) {  // cells 2 and 1 before the segment, 1 .. 3 after its end
+                const int t = kr < 2 ? kr : mz + kr;
+                real      sv[NVZ];
+                stage_cell<2, 0>(st.Ucur, W, (long)(rb + (unsigned)(bx * kw + NGC - 2 + t)), P,
+                                 gam, sv);
+#pragma unroll
+                for (int v = 0; v < NVZ; v++) T[v * ncz + t] = sv[v];
+            }
+            __syncthreads();
+            if (live) {
+                real Wl[5], Wr[5];
+                if constexpr (ZC == 2) {
+#pragma unroll
+                    for (int q = 0; q < 5; q++) Wl[q] = (W + q * P)[cc], Wr[q] = (W + q * P)[cc + 1];
+                }
+                struct {
+                    const real* b;
+                    int         nc;
+                    __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
+                } sz{&T[kr], ncz};
+                face_flux<2, ZC, 0, 0>(sz, Wl, Wr, gam, zp, Learn{nullptr, false});
+#pragma unroll
+                for (int q = 0; q < 5; q++) Fz[q * (kw + 1) + kr + 1] = zp[q];
+                if (kr == 0) {
+#pragma unroll
+                    for (int q = 0; q < 5; q++) Fz[q * (kw + 1)] = (real)(fz + q * P)[cc - 1];
+                }
+            }
+            __syncthreads();
+            if (!live) continue;
+#pragma unroll
+            for (int q = 0; q < 5; q++) zm[q] = Fz[q * (kw + 1) + kr];
+        }
         double         R[5];
         bool           have[5];
 #pragma unroll
@@ -1749,10 +1817,16 @@ __device__ __forceinline__ void residual_fd_body(
                 R[q]   = have[q] ? R[q] + a : a;
                 R[q] += b;
                 if (g.dim == 3) {
+                    double zr, zl;
+                    if constexpr (ZC > 0) {
+                        zr = (double)zp[q], zl = (double)zm[q];
+                    } else {
+                        zr = (fz + q * P)[cc], zl = (fz + q * P)[cc - 1];
+                    }
 #if !NS_FAST
-                    R[q] += dv((fz + q * P)[cc] - (fz + q * P)[cc - 1], g.dz);
+                    R[q] += dv(zr - zl, g.dz);
 #else
-                    R[q] += ((fz + q * P)[cc] - (fz + q * P)[cc - 1]) * g.idz;
+                    R[q] += (zr - zl) * g.idz;
 #endif
                 }
                 have[q] = true;
@@ -1848,6 +1922,28 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
     cell_tile(bx, by, bz);
     residual_fd_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin, which, g, ph, bnd,
                                       bx, by, z0 + bz, KW_CELL, FUSED_MARCH);
+}
+
+// the residual + stage update with the z faces formed in the kernel (convection type ZC, Shu
+// weights, no kernel options; the host launches it for CD4 only).  kw in {64, 128, 256}: the
+// row segment a block's 256 threads cover (256 / kw rows); fz holds the first face of every
+// segment (face plane k = bx kw - 1/2, written by the plane launch before this kernel)
+#ifndef NS_ZFD_WAVES
+#define NS_ZFD_WAVES 2  // waves per SIMD the kernel is compiled for
+#endif
+template <int ORDER, bool SQ, bool SUTH, int ZC>
+__global__ __launch_bounds__(256, NS_ZFD_WAVES) void k_residual_zfd(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, StageArgs st,
+    const unsigned long long* __restrict__ dtmin, Geo g, Phys ph, Bnd bnd, int z0, int kw) {
+    __shared__ real zt[zfd_split_reals<ZC>()];
+    __shared__ real zf[zfd_face_reals<ZC>()];
+    int bx, by, bz;
+    cell_tile(bx, by, bz);
+    residual_fd_body<ORDER, SQ, SUTH, ZC>(fx, fy, fz, W, Dx, nullptr, nullptr, st, dtmin,
+                                          3 /* convection and diffusion */, g, ph, bnd, bx,
+                                          by, z0 + bz, kw,
+                                          FUSED_MARCH, zt, zf);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2603,6 +2699,61 @@ static void l_residual_fd(const double* fx, const double* fy, const double* fz, 
     }
 }
 
+#ifndef NS_ZFD
+#define NS_ZFD 1  // build k_residual_zfd (the host runs it when asked, $NSGPU_ZFD=1)
+#endif
+// segment width of k_residual_zfd for the configuration, 0 where it does not cover it (no z
+// convection, learned or Borges weights, kernel options, other orders than CD4, 3-D viscous
+// terms)
+static int l_zfd_kw(Geo g, Phys ph) {
+    if (!NS_ZFD || g.dim != 3 || !ph.conv || ph.weights != 0 || flux_opt(ph) != 0 ||
+        ph.diff_order != 4 || (ph.physics & 2))
+        return 0;
+    return g.nz <= 64 ? 64 : (g.nz <= 128 ? 128 : 256);
+}
+
+template <int ZC>
+static void residual_zfd_conv(const double* U, const double* fx, const double* fy, double* fz,
+                              const double* W, const double* Dx, const StageArgs& a,
+                              const unsigned long long* dtmin, Geo g, Phys ph, Bnd b, int kw,
+                              int lo, int hi, hipStream_t s) {
+    // the first face of every segment, planes i in [lo, hi)
+    const int  nseg = (g.nz + kw - 1) / kw;
+    const dim3 pb(64, 4, 1), pg((g.ny + 63) / 64, (hi - lo + 3) / 4, 1);
+    for (int m = 0; m < nseg; m++)
+        k_flux_plane<2, ZC, 0, 0><<<pg, pb, 0, s>>>(U, W, fz, g, ph.gamma, NGC - 1 + m * kw,
+                                                    nullptr, lo / 4);
+    const dim3 blk(64, 4, 1);
+    const dim3 grd(nseg, (g.ny + 256 / kw - 1) / (256 / kw),
+                   (hi - lo + FUSED_MARCH - 1) / FUSED_MARCH);
+    const int  z0 = lo / FUSED_MARCH;
+    if (ph.suth) {
+        if (g.square_xy)
+            k_residual_zfd<4, true, true, ZC><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, a, dtmin, g, ph, b, z0, kw);
+        else
+            k_residual_zfd<4, false, true, ZC><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, a, dtmin, g, ph, b, z0, kw);
+    } else {
+        if (g.square_xy)
+            k_residual_zfd<4, true, false, ZC><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, a, dtmin, g, ph, b, z0, kw);
+        else
+            k_residual_zfd<4, false, false, ZC><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, a, dtmin, g, ph, b, z0, kw);
+    }
+}
+
+// residual + stage update of planes [lo, hi) (lo a multiple of FUSED_MARCH) with the z faces
+// formed in the kernel; U, W: the stage's input state (st->Ucur and the primitives)
+static void l_residual_zfd(const double* fx, const double* fy, double* fz, const double* W,
+                           const double* Dx, const StageArgs* st,
+                           const unsigned long long* dtmin, Geo g, Phys ph, Bnd b, int lo,
+                           int hi, hipStream_t s) {
+    const int kw = l_zfd_kw(g, ph);
+    if (hi <= lo || !kw || !st) return;
+    if (ph.conv == 1)
+        residual_zfd_conv<1>(st->Ucur, fx, fy, fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
+    else
+        residual_zfd_conv<2>(st->Ucur, fx, fy, fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
+}
+
 static void l_residual(const double* fx, const double* fy, const double* fz, const double* W,
                        const double* Dx, const double* Dy, const double* Rin, double* Rout,
                        const StageArgs* st, const unsigned long long* dtmin, int which, Geo g,
@@ -2746,7 +2897,8 @@ const Launchers& launchers() {
                                 l_mlp_recon,    l_set_dt,     l_visc3d,     l_p2c_faces,
                                 l_residual_fd,  NS_FUSE_DIFF,
                                 // plane ranges of the cell kernels start on a march boundary
-                                (MARCH > FUSED_MARCH ? MARCH : FUSED_MARCH)};
+                                (MARCH > FUSED_MARCH ? MARCH : FUSED_MARCH),
+                                l_zfd_kw,       l_residual_zfd};
     return L;
 }
 

@@ -47,6 +47,10 @@ struct nsgpu_ctx {
     int                nslab = 1;
     int                slab[MAX_SLABS + 1] = {};
     hipEvent_t         ev_slab[MAX_SLABS] = {};
+    // the stage forms its z faces in the residual kernel where the launchers cover the case:
+    // an option ($NSGPU_ZFD=1 at creation), bitwise the same as z sweep + residual and slower at
+    // 256^3 (the fused kernel runs 2 waves per SIMD: 3.18 ms per stage against 1.27 + 1.68 ms)
+    bool               zfd_on = false;
     Geo                g{};
     Bnd                b{};
     Phys               ph{};
@@ -223,7 +227,7 @@ inline int full_hi(const nsgpu_ctx* c, int dir) { return dir == 0 ? c->g.nx + 1 
 
 // convection face fluxes of the current (U, W); fz stays zero for dim == 2
 void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts = 3,
-                       hipStream_t st = nullptr) {
+                       hipStream_t st = nullptr, bool skip_z = false) {
     if (!c->ph.conv) return;
     if (!st) st = c->stream;
     {
@@ -234,7 +238,7 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts
         Scope s(c, T_FLUX_Y);
         c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, parts, 0, full_hi(c, 1), st);
     }
-    if (c->g.dim == 3) {
+    if (c->g.dim == 3 && !skip_z) {
         Scope s(c, T_FLUX_Z);
         c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, parts, 0, full_hi(c, 2), st);
     }
@@ -244,6 +248,11 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts
 bool fused_diffusion(const nsgpu_ctx* c) {
     return c->L->fuse_diff && c->ph.diff_order && !(c->ph.physics & NSGPU_PHYS_VISCOUS) &&
            c->g.P < (1L << 31) && !c->linked;
+}
+
+// the stage's residual forms the z faces (k_residual_zfd): the z sweep is left out
+bool zfd(const nsgpu_ctx* c) {
+    return c->zfd_on && c->ph.conv && fused_diffusion(c) && c->L->zfd_kw(c->g, c->ph) > 0;
 }
 
 void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
@@ -267,14 +276,14 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
 // stream so that each kernel family is timed alone.
 void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
     if (c->timing || !c->aux) {
-        convection_fluxes(c, U, W);
+        convection_fluxes(c, U, W, 3, nullptr, zfd(c));
         diffusion_derivs(c, W);
         return;
     }
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->aux, c->ev_fork, 0);
     const Geo& g = c->g;
-    if (c->ph.conv && g.dim == 3)
+    if (c->ph.conv && g.dim == 3 && !zfd(c))
         c->L->face_flux(2, U, W, c->F[2], g, c->ph, 3, 0, full_hi(c, 2), c->aux);
     if (c->ph.conv) {
         c->L->face_flux(0, U, W, c->F[0], g, c->ph, 2, 0, full_hi(c, 0), c->aux);
@@ -328,13 +337,17 @@ void stage_pipelined(nsgpu_ctx* c, const double* U, const double* W, const Stage
         const int lo = c->slab[sl], hi = c->slab[sl + 1];
         L.face_flux(0, U, W, c->F[0], g, c->ph, 3, sl == 0 ? 0 : lo + 1, hi + 1, c->stream);
         L.face_flux(1, U, W, c->F[1], g, c->ph, 3, lo, hi, c->stream);
-        if (g.dim == 3) L.face_flux(2, U, W, c->F[2], g, c->ph, 3, lo, hi, c->stream);
+        if (g.dim == 3 && !zfd(c)) L.face_flux(2, U, W, c->F[2], g, c->ph, 3, lo, hi, c->stream);
         L.deriv1(W, c->Dx, nullptr, g, c->ph, lo, hi, c->stream);
         L.deriv_halo(c->Dx, nullptr, g, c->b, lo, hi, c->stream);
         hipEventRecord(c->ev_slab[sl], c->stream);
         hipStreamWaitEvent(c->aux, c->ev_slab[sl], 0);
-        L.residual_fd(c->F[0], c->F[1], c->F[2], W, c->Dx, nullptr, nullptr, &a, &c->dtmin[0],
-                      NSGPU_RHS_BOTH, g, c->ph, c->b, lo, hi, c->aux);
+        if (zfd(c))
+            L.residual_zfd(c->F[0], c->F[1], c->F[2], W, c->Dx, &a, &c->dtmin[0], g, c->ph, c->b,
+                           lo, hi, c->aux);
+        else
+            L.residual_fd(c->F[0], c->F[1], c->F[2], W, c->Dx, nullptr, nullptr, &a,
+                          &c->dtmin[0], NSGPU_RHS_BOTH, g, c->ph, c->b, lo, hi, c->aux);
     }
     hipEventRecord(c->ev_join, c->aux);
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
@@ -349,7 +362,10 @@ const double* visc_in(const nsgpu_ctx* c) {
 // residual (+ stage update) of the single block on the face fluxes F and the primitives W
 void residual(nsgpu_ctx* c, const double* W, const double* Rin, double* Rout, const StageArgs* a,
               const unsigned long long* dtmin, int which) {
-    if ((which & NSGPU_RHS_DIFFUSION) && fused_diffusion(c))
+    if (a && !Rin && !Rout && which == NSGPU_RHS_BOTH && zfd(c))  // a stage (stage_inputs)
+        c->L->residual_zfd(c->F[0], c->F[1], c->F[2], W, c->Dx, a, dtmin, c->g, c->ph, c->b, 0,
+                           c->g.nx, c->stream);
+    else if ((which & NSGPU_RHS_DIFFUSION) && fused_diffusion(c))
         c->L->residual_fd(c->F[0], c->F[1], c->F[2], W, c->Dx, Rin, Rout, a, dtmin, which, c->g,
                           c->ph, c->b, 0, c->g.nx, c->stream);
     else
@@ -456,6 +472,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
         if (hipEventCreateWithFlags(&c->ev_slab[t], hipEventDisableTiming) != hipSuccess)
             rc = NSGPU_ERR_HIP;
     if (!rc) plan_slabs(c);
+    if (const char* e = std::getenv("NSGPU_ZFD")) c->zfd_on = std::atoi(e) != 0;
     c->Ulast    = c->Ua;
     c->dt_scale = cfg->cfl;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
