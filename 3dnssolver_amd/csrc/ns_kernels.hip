@@ -1646,7 +1646,7 @@ __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
 // per access.
 // ------------------------------------------------------------------------------------------
 #ifndef NS_FUSED_WAVES
-#define NS_FUSED_WAVES 3  // waves per SIMD the fused kernel is compiled for (CD2 / CD4)
+#define NS_FUSED_WAVES 2  // waves per SIMD the fused kernel is compiled for (CD2 / CD4)
 #endif
 // the sixth-order stencils and the second derivatives on unequal dx, dy need more registers than
 // three waves per SIMD leave: two there
@@ -1665,6 +1665,152 @@ constexpr int FUSED_MARCH = NS_FUSED_MARCH;
 // (256 / kw rows): the split values of the kw + 5 cells of each row, and the kw + 1 face fluxes
 template <int ZC> constexpr int zfd_split_reals() { return ZC ? 4 * (64 + 5) * split_nv<0>() : 1; }
 template <int ZC> constexpr int zfd_face_reals() { return ZC ? 4 * 5 * (64 + 1) : 1; }
+
+// The march of a stage (convection and diffusion of a 3-D block, no residual input or output):
+// every value a plane needs from memory (face fluxes, the plane entering the x window, the rows
+// j +- 1 .. H of the primitives and of dW/dx, the stage inputs) is loaded at the top of the
+// plane, before any arithmetic, and nothing between the loads and the stores branches, so a
+// wavefront has one round trip to memory per plane in flight rather than one per group of
+// values.  The arithmetic is that of the general march below, operation for operation.
+template <int ORDER, bool SQ, bool SUTH>
+__device__ __forceinline__ void residual_fd_stage_march(
+    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const double* __restrict__ W, const double* __restrict__ Dx, const StageArgs& st,
+    const Geo& g, const Phys& ph, unsigned row, int i0, int i1,
+    double (&wd)[4][ORDER + 1], real (&dyx)[2][ORDER + 1], double (&fxm)[5], double m1,
+    double m2, double m3) {
+    constexpr int H  = ORDER / 2;
+    const long    P  = g.P;
+    const unsigned sx = (unsigned)g.sx;
+    const int      sy = g.sy;
+    const double* __restrict__ Un = st.Un;
+    const double* __restrict__ Uc = st.Ucur;
+    double* __restrict__       Uo = st.Uout;
+    double* __restrict__       Wo = st.Wout;
+    for (int i = i0; i < i1; i++) {
+        const unsigned cc = row + (unsigned)(i + NGC) * sx;
+        const unsigned cn = cc + (unsigned)H * sx;
+        // plane e = i + H enters the window; dW/dy of u, v there (across a periodic x the
+        // wrapped plane's; the caller keeps this march off planes beyond an x wall)
+        const int      e  = i + H;
+        const unsigned eb = row + (unsigned)((e >= g.nx ? e - g.nx : e) + NGC) * sx;
+        // ---- loads ----
+        double fxc[5], fyc[5], fym[5], fzc[5], fzm[5], unv[5], ucv[5], wn[4];
+        double uy[2][2 * H + 1], dxn[2][2 * H + 1], ry[2 * H + 1], py[2 * H + 1];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            fxc[q] = fx[q * P + cc];
+            fyc[q] = fy[q * P + cc];
+            fym[q] = fy[q * P + cc - sy];
+            fzc[q] = fz[q * P + cc];
+            fzm[q] = fz[q * P + cc - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) wn[q] = W[q * P + cn];
+#pragma unroll
+        for (int t = 0; t <= 2 * H; t++) {
+            if (t == H) continue;
+            const int o = (t - H) * sy;
+            uy[0][t]    = W[2 * P + eb + o];
+            uy[1][t]    = W[3 * P + eb + o];
+            dxn[0][t]   = Dx[2 * P + cc + o];
+            dxn[1][t]   = Dx[3 * P + cc + o];
+            ry[t]       = W[cc + o];
+            py[t]       = W[P + cc + o];
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            unv[q] = Un[q * P + cc];
+            ucv[q] = Uc[q * P + cc];
+        }
+        // ---- convection: x, then y, then z flux differences (weno-5.cpp:60-66) ----
+        double R[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const double fc = fxc[q];
+#if !NS_FAST
+            double a = dv(fc - fxm[q], g.dx);
+            double b = dv(fyc[q] - fym[q], g.dy);
+#else
+            double a = (fc - fxm[q]) * g.idx;
+            double b = (fyc[q] - fym[q]) * g.idy;
+#endif
+            fxm[q] = fc;
+            R[q]   = a;
+            R[q] += b;
+            double zr = fzc[q], zl = fzm[q];
+#if !NS_FAST
+            R[q] += dv(zr - zl, g.dz);
+#else
+            R[q] += (zr - zl) * g.idz;
+#endif
+        }
+        // ---- diffusion ----
+#pragma unroll
+        for (int q = 0; q < 4; q++) wd[q][2 * H] = wn[q];
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+            dyx[a][2 * H] = (real)fd_sum_t<ORDER, false, double>(
+                [&](int t) { return uy[a][t]; }, g.dy, g.idy);
+        double dxo[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            dxo[q] = fd_sum_t<ORDER, false, double>([&](int t) { return wd[q][t]; }, g.dx, g.idx);
+        double dxy[2][2 * H + 1];
+#pragma unroll
+        for (int t = 0; t <= 2 * H; t++) {
+            if (t == H) continue;
+            dxy[0][t] = dxn[0][t];
+            dxy[1][t] = dxn[1][t];
+        }
+        dxy[0][H]        = dxo[2];
+        dxy[1][H]        = dxo[3];
+        const double dxr = dxo[0], dxp = dxo[1];
+        double dyr = fd_sum_t<ORDER, false, double>([&](int t) { return ry[t]; }, g.dy, g.idy);
+        double dyp = fd_sum_t<ORDER, false, double>([&](int t) { return py[t]; }, g.dy, g.idy);
+        real   dmu_dy = 0;
+        if constexpr (SUTH)
+            dmu_dy = dmu_dy_of<ORDER>(
+                [&](int t, double& r, double& q) {
+                    r = ry[t];
+                    q = py[t];
+                },
+                g, ph);
+        real wx[4][2 * H + 1];
+#pragma unroll
+        for (int t = 0; t <= 2 * H; t++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) wx[q][t] = (real)wd[q][t];
+        real ene, xm, ym;
+        viscous_v<ORDER, SQ, SUTH>(wx, dyx, dxy, dxr, dxp, dyr, dyp, dmu_dy, g, ph, ene, xm, ym);
+        R[1] = R[1] + ene;
+        R[2] = R[2] + xm;
+        R[3] = R[3] + ym;
+#pragma unroll
+        for (int t = 0; t < 2 * H; t++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) wd[q][t] = wd[q][t + 1];
+            dyx[0][t] = dyx[0][t + 1];
+            dyx[1][t] = dyx[1][t + 1];
+        }
+        // ---- SSP-RK3 stage (time_scheme.cpp:72-96), primitives of the result ----
+        double un[5], w[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double r = R[q];
+#if !NS_FAST
+            un[q] = (unv[q] * m1) + (ucv[q] * m2) + (r * m3);
+#else
+            un[q] = unv[q] * m1 + ucv[q] * m2 + r * m3;
+#endif
+            Uo[q * P + cc] = un[q];
+        }
+        c2p(un, ph.gamma, w);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Wo[q * P + cc] = w[q];
+        if (st.flags) stage_health(st.flags, un, w);
+    }
+}
 
 // 256 threads: kw lanes along k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of
 // `march` planes i).  ZC > 0: the z face fluxes of the plane are formed here (convection type
@@ -1749,6 +1895,11 @@ __device__ __forceinline__ void residual_fd_body(
         } else {
             m1 = 1. / 3., m2 = 2. / 3., m3 = -(2. / 3.) * dt;
         }
+    }
+    if (ZC == 0 && st.Uout && !Rin && !Rout && conv && g.dim == 3 && (xper || i1 + H <= g.nx)) {
+        residual_fd_stage_march<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1, wd,
+                                                 dyx, fxm, m1, m2, m3);
+        return;
     }
     for (int i = i0; i < i1; i++) {
         const unsigned cc = row + (unsigned)(i + NGC) * sx;
@@ -2385,7 +2536,7 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
 // u and v from the derivative pass, the rest formed in the residual
 // (blocks of a decomposition take the separate path: their derivative halos are exchanged)
 __device__ __forceinline__ bool ens_fused(const Geo& g, int linked = 0) {
-    return NS_FUSE_DIFF && g.P < (1L << 31) && !linked;
+    return NS_FUSE_DIFF && (long)g.NX * g.sx < (1L << 31) && !linked;
 }
 
 template <int ORDER>

@@ -56,6 +56,7 @@ struct nsgpu_ctx {
     Phys               ph{};
     const Launchers*   L = nullptr;
     size_t             plane_bytes = 0;  // one padded plane
+    size_t             arr_bytes   = 0;  // one 5-variable array (4-variable arrays take as much)
     // device buffers (SoA)
     double*            W  = nullptr;  // primitive state (5 planes)
     double*            W2 = nullptr;  // next primitive state (written by the last stage)
@@ -247,7 +248,7 @@ void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts
 // the residual kernel forms the y derivatives of the 2-D viscous terms itself
 bool fused_diffusion(const nsgpu_ctx* c) {
     return c->L->fuse_diff && c->ph.diff_order && !(c->ph.physics & NSGPU_PHYS_VISCOUS) &&
-           c->g.P < (1L << 31) && !c->linked;
+           (long)c->g.NX * c->g.sx < (1L << 31) && !c->linked;
 }
 
 // the stage's residual forms the z faces (k_residual_zfd): the z sweep is left out
@@ -407,9 +408,17 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     Geo& g    = c->g;
     g.nx = cfg->nx, g.ny = cfg->ny, g.nz = cfg->nz;
     g.NX = g.nx + 2 * NGC, g.NY = g.ny + 2 * NGC, g.NZ = g.nz + 2 * NGC;
-    g.sy = (g.NZ + KALIGN - 1) / KALIGN * KALIGN;
-    g.sx = g.NY * g.sy;
-    g.P  = (long)g.NX * g.sx;
+    const int row = (g.NZ + KALIGN - 1) / KALIGN * KALIGN;  // one padded row of one variable
+    if (const char* e = std::getenv("NSGPU_LAYOUT"); e && std::atoi(e) == 1) {
+        // the five variables of a row side by side: [i][j][q][k]
+        g.P  = row;
+        g.sy = 5 * row;
+        g.sx = g.NY * g.sy;
+    } else {
+        g.sy = row;
+        g.sx = g.NY * g.sy;
+        g.P  = (long)g.NX * g.sx;
+    }
     // Grid::compute_step_sizes (grid.cpp:14-16)
     set_spacing(g, cfg->Lx / cfg->nx, cfg->Ly / cfg->ny, cfg->Lz / cfg->nz);
     g.dim = cfg->dim;
@@ -430,8 +439,9 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     c->L = (cfg->math == NSGPU_MATH_FAST)   ? &fast::launchers()
            : (cfg->math == NSGPU_MATH_FP32) ? &fp32::launchers()
                                             : &exact::launchers();
-    c->plane_bytes = (size_t)g.P * sizeof(double);
-    size_t five = 5 * c->plane_bytes, four = 4 * c->plane_bytes;
+    c->plane_bytes = (size_t)g.NX * g.NY * row * sizeof(double);
+    c->arr_bytes   = 5 * c->plane_bytes;
+    size_t five = c->arr_bytes, four = (g.P == row) ? five : 4 * c->plane_bytes;
     rc = NSGPU_OK;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = NSGPU_ERR_HIP;
     if (!rc) rc = alloc_array(&c->W, five, c->stream);
@@ -977,8 +987,8 @@ int ens_save(nsgpu_ensemble* e, EnsSaved& sv) {
     sv.time.resize(n), sv.dt_scale.resize(n), sv.dtbits.resize(n);
     for (size_t b = 0; b < n; b++) {
         nsgpu_ctx* c = e->m[b];
-        HIPCHK(hipMemcpyAsync(c->R, c->W, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
-        HIPCHK(hipMemcpyAsync(c->stage_aos, c->Ua, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(c->R, c->W, c->arr_bytes, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(c->stage_aos, c->Ua, c->arr_bytes, hipMemcpyDeviceToDevice,
                               e->stream));
         HIPCHK(hipMemcpyAsync(&sv.time[b], c->dtime, sizeof(double), hipMemcpyDeviceToHost,
                               e->stream));
@@ -993,8 +1003,8 @@ int ens_save(nsgpu_ensemble* e, EnsSaved& sv) {
 int ens_restore(nsgpu_ensemble* e, const EnsSaved& sv) {
     for (size_t b = 0; b < e->m.size(); b++) {
         nsgpu_ctx* c = e->m[b];
-        HIPCHK(hipMemcpyAsync(c->W, c->R, 5 * c->plane_bytes, hipMemcpyDeviceToDevice, e->stream));
-        HIPCHK(hipMemcpyAsync(c->Ua, c->stage_aos, 5 * c->plane_bytes, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(c->W, c->R, c->arr_bytes, hipMemcpyDeviceToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(c->Ua, c->stage_aos, c->arr_bytes, hipMemcpyDeviceToDevice,
                               e->stream));
         HIPCHK(hipMemcpyAsync(c->dtime, &sv.time[b], sizeof(double), hipMemcpyHostToDevice,
                               e->stream));
