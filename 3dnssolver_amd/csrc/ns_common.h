@@ -28,7 +28,8 @@ struct Geo {
     long   P;           // plane stride
     double dx, dy, dz;  // Grid::dx() etc. (grid.cpp:14-16)
     double dxx, dyy;    // dx*dx, dy*dy (CDN second-derivative divisors, cd-n.cpp:67-79)
-    double idx, idy, idz, idxx, idyy;  // reciprocals (FAST math only)
+    double idx, idy, idz, idxx, idyy, idzz;  // correctly rounded reciprocals 1.0 / dx etc.
+                                             // (FAST: factors; EXACT: the divisions' reciprocals)
     int    dim;
     int    square_xy;   // dx*dx == dy*dy bitwise: the two x-stencil second derivatives coincide
 };

@@ -657,6 +657,10 @@ __device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const 
     }
 }
 
+// an empty instruction that reads and may rewrite x: x's load is complete here and cannot be
+// moved below this point (the compiler otherwise sinks loads towards their first use)
+template <class T> __device__ __forceinline__ void pin(T& x) { asm volatile("" : "+v"(x)); }
+
 template <int NV>
 struct RegSplit {  // split values held in registers: s[ii][v]
     const real (*s)[NV];
@@ -1158,7 +1162,10 @@ __device__ __forceinline__ T fd_sum_t(const F& f, T d, T id) {
     T             s     = T(0.);
     bool          first = true;
 #if !NS_FAST
-    const auto rd = rcp(d);
+    // id is the correctly rounded 1 / d (host): the corrected quotient below is then the
+    // correctly rounded one (the same value as the device's refined reciprocal gives), and a
+    // loop-invariant divisor costs no reciprocal in the kernel
+    const Rcp rd{d, id};
 #endif
 #pragma unroll
     for (int t = 0; t <= 2 * H; t++) {
@@ -1335,13 +1342,15 @@ __device__ __forceinline__ real mu_suth(real rho, real p, const Phys& ph) {
 // dxy[a][t]: dW/dx of u (a = 0), v (a = 1) at y offsets t - H (FP64, as the derivative planes
 // hold them); dxr, dxp, dyr, dyp: dW/dx and dW/dy of rho and p at the cell; dmu_dy (Sutherland):
 // the y derivative of the viscosity at the cell
+// muw (Sutherland, optional): the viscosity mu_suth(wx[0][t], wx[1][t]) at the x offsets,
+// formed by a caller that carries it along its march
 template <int ORDER, bool SQ, bool SUTH>
 __device__ __forceinline__ void viscous_v(const real (&wx)[4][ORDER + 1],
                                           const real (&dyx)[2][ORDER + 1],
                                           const double (&dxy)[2][ORDER + 1], double dxr,
                                           double dxp, double dyr, double dyp, real dmu_dy_c,
                                           const Geo& g, const Phys& ph, real& ene, real& x_mom,
-                                          real& y_mom) {
+                                          real& y_mom, const real* muw = nullptr) {
     constexpr int H   = ORDER / 2;
     const int     rho = 0, p = 1, u = 2, v = 3;
     // second derivatives: ddf_dxdx and ddf_dydy both run the x stencil (cd-n.cpp:67-79); when
@@ -1383,9 +1392,14 @@ __device__ __forceinline__ void viscous_v(const real (&wx)[4][ORDER + 1],
     } else {
         // viscosity and its first derivatives from the primitive stencils; the full expressions
         // of diffusion_scheme.cpp:3-58 with the dmu / dlamb terms (centered_diffusion.cpp:96-145)
-        mu     = mu_suth(RHO, Pp, ph);
-        dmu_dx = fd_sum<ORDER, false>([&](int t) { return mu_suth(wx[rho][t], wx[p][t], ph); },
-                                      g.dx, g.idx);
+        if (muw) {
+            mu     = muw[H];
+            dmu_dx = fd_sum<ORDER, false>([&](int t) { return muw[t]; }, g.dx, g.idx);
+        } else {
+            mu     = mu_suth(RHO, Pp, ph);
+            dmu_dx = fd_sum<ORDER, false>(
+                [&](int t) { return mu_suth(wx[rho][t], wx[p][t], ph); }, g.dx, g.idx);
+        }
         dmu_dy = dmu_dy_c;
         real lamb     = (-RL(2.) / RL(3.)) * mu;
         real dlamb_dx = -(RL(2.) / RL(3.)) * dmu_dx;
@@ -1544,8 +1558,8 @@ __device__ __forceinline__ void residual_body(
             for (int q = 0; q < 5; q++) {
                 double fc = fx[c + q * P];
 #if !NS_FAST
-                double a = dv(fc - fxm[q], g.dx);
-                double b = dv(fy[c + q * P] - fy[c - g.sy + q * P], g.dy);
+                double a = dv(fc - fxm[q], Rcp{g.dx, g.idx});
+                double b = dv(fy[c + q * P] - fy[c - g.sy + q * P], Rcp{g.dy, g.idy});
 #else
                 double a = (fc - fxm[q]) * g.idx;
                 double b = (fy[c + q * P] - fy[c - g.sy + q * P]) * g.idy;
@@ -1555,7 +1569,7 @@ __device__ __forceinline__ void residual_body(
                 R[q] += b;
                 if (g.dim == 3) {
 #if !NS_FAST
-                    R[q] += dv(fz[c + q * P] - fz[c - 1 + q * P], g.dz);
+                    R[q] += dv(fz[c + q * P] - fz[c - 1 + q * P], Rcp{g.dz, g.idz});
 #else
                     R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) * g.idz;
 #endif
@@ -1669,24 +1683,54 @@ template <int ZC> constexpr int zfd_face_reals() { return ZC ? 4 * 5 * (64 + 1) 
 // The march of a stage (convection and diffusion of a 3-D block, no residual input or output):
 // every value a plane needs from memory (face fluxes, the plane entering the x window, the rows
 // j +- 1 .. H of the primitives and of dW/dx, the stage inputs) is loaded at the top of the
-// plane, before any arithmetic, and nothing between the loads and the stores branches, so a
-// wavefront has one round trip to memory per plane in flight rather than one per group of
-// values.  The arithmetic is that of the general march below, operation for operation.
-template <int ORDER, bool SQ, bool SUTH>
+// plane and held (pin) before any arithmetic, and nothing between the loads and the stores
+// branches: a wavefront waits for memory once per plane.  The x window (rho, p, u, v, dW/dy of
+// u and v and, for the Sutherland law, mu at the 2H + 1 planes) lives in LDS when `ring` is
+// given ([variable][slot][thread], a ring over the planes, each thread its own column: no
+// barrier), else in registers; the registers then go to the loads in flight.  The arithmetic
+// is that of the general march below, operation for operation.
+#ifndef NS_RING
+#define NS_RING 1  // the single-block stage march keeps its x window in LDS
+#endif
+template <int ORDER, bool SUTH> constexpr int ring_vars() { return SUTH ? 7 : 6; }
+template <int ORDER, bool SUTH> constexpr int ring_doubles() {
+    return ring_vars<ORDER, SUTH>() * (ORDER + 1) * 256;
+}
+template <int ORDER, bool SQ, bool SUTH, bool LW>
 __device__ __forceinline__ void residual_fd_stage_march(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const StageArgs& st,
     const Geo& g, const Phys& ph, unsigned row, int i0, int i1,
     double (&wd)[4][ORDER + 1], real (&dyx)[2][ORDER + 1], double (&fxm)[5], double m1,
-    double m2, double m3) {
+    double m2, double m3, double* __restrict__ ring) {
     constexpr int H  = ORDER / 2;
+    constexpr int NS = 2 * H + 1;  // planes of the window
     const long    P  = g.P;
     const unsigned sx = (unsigned)g.sx;
     const int      sy = g.sy;
+    const int      tid = threadIdx.x + threadIdx.y * 64;
     const double* __restrict__ Un = st.Un;
     const double* __restrict__ Uc = st.Ucur;
     double* __restrict__       Uo = st.Uout;
     double* __restrict__       Wo = st.Wout;
+    // window variable v (0..3 rho, p, u, v; 4, 5 dW/dy of u, v; 6 mu) at ring slot sl
+    auto rs = [&](int v, int sl) -> double& { return ring[(v * NS + sl) * 256 + tid]; };
+    // Sutherland: the viscosity along the x window, carried like the primitives
+    real muw[NS];
+    if constexpr (SUTH) {
+#pragma unroll
+        for (int t = 0; t < 2 * H; t++) muw[t] = mu_suth((real)wd[0][t], (real)wd[1][t], ph);
+    }
+    if constexpr (LW) {  // the first 2H planes of the window into slots 0 .. 2H-1
+#pragma unroll
+        for (int t = 0; t < 2 * H; t++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) rs(q, t) = wd[q][t];
+            rs(4, t) = dyx[0][t], rs(5, t) = dyx[1][t];
+            if constexpr (SUTH) rs(6, t) = muw[t];
+        }
+    }
+    int sl0 = 0;  // ring slot of window offset 0
     for (int i = i0; i < i1; i++) {
         const unsigned cc = row + (unsigned)(i + NGC) * sx;
         const unsigned cn = cc + (unsigned)H * sx;
@@ -1696,7 +1740,7 @@ __device__ __forceinline__ void residual_fd_stage_march(
         const unsigned eb = row + (unsigned)((e >= g.nx ? e - g.nx : e) + NGC) * sx;
         // ---- loads ----
         double fxc[5], fyc[5], fym[5], fzc[5], fzm[5], unv[5], ucv[5], wn[4];
-        double uy[2][2 * H + 1], dxn[2][2 * H + 1], ry[2 * H + 1], py[2 * H + 1];
+        double uy[2][NS], dxn[2][NS], ry[NS], py[NS];
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             fxc[q] = fx[q * P + cc];
@@ -1708,7 +1752,7 @@ __device__ __forceinline__ void residual_fd_stage_march(
 #pragma unroll
         for (int q = 0; q < 4; q++) wn[q] = W[q * P + cn];
 #pragma unroll
-        for (int t = 0; t <= 2 * H; t++) {
+        for (int t = 0; t < NS; t++) {
             if (t == H) continue;
             const int o = (t - H) * sy;
             uy[0][t]    = W[2 * P + eb + o];
@@ -1720,8 +1764,23 @@ __device__ __forceinline__ void residual_fd_stage_march(
         }
 #pragma unroll
         for (int q = 0; q < 5; q++) {
-            unv[q] = Un[q * P + cc];
             ucv[q] = Uc[q * P + cc];
+            unv[q] = Un[q * P + cc];
+        }
+        if constexpr (LW) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                pin(fxc[q]), pin(fyc[q]), pin(fym[q]), pin(fzc[q]), pin(fzm[q]);
+                pin(ucv[q]), pin(unv[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) pin(wn[q]);
+#pragma unroll
+            for (int t = 0; t < NS; t++) {
+                if (t == H) continue;
+                pin(uy[0][t]), pin(uy[1][t]), pin(dxn[0][t]), pin(dxn[1][t]), pin(ry[t]),
+                    pin(py[t]);
+            }
         }
         // ---- convection: x, then y, then z flux differences (weno-5.cpp:60-66) ----
         double R[5];
@@ -1729,8 +1788,8 @@ __device__ __forceinline__ void residual_fd_stage_march(
         for (int q = 0; q < 5; q++) {
             const double fc = fxc[q];
 #if !NS_FAST
-            double a = dv(fc - fxm[q], g.dx);
-            double b = dv(fyc[q] - fym[q], g.dy);
+            double a = dv(fc - fxm[q], Rcp{g.dx, g.idx});
+            double b = dv(fyc[q] - fym[q], Rcp{g.dy, g.idy});
 #else
             double a = (fc - fxm[q]) * g.idx;
             double b = (fyc[q] - fym[q]) * g.idy;
@@ -1740,25 +1799,56 @@ __device__ __forceinline__ void residual_fd_stage_march(
             R[q] += b;
             double zr = fzc[q], zl = fzm[q];
 #if !NS_FAST
-            R[q] += dv(zr - zl, g.dz);
+            R[q] += dv(zr - zl, Rcp{g.dz, g.idz});
 #else
             R[q] += (zr - zl) * g.idz;
 #endif
         }
-        // ---- diffusion ----
+        // ---- diffusion: the window at this plane ----
+        const real dyu = (real)fd_sum_t<ORDER, false, double>([&](int t) { return uy[0][t]; },
+                                                              g.dy, g.idy);
+        const real dyv = (real)fd_sum_t<ORDER, false, double>([&](int t) { return uy[1][t]; },
+                                                              g.dy, g.idy);
+        double     wl[4][NS];
+        real       wx[4][NS], dy2[2][NS], mw[NS];
+        if constexpr (LW) {
+            const int sn = (sl0 + 2 * H < NS) ? sl0 + 2 * H : sl0 + 2 * H - NS;
 #pragma unroll
-        for (int q = 0; q < 4; q++) wd[q][2 * H] = wn[q];
+            for (int q = 0; q < 4; q++) rs(q, sn) = wn[q];
+            rs(4, sn) = dyu, rs(5, sn) = dyv;
+            if constexpr (SUTH) rs(6, sn) = mu_suth((real)wn[0], (real)wn[1], ph);
 #pragma unroll
-        for (int a = 0; a < 2; a++)
-            dyx[a][2 * H] = (real)fd_sum_t<ORDER, false, double>(
-                [&](int t) { return uy[a][t]; }, g.dy, g.idy);
+            for (int t = 0; t < NS; t++) {
+                const int sl = (sl0 + t < NS) ? sl0 + t : sl0 + t - NS;
+#pragma unroll
+                for (int q = 0; q < 4; q++) wl[q][t] = rs(q, sl), wx[q][t] = (real)wl[q][t];
+                dy2[0][t] = (real)rs(4, sl), dy2[1][t] = (real)rs(5, sl);
+                if constexpr (SUTH) mw[t] = (real)rs(6, sl);
+            }
+            sl0 = (sl0 + 1 < NS) ? sl0 + 1 : 0;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) wd[q][2 * H] = wn[q];
+            dyx[0][2 * H] = dyu, dyx[1][2 * H] = dyv;
+#pragma unroll
+            for (int t = 0; t < NS; t++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) wl[q][t] = wd[q][t], wx[q][t] = (real)wd[q][t];
+                dy2[0][t] = dyx[0][t], dy2[1][t] = dyx[1][t];
+            }
+            if constexpr (SUTH) {
+                muw[2 * H] = mu_suth(wx[0][2 * H], wx[1][2 * H], ph);
+#pragma unroll
+                for (int t = 0; t < NS; t++) mw[t] = muw[t];
+            }
+        }
         double dxo[4];
 #pragma unroll
         for (int q = 0; q < 4; q++)
-            dxo[q] = fd_sum_t<ORDER, false, double>([&](int t) { return wd[q][t]; }, g.dx, g.idx);
-        double dxy[2][2 * H + 1];
+            dxo[q] = fd_sum_t<ORDER, false, double>([&](int t) { return wl[q][t]; }, g.dx, g.idx);
+        double dxy[2][NS];
 #pragma unroll
-        for (int t = 0; t <= 2 * H; t++) {
+        for (int t = 0; t < NS; t++) {
             if (t == H) continue;
             dxy[0][t] = dxn[0][t];
             dxy[1][t] = dxn[1][t];
@@ -1776,22 +1866,21 @@ __device__ __forceinline__ void residual_fd_stage_march(
                     q = py[t];
                 },
                 g, ph);
-        real wx[4][2 * H + 1];
-#pragma unroll
-        for (int t = 0; t <= 2 * H; t++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) wx[q][t] = (real)wd[q][t];
         real ene, xm, ym;
-        viscous_v<ORDER, SQ, SUTH>(wx, dyx, dxy, dxr, dxp, dyr, dyp, dmu_dy, g, ph, ene, xm, ym);
+        viscous_v<ORDER, SQ, SUTH>(wx, dy2, dxy, dxr, dxp, dyr, dyp, dmu_dy, g, ph, ene, xm, ym,
+                                   SUTH ? mw : nullptr);
         R[1] = R[1] + ene;
         R[2] = R[2] + xm;
         R[3] = R[3] + ym;
+        if constexpr (!LW) {
 #pragma unroll
-        for (int t = 0; t < 2 * H; t++) {
+            for (int t = 0; t < 2 * H; t++) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) wd[q][t] = wd[q][t + 1];
-            dyx[0][t] = dyx[0][t + 1];
-            dyx[1][t] = dyx[1][t + 1];
+                for (int q = 0; q < 4; q++) wd[q][t] = wd[q][t + 1];
+                dyx[0][t] = dyx[0][t + 1];
+                dyx[1][t] = dyx[1][t + 1];
+                if constexpr (SUTH) muw[t] = muw[t + 1];
+            }
         }
         // ---- SSP-RK3 stage (time_scheme.cpp:72-96), primitives of the result ----
         double un[5], w[5];
@@ -1820,13 +1909,14 @@ __device__ __forceinline__ void residual_fd_stage_march(
 // from fz, where the plane launch of that face stored it.  The same face_flux on the same split
 // values as the z sweep: the residual is bitwise the one the sweep's faces give.  Every thread
 // of the block reaches the two barriers of each plane; threads without a cell skip the work.
-template <int ORDER, bool SQ, bool SUTH, int ZC = 0>
+template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false>
 __device__ __forceinline__ void residual_fd_body(
     const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, const StageArgs& st, const unsigned long long* __restrict__ dtmin,
     int which, const Geo& g, const Phys& ph, const Bnd& bnd, int bx, int by, int bz, int kw,
-    int march, real* __restrict__ zt = nullptr, real* __restrict__ zf = nullptr) {
+    int march, real* __restrict__ zt = nullptr, real* __restrict__ zf = nullptr,
+    double* __restrict__ ring = nullptr) {
     constexpr int H   = ORDER / 2;
     const int     tid = threadIdx.x + threadIdx.y * 64;
     const int     kr  = tid & (kw - 1);  // lane within the row segment
@@ -1897,8 +1987,8 @@ __device__ __forceinline__ void residual_fd_body(
         }
     }
     if (ZC == 0 && st.Uout && !Rin && !Rout && conv && g.dim == 3 && (xper || i1 + H <= g.nx)) {
-        residual_fd_stage_march<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1, wd,
-                                                 dyx, fxm, m1, m2, m3);
+        residual_fd_stage_march<ORDER, SQ, SUTH, LW>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1,
+                                                     wd, dyx, fxm, m1, m2, m3, ring);
         return;
     }
     for (int i = i0; i < i1; i++) {
@@ -1958,8 +2048,8 @@ __device__ __forceinline__ void residual_fd_body(
             for (int q = 0; q < 5; q++) {
                 double fc = (fx + q * P)[cc];
 #if !NS_FAST
-                double a = dv(fc - fxm[q], g.dx);
-                double b = dv((fy + q * P)[cc] - (fy + q * P)[cc - sy], g.dy);
+                double a = dv(fc - fxm[q], Rcp{g.dx, g.idx});
+                double b = dv((fy + q * P)[cc] - (fy + q * P)[cc - sy], Rcp{g.dy, g.idy});
 #else
                 double a = (fc - fxm[q]) * g.idx;
                 double b = ((fy + q * P)[cc] - (fy + q * P)[cc - sy]) * g.idy;
@@ -1975,7 +2065,7 @@ __device__ __forceinline__ void residual_fd_body(
                         zr = (fz + q * P)[cc], zl = (fz + q * P)[cc - 1];
                     }
 #if !NS_FAST
-                    R[q] += dv(zr - zl, g.dz);
+                    R[q] += dv(zr - zl, Rcp{g.dz, g.idz});
 #else
                     R[q] += (zr - zl) * g.idz;
 #endif
@@ -2069,10 +2159,14 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
     int which, Geo g, Phys ph, Bnd bnd, int z0) {
+    // the x window of the stage march (one thread's column per slot: no barrier)
+    __shared__ double ring[NS_RING ? ring_doubles<ORDER, SUTH>() : 1];
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    residual_fd_body<ORDER, SQ, SUTH>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin, which, g, ph, bnd,
-                                      bx, by, z0 + bz, KW_CELL, FUSED_MARCH);
+    residual_fd_body<ORDER, SQ, SUTH, 0, NS_RING != 0>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin,
+                                                       which, g, ph, bnd, bx, by, z0 + bz,
+                                                       KW_CELL, FUSED_MARCH, nullptr, nullptr,
+                                                       ring);
 }
 
 // the residual + stage update with the z faces formed in the kernel (convection type ZC, Shu
@@ -2183,7 +2277,7 @@ __global__ __launch_bounds__(256) void k_visc3d(const double* __restrict__ W,
     const double* D[3]  = {Dx, Dy, Dz};
     const double  dd[3] = {g.dx, g.dy, g.dz}, idd[3] = {g.idx, g.idy, g.idz};
     const double  d2[3] = {g.dxx, g.dyy, g.dz * g.dz};
-    const double  id2[3] = {g.idxx, g.idyy, g.idz * g.idz};
+    const double  id2[3] = {g.idxx, g.idyy, g.idzz};
     real gr[3][3];
 #pragma unroll
     for (int a = 0; a < 3; a++)

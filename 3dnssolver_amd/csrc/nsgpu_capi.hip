@@ -207,7 +207,7 @@ void set_spacing(Geo& g, double dx, double dy, double dz) {
     g.dxx = g.dx * g.dx;  // CDN::ddf_dxdx / ddf_dydy divisors (cd-n.cpp:67-79)
     g.dyy = g.dy * g.dy;
     g.idx = 1.0 / g.dx, g.idy = 1.0 / g.dy, g.idz = 1.0 / g.dz;
-    g.idxx = 1.0 / g.dxx, g.idyy = 1.0 / g.dyy;
+    g.idxx = 1.0 / g.dxx, g.idyy = 1.0 / g.dyy, g.idzz = 1.0 / (g.dz * g.dz);
     g.square_xy = (g.dxx == g.dyy);
 }
 
