@@ -45,6 +45,10 @@ namespace NS_MODE {
 
 typedef NS_REAL real;
 #define RL(x) ((real)(x))
+// face fluxes are stored in the precision they are formed in: FP64 planes in the EXACT and FAST
+// builds, FP32 planes in the FP32 build (half the bytes the sweeps write and the residual reads;
+// the values are the ones the FP64 planes held, float to double is exact)
+typedef real flux_t;
 
 // ------------------------------------------------------------------------------------------
 // constants (convection_constants.hpp:4-12, cd_constants.hpp:9-27)
@@ -717,7 +721,7 @@ constexpr int TILE_S1 = NS_TILE_S1;
 template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
-                                                  double* __restrict__ f, const Geo& g,
+                                                  flux_t* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
                                                   const MlpPacked* __restrict__ mlp, int lo,
@@ -773,7 +777,7 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
 
 template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy(
-    const double* __restrict__ U, const double* __restrict__ W, double* __restrict__ f, Geo g,
+    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, const MlpPacked* mlp, int lo, int hi) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
     __shared__ real lq[WT == 2 ? TILE_S1 * LQ_REALS : 1];  // network queues (learned weights)
@@ -790,7 +794,7 @@ template <int OPT> constexpr int zlds_wave_max() { return 4 * (16 + 5) * split_n
 template <int CONV, int WT, int OPT>
 __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
                                                  const double* __restrict__ W,
-                                                 double* __restrict__ f, const Geo& g,
+                                                 flux_t* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
                                                  real* __restrict__ lds,
                                                  const MlpPacked* __restrict__ mlp,
@@ -846,7 +850,7 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
 template <int CONV, int WT, int OPT>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
-                                                     double* __restrict__ f, Geo g,
+                                                     flux_t* __restrict__ f, Geo g,
                                                      real gamma, const MlpPacked* mlp, int lo) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<OPT>()];
     __shared__ real lq[WT == 2 ? 8 * LQ_REALS : 1];
@@ -859,7 +863,7 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
 template <int DIR, int CONV, int WT, int OPT>
 __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
                                                 const double* __restrict__ W,
-                                                double* __restrict__ f, const Geo& g, real gamma,
+                                                flux_t* __restrict__ f, const Geo& g, real gamma,
                                                 int i, int bx, int by,
                                                 const MlpPacked* __restrict__ mlp,
                                                 real* lq = nullptr) {
@@ -899,7 +903,7 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
 template <int DIR, int CONV, int WT, int OPT>
 __global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
                                                     const double* __restrict__ W,
-                                                    double* __restrict__ f, Geo g, real gamma,
+                                                    flux_t* __restrict__ f, Geo g, real gamma,
                                                     int i, const MlpPacked* mlp, int by0) {
     __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
     flux_plane_body<DIR, CONV, WT, OPT>(U, W, f, g, gamma, i, blockIdx.x, by0 + blockIdx.y, mlp,
@@ -1499,7 +1503,7 @@ __device__ __forceinline__ void viscous(const real (&wx)[4][ORDER + 1],
 // (4 planes: energy, x, y, z momentum); Dy is unused
 template <int ORDER, bool SQ, bool SUTH, bool VC = false>
 __device__ __forceinline__ void residual_body(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
     const double* __restrict__ Rin, double* __restrict__ Rout, const StageArgs& st,
     const unsigned long long* __restrict__ dtmin, int which, const Geo& g, const Phys& ph, int bx,
@@ -1559,19 +1563,19 @@ __device__ __forceinline__ void residual_body(
                 double fc = fx[c + q * P];
 #if !NS_FAST
                 double a = dv(fc - fxm[q], Rcp{g.dx, g.idx});
-                double b = dv(fy[c + q * P] - fy[c - g.sy + q * P], Rcp{g.dy, g.idy});
+                double b = dv((double)fy[c + q * P] - (double)fy[c - g.sy + q * P], Rcp{g.dy, g.idy});
 #else
                 double a = (fc - fxm[q]) * g.idx;
-                double b = (fy[c + q * P] - fy[c - g.sy + q * P]) * g.idy;
+                double b = ((double)fy[c + q * P] - (double)fy[c - g.sy + q * P]) * g.idy;
 #endif
                 fxm[q] = fc;
                 R[q]   = have[q] ? R[q] + a : a;
                 R[q] += b;
                 if (g.dim == 3) {
 #if !NS_FAST
-                    R[q] += dv(fz[c + q * P] - fz[c - 1 + q * P], Rcp{g.dz, g.idz});
+                    R[q] += dv((double)fz[c + q * P] - (double)fz[c - 1 + q * P], Rcp{g.dz, g.idz});
 #else
-                    R[q] += (fz[c + q * P] - fz[c - 1 + q * P]) * g.idz;
+                    R[q] += ((double)fz[c + q * P] - (double)fz[c - 1 + q * P]) * g.idz;
 #endif
                 }
                 have[q] = true;
@@ -1633,7 +1637,7 @@ __device__ __forceinline__ void residual_body(
 
 template <int ORDER, bool SQ, bool SUTH, bool VC = false>
 __global__ __launch_bounds__(256, NS_CELL_WAVES) void k_residual(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Dy,
     const double* __restrict__ Rin, double* __restrict__ Rout, StageArgs st,
     const unsigned long long* __restrict__ dtmin, int which, Geo g, Phys ph) {
@@ -1698,7 +1702,7 @@ template <int ORDER, bool SUTH> constexpr int ring_doubles() {
 }
 template <int ORDER, bool SQ, bool SUTH, bool LW>
 __device__ __forceinline__ void residual_fd_stage_march(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const StageArgs& st,
     const Geo& g, const Phys& ph, unsigned row, int i0, int i1,
     double (&wd)[4][ORDER + 1], real (&dyx)[2][ORDER + 1], double (&fxm)[5], double m1,
@@ -1911,7 +1915,7 @@ __device__ __forceinline__ void residual_fd_stage_march(
 // of the block reaches the two barriers of each plane; threads without a cell skip the work.
 template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false>
 __device__ __forceinline__ void residual_fd_body(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, const StageArgs& st, const unsigned long long* __restrict__ dtmin,
     int which, const Geo& g, const Phys& ph, const Bnd& bnd, int bx, int by, int bz, int kw,
@@ -2049,10 +2053,10 @@ __device__ __forceinline__ void residual_fd_body(
                 double fc = (fx + q * P)[cc];
 #if !NS_FAST
                 double a = dv(fc - fxm[q], Rcp{g.dx, g.idx});
-                double b = dv((fy + q * P)[cc] - (fy + q * P)[cc - sy], Rcp{g.dy, g.idy});
+                double b = dv((double)(fy + q * P)[cc] - (double)(fy + q * P)[cc - sy], Rcp{g.dy, g.idy});
 #else
                 double a = (fc - fxm[q]) * g.idx;
-                double b = ((fy + q * P)[cc] - (fy + q * P)[cc - sy]) * g.idy;
+                double b = ((double)(fy + q * P)[cc] - (double)(fy + q * P)[cc - sy]) * g.idy;
 #endif
                 fxm[q] = fc;
                 R[q]   = have[q] ? R[q] + a : a;
@@ -2155,7 +2159,7 @@ __device__ __forceinline__ void residual_fd_body(
 
 template <int ORDER, bool SQ, bool SUTH>
 __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_residual_fd(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
     double* __restrict__ Rout, StageArgs st, const unsigned long long* __restrict__ dtmin,
     int which, Geo g, Phys ph, Bnd bnd, int z0) {
@@ -2178,7 +2182,7 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
 #endif
 template <int ORDER, bool SQ, bool SUTH, int ZC>
 __global__ __launch_bounds__(256, NS_ZFD_WAVES) void k_residual_zfd(
-    const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+    const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, StageArgs st,
     const unsigned long long* __restrict__ dtmin, Geo g, Phys ph, Bnd bnd, int z0, int kw) {
     __shared__ real zt[zfd_split_reals<ZC>()];
@@ -2588,7 +2592,7 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     const real      gam = (real)E->ph.gamma;
     const int       kw  = E->t.kw_flux;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
-    double*       f = E->F[DIR];
+    flux_t*       f = (flux_t*)E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
     // the whole block: faces 0 .. nx-1 of the x sweep, planes 0 .. nx-1 of the y sweep
     flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx,
@@ -2605,7 +2609,7 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T,
     const real      gam = (real)E->ph.gamma;
     const int       kw  = E->t.kw_flux;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
-    double*       f = E->F[2];
+    flux_t*       f = (flux_t*)E->F[2];
     const MlpPacked* mlp = E->ph.mlp;
     flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp,
                                     WT == 2 ? lq : nullptr);
@@ -2619,7 +2623,7 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     const real      gam = (real)E->ph.gamma;
     const int       n   = (DIR == 0) ? g.nx : (DIR == 1) ? g.ny : g.nz;
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
-    double*       f = E->F[DIR];
+    flux_t*       f = (flux_t*)E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
     __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
     flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp,
@@ -2679,7 +2683,9 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
     const Geo       g  = E->g;
     const Phys      ph = E->ph;
     const int       kw = E->t.kw_cell, march = E->t.march;
-    const double *fx = E->F[0], *fy = E->F[1], *fz = E->F[2], *Dx = E->Dx, *Dy = E->Dy;
+    const flux_t *fx = (const flux_t*)E->F[0], *fy = (const flux_t*)E->F[1],
+                 *fz = (const flux_t*)E->F[2];
+    const double *Dx = E->Dx, *Dy = E->Dy;
     const double* Wp = ens_w(E, s);
     StageArgs     a{E->Un, ens_u(E, s), s == 1 ? E->Ub : E->Ua,
                     s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s, E->flags};
@@ -2769,7 +2775,7 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
 // the tile kernel takes f < nx, the last plane f = nx), for the y and z sweeps the planes i (both
 // kernels); the whole block is [0, nx + 1) resp. [0, nx).  lo is a multiple of 4 for y and z.
 template <int DIR, int CONV, int B, int OPT>
-static void flux_dir(const double* U, const double* W, double* f, Geo g, double gamma,
+static void flux_dir(const double* U, const double* W, flux_t* f, Geo g, double gamma,
                      const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s) {
     if (!(parts & 1)) {
     } else if constexpr (DIR == 2) {
@@ -2813,7 +2819,7 @@ static int flux_opt(const Phys& ph) {
 }
 
 template <int DIR, int CONV, int B>
-static void flux_dir_opt(const double* U, const double* W, double* f, Geo g, double gamma,
+static void flux_dir_opt(const double* U, const double* W, flux_t* f, Geo g, double gamma,
                          const MlpPacked* m, int opt, int parts, int lo, int hi, hipStream_t s) {
     if constexpr (B == 2) {
         flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
@@ -2829,7 +2835,7 @@ static void flux_dir_opt(const double* U, const double* W, double* f, Geo g, dou
 }
 
 template <int DIR>
-static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Phys ph,
+static void face_flux_dir(const double* U, const double* W, flux_t* f, Geo g, Phys ph,
                           int parts, int lo, int hi, hipStream_t s) {
     const MlpPacked* m   = ph.mlp;
     const int        opt = flux_opt(ph);
@@ -2846,9 +2852,10 @@ static void face_flux_dir(const double* U, const double* W, double* f, Geo g, Ph
 
 static void l_face_flux(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
                         int parts, int lo, int hi, hipStream_t s) {
-    if (dir == 0) face_flux_dir<0>(U, W, f, g, ph, parts, lo, hi, s);
-    else if (dir == 1) face_flux_dir<1>(U, W, f, g, ph, parts, lo, hi, s);
-    else face_flux_dir<2>(U, W, f, g, ph, parts, lo, hi, s);
+    flux_t* ff = (flux_t*)f;  // the build's flux precision (see flux_t)
+    if (dir == 0) face_flux_dir<0>(U, W, ff, g, ph, parts, lo, hi, s);
+    else if (dir == 1) face_flux_dir<1>(U, W, ff, g, ph, parts, lo, hi, s);
+    else face_flux_dir<2>(U, W, ff, g, ph, parts, lo, hi, s);
 }
 
 // planes [lo, hi) (lo a multiple of MARCH)
@@ -2885,7 +2892,7 @@ static void l_deriv_halo(double* Dx, double* Dy, Geo g, Bnd b, int lo, int hi, h
 }
 
 template <int ORDER>
-static void residual_order(const double* fx, const double* fy, const double* fz, const double* W,
+static void residual_order(const flux_t* fx, const flux_t* fy, const flux_t* fz, const double* W,
                            const double* Dx, const double* Dy, const double* Rin, double* Rout,
                            StageArgs a, const unsigned long long* dtmin, int which, Geo g,
                            Phys ph, hipStream_t s) {
@@ -2908,7 +2915,7 @@ static void residual_order(const double* fx, const double* fy, const double* fz,
 
 // planes [lo, hi) (lo a multiple of FUSED_MARCH)
 template <int ORDER>
-static void residual_fd_order(const double* fx, const double* fy, const double* fz,
+static void residual_fd_order(const flux_t* fx, const flux_t* fy, const flux_t* fz,
                               const double* W, const double* Dx, const double* Rin, double* Rout,
                               StageArgs a,
                               const unsigned long long* dtmin, int which, Geo g, Phys ph, Bnd b,
@@ -2938,9 +2945,9 @@ static void l_residual_fd(const double* fx, const double* fy, const double* fz, 
     StageArgs a{};
     if (st) a = *st;
     switch (ph.diff_order) {
-    case 2: residual_fd_order<2>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
-    case 4: residual_fd_order<4>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
-    default: residual_fd_order<6>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
+    case 2: residual_fd_order<2>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
+    case 4: residual_fd_order<4>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
+    default: residual_fd_order<6>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, lo, hi, s); break;
     }
 }
 
@@ -2958,7 +2965,7 @@ static int l_zfd_kw(Geo g, Phys ph) {
 }
 
 template <int ZC>
-static void residual_zfd_conv(const double* U, const double* fx, const double* fy, double* fz,
+static void residual_zfd_conv(const double* U, const flux_t* fx, const flux_t* fy, flux_t* fz,
                               const double* W, const double* Dx, const StageArgs& a,
                               const unsigned long long* dtmin, Geo g, Phys ph, Bnd b, int kw,
                               int lo, int hi, hipStream_t s) {
@@ -2994,9 +3001,9 @@ static void l_residual_zfd(const double* fx, const double* fy, double* fz, const
     const int kw = l_zfd_kw(g, ph);
     if (hi <= lo || !kw || !st) return;
     if (ph.conv == 1)
-        residual_zfd_conv<1>(st->Ucur, fx, fy, fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
+        residual_zfd_conv<1>(st->Ucur, (const flux_t*)fx, (const flux_t*)fy, (flux_t*)fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
     else
-        residual_zfd_conv<2>(st->Ucur, fx, fy, fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
+        residual_zfd_conv<2>(st->Ucur, (const flux_t*)fx, (const flux_t*)fy, (flux_t*)fz, W, Dx, *st, dtmin, g, ph, b, kw, lo, hi, s);
 }
 
 static void l_residual(const double* fx, const double* fy, const double* fz, const double* W,
@@ -3006,10 +3013,10 @@ static void l_residual(const double* fx, const double* fy, const double* fz, con
     StageArgs a{};
     if (st) a = *st;
     switch (ph.diff_order) {
-    case 2: residual_order<2>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
-    case 4: residual_order<4>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
-    case 6: residual_order<6>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
-    default: residual_order<0>(fx, fy, fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    case 2: residual_order<2>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    case 4: residual_order<4>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    case 6: residual_order<6>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
+    default: residual_order<0>((const flux_t*)fx, (const flux_t*)fy, (const flux_t*)fz, W, Dx, Dy, Rin, Rout, a, dtmin, which, g, ph, s); break;
     }
 }
 
