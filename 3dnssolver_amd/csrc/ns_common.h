@@ -1,12 +1,15 @@
 // ns_common.h -- block geometry, device layout and the kernel launch table shared by the C-ABI
 // (nsgpu_capi.hip) and the kernel translation units (ns_kernels.hip, compiled once per math mode).
 //
-// Device layout: structure of arrays.  A 5-variable field is 5 planes of the padded block, each
-// plane row-major [NX][NY][sy] with k fastest (the reference Array4d order with q moved out,
-// containers/array_4d.hpp:116-124), NX = nx + 2*NGC.  Rows are padded to sy = NZ rounded up to
-// KALIGN doubles and every array starts KOFF doubles into its allocation, so the first interior
-// cell of every row (k = NGC) sits on a 128-byte boundary: a wavefront reading 64 consecutive
-// interior k of one variable reads exactly four 128-byte lines.  Plane stride P = NX*NY*sy.
+// Device layout: rows of one variable.  A 5-variable field of the padded block holds, for every
+// (i, j), the five rows of k (k fastest; the reference Array4d order, containers/array_4d.hpp:
+// 116-124, with q moved out of the innermost index) side by side: [NX][NY][5][row], NX = nx +
+// 2*NGC, so P (the stride between variables) is one row, sy = 5 rows and sx = NY*sy.  (The
+// planar layout [5][NX][NY][row], P = one plane, is selected at run time; the kernels only use
+// P, sx and sy.)  Rows are padded to NZ rounded up to KALIGN doubles and every array starts KOFF
+// doubles into its allocation, so the first interior cell of every row (k = NGC) sits on a
+// 128-byte boundary: a wavefront reading 64 consecutive interior k of one variable reads exactly
+// four 128-byte lines.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -25,7 +28,7 @@ struct Geo {
     int    nx, ny, nz;  // interior cells
     int    NX, NY, NZ;  // padded extents
     int    sx, sy;      // strides of i and j (k stride is 1)
-    long   P;           // plane stride
+    long   P;           // stride between the variables of an array (one row or one plane)
     double dx, dy, dz;  // Grid::dx() etc. (grid.cpp:14-16)
     double dxx, dyy;    // dx*dx, dy*dy (CDN second-derivative divisors, cd-n.cpp:67-79)
     double idx, idy, idz, idxx, idyy, idzz;  // correctly rounded reciprocals 1.0 / dx etc.

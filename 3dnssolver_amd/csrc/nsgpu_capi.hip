@@ -409,15 +409,19 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     g.nx = cfg->nx, g.ny = cfg->ny, g.nz = cfg->nz;
     g.NX = g.nx + 2 * NGC, g.NY = g.ny + 2 * NGC, g.NZ = g.nz + 2 * NGC;
     const int row = (g.NZ + KALIGN - 1) / KALIGN * KALIGN;  // one padded row of one variable
-    if (const char* e = std::getenv("NSGPU_LAYOUT"); e && std::atoi(e) == 1) {
-        // the five variables of a row side by side: [i][j][q][k]
-        g.P  = row;
-        g.sy = 5 * row;
-        g.sx = g.NY * g.sy;
-    } else {
+    // Device layout of a 5-variable array: the five rows of one (i, j) side by side,
+    // [i][j][q][k] (variable stride P = one row), so the values a cell needs from one array lie
+    // in one 5-row stretch of memory; $NSGPU_LAYOUT=0 selects whole planes per variable,
+    // [q][i][j][k] (P = one plane).  The kernels only see P, sx and sy.  At 256^3 the interleaved
+    // rows take the residual kernel from 1.30 to 1.22 ms and the step from 14.7 to 14.5 ms.
+    if (const char* e = std::getenv("NSGPU_LAYOUT"); e && std::atoi(e) == 0) {
         g.sy = row;
         g.sx = g.NY * g.sy;
         g.P  = (long)g.NX * g.sx;
+    } else {
+        g.P  = row;
+        g.sy = 5 * row;
+        g.sx = g.NY * g.sy;
     }
     // Grid::compute_step_sizes (grid.cpp:14-16)
     set_spacing(g, cfg->Lx / cfg->nx, cfg->Ly / cfg->ny, cfg->Lz / cfg->nz);
