@@ -27,7 +27,7 @@ FLOP_PER_FACE = {"weno5cw": 1190.0, "weno5": 725.0}
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector = matrix), dense
 # planes the residual + stage-update kernel moves per launch, each once: 31 read (fx, fy, fz, Un,
 # Ucur: 5 each; W: 4; dW/dx of u, v: 2; it forms the other derivatives itself) and 10 written
-# (Uout, Wout)
+# (Uout, Wout); FP64 planes, except the face fluxes of the FP32 build (FP32 planes)
 RES_PLANES = 41
 # with the z sweep fused into it, the residual kernel reads U and W of the z stencils instead of
 # fz: 31 read (fx, fy, Un, Ucur: 5 each; W: 5; dW/dx of u, v: 2; the first face of each row: a
@@ -140,7 +140,8 @@ def precision_workload(args):
     flop = FLOP_PER_FACE["weno5cw"] * (n + 1) * n * n
     fp32_tf = flop / (dom_ms * 1e-3) / 1e12
     res_ms = tm["residual_update"][0] / tm["residual_update"][1]
-    res_gbs = RES_PLANES * 8.0 * n ** 3 / (res_ms * 1e-3) / 1e9
+    # the FP32 build's residual reads its 15 face-flux planes in single precision
+    res_gbs = ((RES_PLANES - 15) * 8.0 + 15 * 4.0) * n ** 3 / (res_ms * 1e-3) / 1e9
     fp64_dom = max((k for k in out["exact_timers"] if k.startswith("face_flux")),
                    key=lambda k: out["exact_timers"][k][0])
     fp64_ms = out["exact_timers"][fp64_dom][0] / out["exact_timers"][fp64_dom][1]
@@ -487,7 +488,9 @@ def kernel_rooflines(timers, args, n):
     if res and res[1]:
         ms = res[0] / res[1]
         planes = RES_PLANES_FUSED_Z if timers.get("face_flux_z", (0, 0))[1] == 0 else RES_PLANES
-        b = planes * 8.0 * n ** 3
+        # the FP32 build stores the 15 face-flux planes it reads in single precision
+        flux_bytes = 4.0 if args.math == "fp32" else 8.0
+        b = ((planes - 15) * 8.0 + 15 * flux_bytes) * n ** 3
         gbs = b / (ms * 1e-3) / 1e9
         out["residual_update"] = {"bound": "hbm", "kernel": "residual_update",
                                   "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
