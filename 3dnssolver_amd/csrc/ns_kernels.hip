@@ -115,10 +115,11 @@ __device__ __forceinline__ float dv(float n, float d) { return n * __builtin_amd
 // ------------------------------------------------------------------------------------------
 // WENO-5 (weno-5.cpp:222-322)
 // ------------------------------------------------------------------------------------------
+// bo (optional): the smoothness indicators b0, b1, b2 of the stencil
 template <bool BORGES>
 __device__ __forceinline__ real weno_combine(real a, real b, real c, real d, real e,
                                                real d0, real d1, real d2, real f0,
-                                               real f1, real f2) {
+                                               real f1, real f2, real* bo = nullptr) {
     real t1, t2;
     t1        = a - RL(2.) * b + c;
     t2        = a - RL(4.) * b + RL(3.) * c;
@@ -129,6 +130,7 @@ __device__ __forceinline__ real weno_combine(real a, real b, real c, real d, rea
     t1        = c - RL(2.) * d + e;
     t2        = RL(3.) * c - RL(4.) * d + e;
     real b2 = THIRTEENPERTWELVE * t1 * t1 + ONEPERFOUR * t2 * t2;
+    if (bo) bo[0] = b0, bo[1] = b1, bo[2] = b2;
 #if !NS_FAST
     real a0, a1, a2;
     if constexpr (BORGES) {
@@ -169,20 +171,22 @@ __device__ __forceinline__ real weno_combine(real a, real b, real c, real d, rea
 
 // Weno5::WENO5_left (weno-5.cpp:305-322), ideal weights d_left = {0.1, 0.6, 0.3}
 template <bool BORGES>
-__device__ __forceinline__ real weno_left_c(real a, real b, real c, real d, real e) {
+__device__ __forceinline__ real weno_left_c(real a, real b, real c, real d, real e,
+                                            real* bo = nullptr) {
     real f0 = (RL(2.) * a - RL(7.) * b + RL(11.) * c) * ONEPERSIX;
     real f1 = (-b + RL(5.) * c + RL(2.) * d) * ONEPERSIX;
     real f2 = (RL(2.) * c + RL(5.) * d - e) * ONEPERSIX;
-    return weno_combine<BORGES>(a, b, c, d, e, RL(0.1), RL(0.6), RL(0.3), f0, f1, f2);
+    return weno_combine<BORGES>(a, b, c, d, e, RL(0.1), RL(0.6), RL(0.3), f0, f1, f2, bo);
 }
 
 // Weno5::WENO5_right (weno-5.cpp:285-303), ideal weights d_right = {0.3, 0.6, 0.1}
 template <bool BORGES>
-__device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, real e) {
+__device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, real e,
+                                             real* bo = nullptr) {
     real f0 = (-a + RL(5.) * b + RL(2.) * c) * ONEPERSIX;
     real f1 = (RL(2.) * b + RL(5.) * c - d) * ONEPERSIX;
     real f2 = (RL(11.) * c - RL(7.) * d + RL(2.) * e) * ONEPERSIX;
-    return weno_combine<BORGES>(a, b, c, d, e, RL(0.3), RL(0.6), RL(0.1), f0, f1, f2);
+    return weno_combine<BORGES>(a, b, c, d, e, RL(0.3), RL(0.6), RL(0.1), f0, f1, f2, bo);
 }
 
 
@@ -202,10 +206,32 @@ struct Learn {
     real*            q = nullptr;
 };
 
-// the classic Shu reconstruction of the stencil v in left-biased order (side 1: mirrored)
-__device__ __forceinline__ real learned_classic(const real v[5], int side) {
-    return side ? weno_right_c<false>(v[4], v[3], v[2], v[1], v[0])
-                : weno_left_c<false>(v[0], v[1], v[2], v[3], v[4]);
+// the classic Shu reconstruction of the stencil v in left-biased order (side 1: mirrored);
+// bo (optional): its smoothness indicators
+__device__ __forceinline__ real learned_classic(const real v[5], int side, real* bo = nullptr) {
+    return side ? weno_right_c<false>(v[4], v[3], v[2], v[1], v[0], bo)
+                : weno_left_c<false>(v[0], v[1], v[2], v[3], v[4], bo);
+}
+
+// regime of the stencil v (0 smooth, 1 network, 2 flagged discontinuity) from the smoothness
+// indicators b of its classic reconstruction: the indicators of the range-normalised stencil
+// (v - min) / r are b / r^2, so their ratio (max + EPS) / (min + EPS) is (max b + EPS r^2) /
+// (min b + EPS r^2), compared here without a division.  A stencil whose range r is at most
+// MLP_RANGE of its largest magnitude counts as constant (smooth).  The same classification as
+// mlp_features' up to rounding, for the price of a few operations on values the classic
+// reconstruction has formed anyway.
+__device__ __forceinline__ int learned_regime(const real v[5], const real b[3]) {
+    real m = v[0], M = v[0];
+#pragma unroll
+    for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
+    const real r  = M - m;
+    const real am = (m < RL(0)) ? -m : m, aM = (M < RL(0)) ? -M : M;
+    if (!(r > (real)MLP_RANGE * (am > aM ? am : aM))) return 0;
+    const real e  = (real)MLP_EPS * (r * r);
+    real       lo = b[0] < b[1] ? b[0] : b[1], hi = b[0] < b[1] ? b[1] : b[0];
+    lo = b[2] < lo ? b[2] : lo, hi = b[2] > hi ? b[2] : hi;
+    const real num = hi + e, den = lo + e;
+    return num < (real)MLP_SMOOTH * den ? 0 : (num > (real)MLP_FLAG * den ? 2 : 1);
 }
 
 // the blend of the network's reconstruction (weights w of the three ENO candidates of v) with
@@ -223,12 +249,14 @@ template <int SIDE>
 __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, real e,
                                              const Learn& L, int* regime = nullptr) {
     const real v[5]    = {SIDE ? e : a, SIDE ? d : b, c, SIDE ? b : d, SIDE ? a : e};
-    const real classic = learned_classic(v, SIDE);
-    float      x[8], ratio;
-    const int  rg   = mlp_features(v, x, &ratio);
-    const bool need = L.active && rg == 1;
+    real       bs[3];
+    const real classic = learned_classic(v, SIDE, bs);
+    const int  rg      = learned_regime(v, bs);
+    const bool need    = L.active && rg == 1;
     if (regime) *regime = rg;
     if (__ballot(need) == 0) return classic;  // wavefront-uniform
+    float x[8], ratio;
+    mlp_features(v, x, &ratio);
     float xs[8], w[3];
 #pragma unroll
     for (int k = 0; k < 8; k++) xs[k] = need ? x[k] : 0.0f;
@@ -292,9 +320,9 @@ __device__ __forceinline__ void face_flux_learned_cw(const S& st, real out[5], c
 #pragma unroll
             for (int k = 0; k < 5; k++) v[k] = st(5 - k, 5 + qv);  // WENO5_right mirrored
         }
-        const real classic = learned_classic(v, side);
-        float      x[8], ratio;
-        const bool need = L.active && mlp_features(v, x, &ratio) == 1;
+        real       bs[3];
+        const real classic = learned_classic(v, side, bs);
+        const bool need    = L.active && learned_regime(v, bs) == 1;
         const unsigned long long m = __ballot(need);
         const int cnt = __popcll(m);
         if (base + cnt > 64) {  // wavefront-uniform
