@@ -267,7 +267,7 @@ __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, rea
 
 // ---- batched network samples of one face (component-wise sweeps) ---------------------------
 // A wavefront's queue holds up to 64 stencils that need the network: their five values
-// (left-biased order) and side, as planes [6][64] of `real` in LDS.  The reconstructions of a
+// (left-biased order) and their classic reconstruction, as planes [6][64] of `real` in LDS.  The reconstructions of a
 // face enqueue their network stencils; when the next one would overflow the queue, and after the
 // last, the wavefront evaluates the queued samples in one 64-sample network call (lane l takes
 // sample l), writes each result over the sample's first value, and every lane takes back the
@@ -291,13 +291,13 @@ __device__ __forceinline__ void learn_flush(const MlpPacked* P, real* q, int n) 
     wave_sync();
 #pragma unroll
     for (int k = 0; k < 5; k++) v[k] = has ? q[64 * k + l] : RL(0.0);
-    const int side = has ? (int)q[64 * 5 + l] : 0;
-    float     x[8], xs[8], ratio, w[3];
+    const real classic = has ? q[64 * 5 + l] : RL(0.0);
+    float      x[8], xs[8], ratio, w[3];
     mlp_features(v, x, &ratio);
 #pragma unroll
     for (int k = 0; k < 8; k++) xs[k] = has ? x[k] : 0.0f;
     mlp_weights_lane(P, xs, w);
-    if (has) q[l] = learned_finish(v, w, learned_classic(v, side), ratio);
+    if (has) q[l] = learned_finish(v, w, classic, ratio);
     wave_sync();
 }
 
@@ -339,7 +339,7 @@ __device__ __forceinline__ void face_flux_learned_cw(const S& st, real out[5], c
             const int s = base + __popcll(m & below);
 #pragma unroll
             for (int k = 0; k < 5; k++) L.q[64 * k + s] = v[k];
-            L.q[64 * 5 + s] = (real)side;
+            L.q[64 * 5 + s] = classic;
             slot[r]         = s;
         }
         base += cnt;
