@@ -59,6 +59,23 @@ struct HaloLink {
 
 struct MlpPacked;  // ns_mlp.h
 
+// Network samples of the learned weights, emitted by the single-block component-wise face tiles
+// and evaluated by a separate network kernel.  Two sets (0: the x and y sweeps, which run one
+// after the other on one stream; 1: the z sweep, which may run beside them), each with two sides
+// (0: left-biased reconstructions, 1: right-biased).  A side holds `cap` samples: planes
+// [5][cap] of the build's real type (the stencil in left-biased order) and the flux index of
+// each sample.  A launch splits its workgroups over
+// LS_REGIONS regions (workgroup id modulo LS_REGIONS), each with its own counter and a fixed
+// share of `cap`, so no counter serialises the whole launch.
+constexpr int LS_REGIONS = 64;
+constexpr int LS_STRIDE  = 32;  // counters one 128-byte line apart
+struct LearnSink {
+    void*     v[2][2];     // [set][side]: samples, planes [5][cap] of real
+    unsigned* dest[2][2];  // [set][side]: flux index of each sample
+    unsigned* cnt[2];      // [set]: counters [side][LS_REGIONS] at stride LS_STRIDE
+    long      cap;         // samples per side and set
+};
+
 struct Phys {
     double gamma, mu, Pr, cfl;
     double lamb;        // (-2./3.) * mu            (centered_diffusion.cpp:101)
@@ -74,6 +91,9 @@ struct Phys {
     double suth_C;      // Sutherland constant (T0 = 1)
     double suth_k;      // mu * (1.0 + C)
     const MlpPacked* mlp;  // packed network of the learned weights (device), or null
+    const LearnSink* sink; // host-side descriptor of the sample buffers of the single-block
+                           // learned face tiles, or null (the tiles then evaluate the network
+                           // themselves)
 };
 
 // SSP-RK3 stage (time_scheme.cpp:37-63); the coefficients are formed on the device from dt
@@ -230,6 +250,9 @@ struct Launchers {
     void (*residual_zfd)(const double* fx, const double* fy, double* fz, const double* W,
                          const double* Dx, const StageArgs* st, const unsigned long long* dtmin,
                          Geo g, Phys ph, Bnd b, int lo, int hi, hipStream_t s);
+    // samples per side and set of the LearnSink the full-block sweeps of g need, and the bytes of
+    // one sample value in this build
+    long (*learn_cap)(Geo g, int* value_bytes);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

@@ -17,6 +17,8 @@
 //       the equation of state and the reductions stay FP64.
 //
 // Citations are paths relative to the reference's src/ directory.
+#include <algorithm>
+
 #include "ns_common.h"
 #include "ns_mlp.h"
 
@@ -112,6 +114,11 @@ __device__ __forceinline__ RcpF  rcp(float d) { return {d, __builtin_amdgcn_rcpf
 __device__ __forceinline__ float dv(float n, const RcpF& R) { return n * R.r; }
 __device__ __forceinline__ float dv(float n, float d) { return n * __builtin_amdgcn_rcpf(d); }
 
+// an empty instruction that reads and may rewrite x: x (a load or a computed value) is complete
+// here and cannot be moved below this point (the compiler otherwise sinks loads and arithmetic
+// towards their first use, which can keep many intermediate values live)
+template <class T> __device__ __forceinline__ void pin(T& x) { asm volatile("" : "+v"(x)); }
+
 // ------------------------------------------------------------------------------------------
 // WENO-5 (weno-5.cpp:222-322)
 // ------------------------------------------------------------------------------------------
@@ -200,10 +207,24 @@ __device__ __forceinline__ real weno_right_c(real a, real b, real c, real d, rea
 // ------------------------------------------------------------------------------------------
 // q: this wavefront's queue of network samples in LDS (LearnQueue), or null: the face kernels
 // batch the network samples of all reconstructions of a face (face_flux_learned_cw)
+// e: emit mode (single-block component-wise tiles): the network stencils go to the sample
+// buffers of the launch (LearnSink) and a separate kernel evaluates them; fc is this thread's
+// face (flux index of variable 0), fP the stride between variables, region its workgroup's
+// region of the buffers
+struct LearnEmit {
+    real*     v[2];     // per side: planes [5][cap] (stencil in left-biased order)
+    unsigned* dest[2];  // per side: flux index of each sample
+    unsigned* cnt;      // counters [side][LS_REGIONS] at stride LS_STRIDE
+    long      cap;      // samples per side
+    long      capr;     // samples per region of this launch
+};
 struct Learn {
     const MlpPacked* P;
     bool             active;
     real*            q = nullptr;
+    LearnEmit        e = {};
+    unsigned         fc = 0, fP = 0;
+    int              region = 0;
 };
 
 // the classic Shu reconstruction of the stencil v in left-biased order (side 1: mirrored);
@@ -219,14 +240,18 @@ __device__ __forceinline__ real learned_classic(const real v[5], int side, real*
 // (min b + EPS r^2), compared here without a division.  A stencil whose range r is at most
 // MLP_RANGE of its largest magnitude counts as constant (smooth).  The same classification as
 // mlp_features' up to rounding, for the price of a few operations on values the classic
-// reconstruction has formed anyway.
-__device__ __forceinline__ int learned_regime(const real v[5], const real b[3]) {
+// reconstruction has formed anyway.  scale (>= 0): a magnitude the range is also measured
+// against (the face's momentum scale for a momentum component, learned_face_scale)
+__device__ __forceinline__ int learned_regime(const real v[5], const real b[3],
+                                              real scale = RL(0.0)) {
     real m = v[0], M = v[0];
 #pragma unroll
     for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
     const real r  = M - m;
     const real am = (m < RL(0)) ? -m : m, aM = (M < RL(0)) ? -M : M;
-    if (!(r > (real)MLP_RANGE * (am > aM ? am : aM))) return 0;
+    real       mg = am > aM ? am : aM;
+    mg            = scale > mg ? scale : mg;
+    if (!(r > (real)MLP_RANGE * mg)) return 0;
     const real e  = (real)MLP_EPS * (r * r);
     real       lo = b[0] < b[1] ? b[0] : b[1], hi = b[0] < b[1] ? b[1] : b[0];
     lo = b[2] < lo ? b[2] : lo, hi = b[2] > hi ? b[2] : hi;
@@ -245,13 +270,15 @@ __device__ __forceinline__ real learned_finish(const real v[5], const float w[3]
     return classic + (real)mlp_blend(ratio) * (net - classic);
 }
 
+// scale: as in learned_regime
 template <int SIDE>
 __device__ __forceinline__ real weno_learned(real a, real b, real c, real d, real e,
-                                             const Learn& L, int* regime = nullptr) {
+                                             const Learn& L, int* regime = nullptr,
+                                             real scale = RL(0.0)) {
     const real v[5]    = {SIDE ? e : a, SIDE ? d : b, c, SIDE ? b : d, SIDE ? a : e};
     real       bs[3];
     const real classic = learned_classic(v, SIDE, bs);
-    const int  rg      = learned_regime(v, bs);
+    const int  rg      = learned_regime(v, bs, scale);
     const bool need    = L.active && rg == 1;
     if (regime) *regime = rg;
     if (__ballot(need) == 0) return classic;  // wavefront-uniform
@@ -301,6 +328,150 @@ __device__ __forceinline__ void learn_flush(const MlpPacked* P, real* q, int n) 
     wave_sync();
 }
 
+// momentum scale of a component-wise face: the largest split value (fL and fR) of the three
+// momentum components (stored positions 2..4) at the face's two cells.  A momentum stencil also
+// counts as constant when its range is at most MLP_RANGE of this scale: a momentum component
+// that vanishes in exact arithmetic (the spanwise velocity of a plane shear flow) holds only
+// round-off, whose stencils are not structure for the network, however they look normalised.
+template <class S>
+__device__ __forceinline__ real learned_face_scale(const S& st) {
+    real s = RL(0.0);
+#pragma unroll
+    for (int q = 2; q < 5; q++) {
+#pragma unroll
+        for (int ii = 2; ii < 4; ii++) {
+            const real a = st(ii, q), b = st(ii, 5 + q);
+            const real x = a < RL(0) ? -a : a, y = b < RL(0) ? -b : b;
+            s = x > s ? x : s;
+            s = y > s ? y : s;
+        }
+    }
+    return s;
+}
+
+// stencil of reconstruction r = 2 q + side of a component-wise face: WENO5_left of fL of cells
+// i-2 .. i+2, or WENO5_right of fR of cells i-1 .. i+3 mirrored (left-biased order)
+template <class S>
+__device__ __forceinline__ void learned_stencil(const S& st, int r, real v[5]) {
+    const int qv = r >> 1;
+    if ((r & 1) == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = st(k, qv);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = st(5 - k, 5 + qv);
+    }
+}
+
+// component-wise face flux with the learned weights, emit mode.  Every reconstruction is first
+// formed with the classic weights.  The stencils in the network regime are appended to the
+// launch's sample buffers (one counter reservation per side and wavefront), and the face flux is
+// stored with -0 in place of each such reconstruction: -0 + x == x for every x, so once the
+// network kernel has added the learned left reconstructions and then the learned right ones
+// (flux = flux + value, in that order), every flux holds exactly left_q + right_q as
+// face_flux_learned_cw forms it.  A sample's network arithmetic does not depend on the other
+// samples of its matrix call, so the results are bitwise those of the in-kernel queue.
+template <class S>
+__device__ __forceinline__ void face_flux_learned_emit(const S& st, real out[5], const Learn& L) {
+    const LearnEmit&         E     = L.e;
+    const int                lane  = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long       m[10];  // wavefront masks of the network samples per reconstruction
+    int                      n0 = 0, n1 = 0;
+    const real               fs = learned_face_scale(st);
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        // a compiler fence per variable: its stencils are read from LDS here and not kept in
+        // registers from an earlier read or for a later one (few live registers)
+        __asm__ __volatile__("" ::: "memory");
+        real half[2];
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const int r = 2 * q + side;
+            real      v[5], bs[3];
+            learned_stencil(st, r, v);
+            const real classic = learned_classic(v, side, bs);
+            const bool need = L.active && learned_regime(v, bs, q >= 2 ? fs : RL(0.0)) == 1;
+            m[r]            = __ballot(need);
+            half[side]      = need ? RL(-0.0) : classic;
+            if (side) n1 += __popcll(m[r]);
+            else n0 += __popcll(m[r]);
+        }
+        out[q] = half[0] + half[1];
+        pin(out[q]);  // formed here, not after the sample writes below
+    }
+    if (n0 + n1 == 0) return;  // wavefront-uniform
+    int t0 = 0, t1 = 0;
+    if (lane == 0) {
+        if (n0) t0 = (int)atomicAdd(E.cnt + (long)L.region * LS_STRIDE, (unsigned)n0);
+        if (n1)
+            t1 = (int)atomicAdd(E.cnt + (long)(LS_REGIONS + L.region) * LS_STRIDE, (unsigned)n1);
+    }
+    long b0 = (long)L.region * E.capr + (unsigned)__shfl(t0, 0);
+    long b1 = (long)L.region * E.capr + (unsigned)__shfl(t1, 0);
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        __asm__ __volatile__("" ::: "memory");  // (as above: the stencil is read again)
+        if (m[r] == 0) continue;  // wavefront-uniform
+        const int side = r & 1;
+        if ((m[r] >> lane) & 1ull) {
+            real v[5];
+            learned_stencil(st, r, v);
+            const long s = (side ? b1 : b0) + __popcll(m[r] & below);
+            real*      V = E.v[side];
+#pragma unroll
+            for (int k = 0; k < 5; k++) V[k * E.cap + s] = v[k];
+            E.dest[side][s] = L.fc + (unsigned)(r >> 1) * L.fP;
+        }
+        if (side) b1 += __popcll(m[r]);
+        else b0 += __popcll(m[r]);
+    }
+}
+
+// the network pass of emit mode: every wavefront evaluates 64 samples of one side per matrix
+// call and adds each result to its flux.  A launch handles one side; each flux receives at most
+// one sample per side, so the launch writes disjoint fluxes.  Chunks of 64 samples are numbered
+// region-fastest (chunk c: region c % LS_REGIONS, local chunk c / LS_REGIONS), so the occupied
+// chunks come first; the wavefronts stride over them.
+__device__ __forceinline__ void learn_net_body(const MlpPacked* __restrict__ P,
+                                               const LearnEmit& E, int side,
+                                               flux_t* __restrict__ f) {
+    const int      l    = threadIdx.x & 63;
+    const long     gw   = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long     nw   = ((long)gridDim.x * blockDim.x) >> 6;
+    const unsigned* cnt = E.cnt + (long)side * LS_REGIONS * LS_STRIDE;
+    unsigned       mx   = (l < LS_REGIONS) ? cnt[(long)l * LS_STRIDE] : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned t = (unsigned)__shfl_xor((int)mx, o, 64);
+        mx               = t > mx ? t : mx;
+    }
+    const long      limit = (long)((mx + 63u) / 64u) * LS_REGIONS;
+    const real*     V     = E.v[side];
+    const unsigned* D     = E.dest[side];
+    for (long ch = gw; ch < limit; ch += nw) {
+        const int      r  = (int)(ch % LS_REGIONS);
+        const long     lc = ch / LS_REGIONS;
+        const unsigned n  = cnt[(long)r * LS_STRIDE];
+        if (lc * 64 >= (long)n) continue;  // wavefront-uniform
+        const bool has = lc * 64 + l < (long)n;
+        const long s   = (long)r * E.capr + lc * 64 + l;
+        real       v[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) v[k] = has ? V[k * E.cap + s] : RL(0.0);
+        const real classic = learned_classic(v, side);
+        float      x[8], xs[8], ratio, w[3];
+        mlp_features(v, x, &ratio);
+#pragma unroll
+        for (int k = 0; k < 8; k++) xs[k] = has ? x[k] : 0.0f;
+        mlp_weights_lane(P, xs, w);
+        if (has) {
+            const unsigned d = D[s];
+            f[d]             = f[d] + (flux_t)learned_finish(v, w, classic, ratio);
+        }
+    }
+}
+
 // component-wise face flux with the learned weights: out[q] = left_q + right_q as in face_flux,
 // each reconstruction as weno_learned computes it, the network samples batched per face
 template <class S>
@@ -309,20 +480,15 @@ __device__ __forceinline__ void face_flux_learned_cw(const S& st, real out[5], c
     int  slot[10];
     int  base = 0;  // queued samples (wavefront-uniform)
     const unsigned long long below = (1ull << (threadIdx.x & 63)) - 1ull;
+    const real               fs    = learned_face_scale(st);
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        const int qv = r >> 1, side = r & 1;
+        const int side = r & 1;
         real      v[5];
-        if (side == 0) {
-#pragma unroll
-            for (int k = 0; k < 5; k++) v[k] = st(k, qv);  // WENO5_left(fL of cells i-2 .. i+2)
-        } else {
-#pragma unroll
-            for (int k = 0; k < 5; k++) v[k] = st(5 - k, 5 + qv);  // WENO5_right mirrored
-        }
+        learned_stencil(st, r, v);
         real       bs[3];
         const real classic = learned_classic(v, side, bs);
-        const bool need    = L.active && learned_regime(v, bs) == 1;
+        const bool need    = L.active && learned_regime(v, bs, r >= 4 ? fs : RL(0.0)) == 1;
         const unsigned long long m = __ballot(need);
         const int cnt = __popcll(m);
         if (base + cnt > 64) {  // wavefront-uniform
@@ -356,14 +522,17 @@ __device__ __forceinline__ void face_flux_learned_cw(const S& st, real out[5], c
 }
 
 // weights WT: 0 Shu, 1 Borges, 2 learned
+// scale: the learned weights' constant-stencil scale (learned_regime)
 template <int WT>
-__device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e, const Learn& L) {
-    if constexpr (WT == 2) return weno_learned<0>(a, b, c, d, e, L);
+__device__ __forceinline__ real weno_left(real a, real b, real c, real d, real e, const Learn& L,
+                                          real scale = RL(0.0)) {
+    if constexpr (WT == 2) return weno_learned<0>(a, b, c, d, e, L, nullptr, scale);
     else return weno_left_c<WT == 1>(a, b, c, d, e);
 }
 template <int WT>
-__device__ __forceinline__ real weno_right(real a, real b, real c, real d, real e, const Learn& L) {
-    if constexpr (WT == 2) return weno_learned<1>(a, b, c, d, e, L);
+__device__ __forceinline__ real weno_right(real a, real b, real c, real d, real e, const Learn& L,
+                                           real scale = RL(0.0)) {
+    if constexpr (WT == 2) return weno_learned<1>(a, b, c, d, e, L, nullptr, scale);
     else return weno_right_c<WT == 1>(a, b, c, d, e);
 }
 
@@ -642,7 +811,8 @@ __device__ __forceinline__ void roe_state(const real l[5], const real r[5], real
 
 // OPT bit 0: per-face alpha (st holds F, U and the eigenvalue per cell); bit 1: corrected
 // characteristic projection
-template <int DIR, int CONV, int WT, int OPT, class S>
+// EMIT: the learned component-wise flux in emit mode (face_flux_learned_emit)
+template <int DIR, int CONV, int WT, int OPT, bool EMIT = false, class S>
 __device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const real Wr[5],
                                           real gamma, real out[5], const Learn& L) {
     if constexpr (OPT & 1) {
@@ -652,18 +822,24 @@ __device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const 
     } else {
     const S& st = raw;
     constexpr bool CORR = (OPT & 2) != 0;
-    if constexpr (CONV == 1) {
+    if constexpr (CONV == 1 && WT == 2 && EMIT) {
+        face_flux_learned_emit(st, out, L);
+    } else if constexpr (CONV == 1) {
         if constexpr (WT == 2) {
             if (L.q) {
                 face_flux_learned_cw(st, out, L);
                 return;
             }
         }
+        real fs = RL(0.0);
+        if constexpr (WT == 2) fs = learned_face_scale(st);
 #pragma unroll
-        for (int q = 0; q < 5; q++)
-            out[q] = weno_left<WT>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q), L) +
+        for (int q = 0; q < 5; q++) {
+            const real sc = q >= 2 ? fs : RL(0.0);
+            out[q] = weno_left<WT>(st(0, q), st(1, q), st(2, q), st(3, q), st(4, q), L, sc) +
                      weno_right<WT>(st(1, 5 + q), st(2, 5 + q), st(3, 5 + q), st(4, 5 + q),
-                                    st(5, 5 + q), L);
+                                    st(5, 5 + q), L, sc);
+        }
     } else {
         real Wa[5];
         if constexpr (CORR) {
@@ -688,10 +864,6 @@ __device__ __forceinline__ void face_flux(const S& raw, const real Wl[5], const 
     }
     }
 }
-
-// an empty instruction that reads and may rewrite x: x's load is complete here and cannot be
-// moved below this point (the compiler otherwise sinks loads towards their first use)
-template <class T> __device__ __forceinline__ void pin(T& x) { asm volatile("" : "+v"(x)); }
 
 template <int NV>
 struct RegSplit {  // split values held in registers: s[ii][v]
@@ -721,6 +893,12 @@ __device__ __forceinline__ void stage_cell(const double* __restrict__ U,
 // of the faces runs from NGC-1 to NGC+n-1; the tile kernels cover the first n faces, the plane
 // kernel the last one (or, for operator-level use, any plane range).
 // ------------------------------------------------------------------------------------------
+// region of the sample buffers (LearnSink) of this workgroup: its linear id modulo LS_REGIONS
+__device__ __forceinline__ int learn_region() {
+    const unsigned id = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    return (int)(id % LS_REGIONS);
+}
+
 constexpr int TILE_S = 8;    // x/y tiles: faces along the sweep per block (one wave each)
 constexpr int TILE_L = 64;   // lanes along k
 // sweep tile of the single-block x/y kernels: 16 faces (1024-thread blocks) in single precision,
@@ -739,6 +917,13 @@ constexpr int TILE_S1 = NS_TILE_S1;
 #ifndef NS_TILE1_WAVES
 #define NS_TILE1_WAVES NS_TILE_WAVES  // the same for the single-block x/y tile kernel
 #endif
+#ifndef NS_LEARN_WAVES
+#define NS_LEARN_WAVES 0  // the same for the single-block tiles with learned weights (0: as above)
+#endif
+// waves per SIMD requested for a single-block tile kernel with weights WT (base: the classic value)
+template <int WT> constexpr int tile_waves(int base) {
+    return (WT == 2 && NS_LEARN_WAVES > 0) ? NS_LEARN_WAVES : base;
+}
 
 // x or y sweep: 8 waves at 8 consecutive sweep positions; a wave covers kw consecutive k (unit
 // stride) of 64/kw consecutive rows of the other tangential index (kw = 64: one row).  The split
@@ -746,14 +931,16 @@ constexpr int TILE_S1 = NS_TILE_S1;
 // row tile, sweep tile).  [lo, hi): for the x sweep the faces of the launch (face f between cells
 // f - 1 and f; hi <= nx), for the y sweep its planes i (a slab of the block); the full range is
 // [0, n) of the sweep resp. of the other index.
-template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S>
+// EMIT (learned weights, component-wise): the network samples go to *le (emit mode)
+template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S, bool EMIT = false>
 __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
                                                   const double* __restrict__ W,
                                                   flux_t* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
                                                   const MlpPacked* __restrict__ mlp, int lo,
-                                                  int hi, real* lq = nullptr) {
+                                                  int hi, real* lq = nullptr,
+                                                  const LearnEmit* le = nullptr) {
     constexpr int NC = TS + 5;
     constexpr int NV = split_nv<OPT>();
     const int lane = threadIdx.x, w = threadIdx.y;
@@ -796,15 +983,21 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     }
     LdsSplit<TILE_L, NV> st{&tile[(w * NV) * TILE_L + lane]};
     real               out[5];
-    face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
-                                  Learn{mlp, valid, lq ? lq + w * LQ_REALS : nullptr});
+    if constexpr (EMIT) {
+        face_flux<DIR, CONV, WT, OPT, true>(st, Wl, Wr, gamma, out,
+                                            Learn{mlp, valid, nullptr, *le, (unsigned)c,
+                                                  (unsigned)g.P, learn_region()});
+    } else {
+        face_flux<DIR, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
+                                      Learn{mlp, valid, lq ? lq + w * LQ_REALS : nullptr});
+    }
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
 template <int DIR, int CONV, int WT, int OPT>
-__global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy(
+__global__ __launch_bounds__(64 * TILE_S1, tile_waves<WT>(NS_TILE1_WAVES)) void k_flux_tile_xy(
     const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, const MlpPacked* mlp, int lo, int hi) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
@@ -819,14 +1012,15 @@ __global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy(
 // (bx, by, bz) = (segment, row tile, i).
 // the largest per-wave need (kw = 16: 4 rows of 21 cells)
 template <int OPT> constexpr int zlds_wave_max() { return 4 * (16 + 5) * split_nv<OPT>(); }
-template <int CONV, int WT, int OPT>
+template <int CONV, int WT, int OPT, bool EMIT = false>
 __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
                                                  const double* __restrict__ W,
                                                  flux_t* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
                                                  real* __restrict__ lds,
                                                  const MlpPacked* __restrict__ mlp,
-                                                 real* lq = nullptr) {
+                                                 real* lq = nullptr,
+                                                 const LearnEmit* le = nullptr) {
     constexpr int NV = split_nv<OPT>();
     const int NC   = kw + 5;
     const int sub  = TILE_L / kw;                   // rows per wave
@@ -867,8 +1061,14 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
         __device__ real operator()(int ii, int v) const { return b[v * nc + ii]; }
     } st{&T[kk], NC};
     real out[5];
-    face_flux<2, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
-                                Learn{mlp, valid, lq ? lq + r * LQ_REALS : nullptr});
+    if constexpr (EMIT) {
+        face_flux<2, CONV, WT, OPT, true>(st, Wl, Wr, gamma, out,
+                                          Learn{mlp, valid, nullptr, *le, (unsigned)c,
+                                                (unsigned)g.P, learn_region()});
+    } else {
+        face_flux<2, CONV, WT, OPT>(st, Wl, Wr, gamma, out,
+                                    Learn{mlp, valid, lq ? lq + r * LQ_REALS : nullptr});
+    }
     if (!valid) return;
 #pragma unroll
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
@@ -876,7 +1076,7 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
 
 // planes i = lo + blockIdx.z
 template <int CONV, int WT, int OPT>
-__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double* __restrict__ U,
+__global__ __launch_bounds__(512, tile_waves<WT>(NS_TILE_WAVES)) void k_flux_tile_z(const double* __restrict__ U,
                                                      const double* __restrict__ W,
                                                      flux_t* __restrict__ f, Geo g,
                                                      real gamma, const MlpPacked* mlp, int lo) {
@@ -884,6 +1084,36 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z(const double
     __shared__ real lq[WT == 2 ? 8 * LQ_REALS : 1];
     flux_tile_z_body<CONV, WT, OPT>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
                                     TILE_L, tile, mlp, WT == 2 ? lq : nullptr);
+}
+
+// emit-mode tiles of the learned component-wise sweeps (no network queue in LDS) and the
+// network pass over one side of their samples
+template <int DIR>
+__global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy_emit(
+    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
+    real gamma, int lo, int hi, LearnEmit e) {
+    __shared__ real tile[(TILE_S1 + 5) * split_nv<0>() * TILE_L];
+    flux_tile_xy_body<DIR, 1, 2, 0, TILE_S1, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
+                                                   blockIdx.z, TILE_L, tile, nullptr, lo, hi,
+                                                   nullptr, &e);
+}
+__global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z_emit(
+    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
+    real gamma, int lo, LearnEmit e) {
+    __shared__ real tile[8 * (TILE_L + 5) * split_nv<0>()];
+    flux_tile_z_body<1, 2, 0, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
+                                    TILE_L, tile, nullptr, nullptr, &e);
+}
+__global__ void k_learn_reset(unsigned* cnt) {
+    const int t = threadIdx.x;
+    if (t < 2 * LS_REGIONS) cnt[(long)t * LS_STRIDE] = 0u;
+}
+#ifndef NS_NET_WAVES
+#define NS_NET_WAVES 5  // minimum waves per SIMD requested for the network pass (95 VGPRs)
+#endif
+__global__ __launch_bounds__(256, NS_NET_WAVES) void k_learn_net(const MlpPacked* __restrict__ P, LearnEmit e,
+                                                   int side, flux_t* __restrict__ f) {
+    learn_net_body(P, e, side, f);
 }
 
 // one plane of faces (sweep index `i`), one thread per face, split values in registers;
@@ -2802,24 +3032,56 @@ static void l_c2p_faces(const double* U, double* W, Geo g, Phys ph, hipStream_t 
 // [lo, hi): for the x sweep the faces f of the launch (f between cells f-1 and f, 0 <= f <= nx:
 // the tile kernel takes f < nx, the last plane f = nx), for the y and z sweeps the planes i (both
 // kernels); the whole block is [0, nx + 1) resp. [0, nx).  lo is a multiple of 4 for y and z.
+// samples per side of the emit-mode buffers a launch of nb workgroups of fb faces needs: every
+// region holds up to 5 samples (one per variable) of each face of its workgroups
+static long learn_capr(long nb, long fb) { return (nb + LS_REGIONS - 1) / LS_REGIONS * fb * 5; }
+
+// the tile launch of a learned component-wise sweep in emit mode: reset the counters of the
+// sweep's sample set, the tiles, then the network pass over the left and then the right samples
+template <int DIR>
+static bool flux_emit(const double* U, const double* W, flux_t* f, Geo g, double gamma,
+                      const MlpPacked* mlp, const LearnSink* sink, dim3 grd, dim3 blk, int lo,
+                      int hi, hipStream_t s) {
+    const long nb   = (long)grd.x * grd.y * grd.z;
+    const long capr = learn_capr(nb, (long)blk.x * blk.y);
+    if (!sink || capr * LS_REGIONS > sink->cap || 4 * g.P + (long)g.NX * g.sx > 0xffffffffL)
+        return false;
+    const int set = (DIR == 2) ? 1 : 0;
+    LearnEmit e{{(real*)sink->v[set][0], (real*)sink->v[set][1]},
+                {sink->dest[set][0], sink->dest[set][1]}, sink->cnt[set], sink->cap, capr};
+    k_learn_reset<<<1, 2 * LS_REGIONS, 0, s>>>(sink->cnt[set]);
+    if constexpr (DIR == 2) k_flux_tile_z_emit<<<grd, blk, 0, s>>>(U, W, f, g, gamma, lo, e);
+    else k_flux_tile_xy_emit<DIR><<<grd, blk, 0, s>>>(U, W, f, g, gamma, lo, hi, e);
+    // enough wavefronts to cover the occupied chunks of a typical launch several times over;
+    // each strides over the rest
+    const long chunks = (capr * LS_REGIONS + 63) / 64;
+    const long nblk   = std::min<long>(2048, std::max<long>(1, (chunks + 3) / 4));
+    k_learn_net<<<(unsigned)nblk, 256, 0, s>>>(mlp, e, 0, f);
+    k_learn_net<<<(unsigned)nblk, 256, 0, s>>>(mlp, e, 1, f);
+    return true;
+}
+
 template <int DIR, int CONV, int B, int OPT>
 static void flux_dir(const double* U, const double* W, flux_t* f, Geo g, double gamma,
-                     const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s) {
+                     const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s,
+                     const LearnSink* sink = nullptr) {
+    constexpr bool EMIT = (CONV == 1 && B == 2 && OPT == 0);
     if (!(parts & 1)) {
     } else if constexpr (DIR == 2) {
         dim3 blk(TILE_L, 8, 1);
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, (g.ny + 7) / 8, hi - lo);
-        if (hi > lo) k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo);
+        if (hi > lo && !(EMIT && flux_emit<DIR>(U, W, f, g, gamma, mlp, sink, grd, blk, lo, hi, s)))
+            k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo);
     } else if constexpr (DIR == 0) {
         const int fe = hi < g.nx ? hi : g.nx;  // faces of the tile kernel: [lo, fe)
         dim3      blk(TILE_L, TILE_S1, 1);
         dim3      grd((g.nz + TILE_L - 1) / TILE_L, g.ny, (fe - lo + TILE_S1 - 1) / TILE_S1);
-        if (fe > lo)
+        if (fe > lo && !(EMIT && flux_emit<DIR>(U, W, f, g, gamma, mlp, sink, grd, blk, lo, fe, s)))
             k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, fe);
     } else {
         dim3 blk(TILE_L, TILE_S1, 1);
         dim3 grd((g.nz + TILE_L - 1) / TILE_L, hi - lo, (g.ny + TILE_S1 - 1) / TILE_S1);
-        if (hi > lo)
+        if (hi > lo && !(EMIT && flux_emit<DIR>(U, W, f, g, gamma, mlp, sink, grd, blk, lo, hi, s)))
             k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, hi);
     }
     if (!(parts & 2)) return;
@@ -2848,9 +3110,10 @@ static int flux_opt(const Phys& ph) {
 
 template <int DIR, int CONV, int B>
 static void flux_dir_opt(const double* U, const double* W, flux_t* f, Geo g, double gamma,
-                         const MlpPacked* m, int opt, int parts, int lo, int hi, hipStream_t s) {
+                         const MlpPacked* m, int opt, int parts, int lo, int hi, hipStream_t s,
+                         const LearnSink* sink = nullptr) {
     if constexpr (B == 2) {
-        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s, sink);
     } else if constexpr (CONV == 1) {
         if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, parts, lo, hi, s);
         else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
@@ -2869,7 +3132,8 @@ static void face_flux_dir(const double* U, const double* W, flux_t* f, Geo g, Ph
     const int        opt = flux_opt(ph);
     if (ph.conv == 1) {
         if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        else if (ph.weights == 2)
+            flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, ph.sink);
         else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
     } else {
         if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
@@ -3169,6 +3433,17 @@ static void l_mlp_recon(const MlpPacked* P, long n, const double* S, int side, d
     k_mlp_recon<<<(unsigned)((n + 63) / 64), 64, 0, s>>>(P, n, S, side, out, regime);
 }
 
+// samples per side of the emit-mode buffers that the full-block sweeps of g need, and the bytes
+// of one sample value (the build's real type)
+static long l_learn_cap(Geo g, int* value_bytes) {
+    if (value_bytes) *value_bytes = (int)sizeof(real);
+    const long kt = (g.nz + TILE_L - 1) / TILE_L;
+    const long fx = learn_capr(kt * g.ny * ((g.nx + TILE_S1 - 1) / TILE_S1), 64L * TILE_S1);
+    const long fy = learn_capr(kt * g.nx * ((g.ny + TILE_S1 - 1) / TILE_S1), 64L * TILE_S1);
+    const long fz = learn_capr(kt * ((g.ny + 7) / 8) * g.nx, 512L);
+    return std::max(fx, std::max(fy, fz)) * LS_REGIONS;
+}
+
 const Launchers& launchers() {
     static const Launchers L = {l_reset_min,    l_step_begin, l_halo,       l_c2p_faces,
                                 l_face_flux,    l_deriv1,     l_deriv_halo, l_residual,
@@ -3178,7 +3453,7 @@ const Launchers& launchers() {
                                 l_residual_fd,  NS_FUSE_DIFF,
                                 // plane ranges of the cell kernels start on a march boundary
                                 (MARCH > FUSED_MARCH ? MARCH : FUSED_MARCH),
-                                l_zfd_kw,       l_residual_zfd};
+                                l_zfd_kw,       l_residual_zfd, l_learn_cap};
     return L;
 }
 

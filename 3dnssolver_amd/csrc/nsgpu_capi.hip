@@ -86,6 +86,9 @@ struct nsgpu_ctx {
                                           // linked to its neighbours; stepped by its domain only
     float*             mlp_params = nullptr;  // learned weights: FP32 parameters
     MlpPacked*         mlp_packed = nullptr;  // and their packed BF16 fragments
+    LearnSink          sink{};                // sample buffers of the learned component-wise
+    void*              sink_mem   = nullptr;  // sweeps (emit mode), allocated at the first
+    bool               sink_tried = false;    // single-block learned sweep (ensure_sink)
     bool               timing = false;
     hipEvent_t         ev[2 * T_COUNT] = {};
     double             ms[T_COUNT]     = {};
@@ -222,6 +225,41 @@ int health_status(unsigned fl) {
 // the learned weights need their network
 bool net_ready(const nsgpu_ctx* c) { return c->ph.weights != NSGPU_WEIGHTS_LEARNED || c->ph.mlp; }
 
+// Sample buffers of the learned component-wise sweeps (LearnSink): the face tiles then leave the
+// network to a separate dense kernel instead of evaluating it themselves.  Sized for the worst
+// case (every reconstruction of every face in the network regime), allocated once at the first
+// single-block learned sweep of the context, and only up to SINK_MAX_BYTES; blocks of a
+// decomposition (stepped inside captured graphs) and contexts without them keep the in-kernel
+// network.  Costs at 256^3: 8.1 GB with the FP32 build, 14.8 GB in FP64.
+constexpr size_t SINK_MAX_BYTES = 32ull << 30;
+void ensure_sink(nsgpu_ctx* c) {
+    if (c->sink_tried || c->ph.weights != NSGPU_WEIGHTS_LEARNED || c->ph.conv != 1 || c->linked)
+        return;
+    c->sink_tried = true;
+    int        vb  = 8;
+    const long cap = c->L->learn_cap(c->g, &vb);
+    const size_t vbytes = ((size_t)cap * 5 * vb + 255) / 256 * 256;
+    const size_t dbytes = ((size_t)cap * 4 + 255) / 256 * 256;
+    const size_t cbytes = (size_t)2 * LS_REGIONS * LS_STRIDE * sizeof(unsigned);
+    const size_t total  = 4 * (vbytes + dbytes) + 2 * cbytes;
+    if (total > SINK_MAX_BYTES) return;
+    char* m = nullptr;
+    if (hipMalloc((void**)&m, total) != hipSuccess) {
+        (void)hipGetLastError();  // the in-kernel network stays in use
+        return;
+    }
+    c->sink_mem = m;
+    for (int set = 0; set < 2; set++) {
+        for (int side = 0; side < 2; side++) {
+            c->sink.v[set][side] = m, m += vbytes;
+            c->sink.dest[set][side] = (unsigned*)m, m += dbytes;
+        }
+        c->sink.cnt[set] = (unsigned*)m, m += cbytes;
+    }
+    c->sink.cap = cap;
+    c->ph.sink  = &c->sink;
+}
+
 // the whole block as a plane range of the launchers: faces [0, nx + 1) of the x sweep, planes
 // [0, nx) of the others
 inline int full_hi(const nsgpu_ctx* c, int dir) { return dir == 0 ? c->g.nx + 1 : c->g.nx; }
@@ -230,6 +268,7 @@ inline int full_hi(const nsgpu_ctx* c, int dir) { return dir == 0 ? c->g.nx + 1 
 void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts = 3,
                        hipStream_t st = nullptr, bool skip_z = false) {
     if (!c->ph.conv) return;
+    ensure_sink(c);
     if (!st) st = c->stream;
     {
         Scope s(c, T_FLUX_X);
@@ -276,6 +315,7 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
 // context's stream waits for both before the residual.  Timed runs keep everything on one
 // stream so that each kernel family is timed alone.
 void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
+    ensure_sink(c);
     if (c->timing || !c->aux) {
         convection_fluxes(c, U, W, 3, nullptr, zfd(c));
         diffusion_derivs(c, W);
@@ -332,6 +372,7 @@ bool pipelined(const nsgpu_ctx* c) {
 // sweeps write faces and derivative planes of later slabs, the residual writes the stage output.
 // The context's stream waits for the last residual before the stage ends.
 void stage_pipelined(nsgpu_ctx* c, const double* U, const double* W, const StageArgs& a) {
+    ensure_sink(c);
     const Launchers& L = *c->L;
     const Geo&       g = c->g;
     for (int sl = 0; sl < c->nslab; sl++) {
@@ -515,7 +556,7 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     for (double* p : arrays) free_array(p);
     double* bufs[] = {c->partial, (double*)c->dtmin, (double*)c->flags, (double*)c->sflags,
                       c->dtime, c->sums,
-                      (double*)c->mlp_params, (double*)c->mlp_packed};
+                      (double*)c->mlp_params, (double*)c->mlp_packed, (double*)c->sink_mem};
     for (double* p : bufs)
         if (p) hipFree(p);
     for (int t = 0; t < 2 * T_COUNT; t++)

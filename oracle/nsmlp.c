@@ -69,7 +69,7 @@ static void betas(const double v[5], double b[3]) {
     b[2] = (13. / 12.) * t1 * t1 + (1. / 4.) * t2 * t2;
 }
 
-void nso_mlp_features(const double v[5], double x[8]) {
+void nso_mlp_features_s(const double v[5], double scale, double x[8]) {
     double m = v[0], M = v[0];
     for (int k = 1; k < 5; k++) {
         m = v[k] < m ? v[k] : m;
@@ -77,9 +77,22 @@ void nso_mlp_features(const double v[5], double x[8]) {
     }
     double r    = M - m;
     double am   = fabs(m), aM = fabs(M);
-    int    vary = r > NSO_MLP_RANGE * (am > aM ? am : aM);
+    double mg   = am > aM ? am : aM;
+    mg          = scale > mg ? scale : mg;
+    int    vary = r > NSO_MLP_RANGE * mg;
     for (int k = 0; k < 5; k++) x[k] = vary ? (v[k] - m) / r : 0.0;
     betas(x, x + 5);
+}
+
+void nso_mlp_features(const double v[5], double x[8]) { nso_mlp_features_s(v, 0.0, x); }
+
+double nso_mlp_face_scale(const double fl[5][5], const double fr[5][5]) {
+    double s = 0.0;
+    for (int p = 2; p < 5; p++) {
+        const double a[4] = {fl[p][2], fl[p][3], fr[p][1], fr[p][2]};
+        for (int t = 0; t < 4; t++) s = fabs(a[t]) > s ? fabs(a[t]) : s;
+    }
+    return s;
 }
 
 double nso_mlp_ratio(const double x[8]) {
@@ -216,11 +229,11 @@ void nso_set_mlp(const double* params, int bf16) {
 /* learned reconstruction of a left- (side 0) or right-biased (side 1) stencil; the right-biased
  * stencil is fed to the network mirrored (WENO5_right(a..e) is WENO5_left(e..a) with the weight
  * order reversed) and the classic Shu reconstruction is used on smooth and flagged faces */
-double nso_weno5_learned_p(const double* params, int bf16, const double f[5], int side,
-                           int* regime) {
+static double learned_ps(const double* params, int bf16, const double f[5], int side,
+                         double scale, int* regime) {
     double v[5], x[8];
     for (int k = 0; k < 5; k++) v[k] = side ? f[4 - k] : f[k];
-    nso_mlp_features(v, x);
+    nso_mlp_features_s(v, scale, x);
     int rg = nso_mlp_regime(x);
     if (regime) *regime = rg;
     double out;
@@ -241,11 +254,20 @@ double nso_weno5_learned_p(const double* params, int bf16, const double f[5], in
     return classic + nso_mlp_blend(nso_mlp_ratio(x)) * (net - classic);
 }
 
+double nso_weno5_learned_p(const double* params, int bf16, const double f[5], int side,
+                           int* regime) {
+    return learned_ps(params, bf16, f, side, 0.0, regime);
+}
+
 void nso_weno5_learned(const double* params, int bf16, const double f[5], int side, double out[1],
                        int* regime) {
     out[0] = nso_weno5_learned_p(params, bf16, f, side, regime);
 }
 
 double nso_weno5_learned_g(const double f[5], int side) {
-    return nso_weno5_learned_p(g_params, g_bf16, f, side, 0);
+    return learned_ps(g_params, g_bf16, f, side, 0.0, 0);
+}
+
+double nso_weno5_learned_gs(const double f[5], int side, double scale) {
+    return learned_ps(g_params, g_bf16, f, side, scale, 0);
 }

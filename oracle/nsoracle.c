@@ -271,11 +271,13 @@ static void weno_weights(double fm2, double fm1, double f0, double fp1, double f
     w[2]     = a2 / a;
 }
 
-/* Weno5::WENO5_left, weno-5.cpp:305-322 */
-static double weno_left(double a, double b, double c, double d, double e, int type) {
+/* Weno5::WENO5_left, weno-5.cpp:305-322; scale: the learned weights' constant-stencil scale
+ * (nso_mlp_features_s) */
+static double weno_left_s(double a, double b, double c, double d, double e, int type,
+                          double scale) {
     if (type == NSO_W_LEARNED) {
         const double f[5] = {a, b, c, d, e};
-        return nso_weno5_learned_g(f, 0);
+        return nso_weno5_learned_gs(f, 0, scale);
     }
     double f0 = (2. * a - 7. * b + 11. * c) * onepersix;
     double f1 = (-b + 5. * c + 2. * d) * onepersix;
@@ -286,10 +288,11 @@ static double weno_left(double a, double b, double c, double d, double e, int ty
 }
 
 /* Weno5::WENO5_right, weno-5.cpp:285-303 */
-static double weno_right(double a, double b, double c, double d, double e, int type) {
+static double weno_right_s(double a, double b, double c, double d, double e, int type,
+                           double scale) {
     if (type == NSO_W_LEARNED) {
         const double f[5] = {a, b, c, d, e};
-        return nso_weno5_learned_g(f, 1);
+        return nso_weno5_learned_gs(f, 1, scale);
     }
     double f0 = (-a + 5. * b + 2. * c) * onepersix;
     double f1 = (2. * b + 5. * c - d) * onepersix;
@@ -297,6 +300,13 @@ static double weno_right(double a, double b, double c, double d, double e, int t
     double w[3];
     weno_weights(a, b, c, d, e, d_right, type, w);
     return w[0] * f0 + w[1] * f1 + w[2] * f2;
+}
+
+static double weno_left(double a, double b, double c, double d, double e, int type) {
+    return weno_left_s(a, b, c, d, e, type, 0.0);
+}
+static double weno_right(double a, double b, double c, double d, double e, int type) {
+    return weno_right_s(a, b, c, d, e, type, 0.0);
 }
 
 void nso_weno5_left(const double f[5], int weights, double out[1]) {
@@ -552,16 +562,23 @@ static void sweep(const nso_cfg* c, int dir, int charwise, const double* F, cons
                 size_t base = ((((size_t)i) * P.NY + j) * P.NZ + k) * NQ;
                 const double alpha = face_alpha(c, W, base, s, dir);
                 if (!charwise) {
-                    for (int q = 0; q < NQ; q++) {
-                        double fl[5], fr[5];
+                    double fl[NQ][5], fr[NQ][5];
+                    for (int q = 0; q < NQ; q++)
                         for (int ii = 0; ii < 5; ii++) {
-                            size_t a = base + (size_t)(ii - 2) * s + q; /* idx = i-2+ii */
-                            size_t b = a + s;                           /* idx + 1 */
-                            fl[ii]   = (F[a] + alpha * U[a]) * 0.5;
-                            fr[ii]   = (F[b] - alpha * U[b]) * 0.5;
+                            size_t a  = base + (size_t)(ii - 2) * s + q; /* idx = i-2+ii */
+                            size_t b  = a + s;                           /* idx + 1 */
+                            fl[q][ii] = (F[a] + alpha * U[a]) * 0.5;
+                            fr[q][ii] = (F[b] - alpha * U[b]) * 0.5;
                         }
-                        double left  = weno_left(fl[0], fl[1], fl[2], fl[3], fl[4], c->weights);
-                        double right = weno_right(fr[0], fr[1], fr[2], fr[3], fr[4], c->weights);
+                    /* learned weights: momentum stencils are also measured against the face's
+                     * momentum scale (no-op for the classic weights) */
+                    const double fs = nso_mlp_face_scale(fl, fr);
+                    for (int q = 0; q < NQ; q++) {
+                        const double* l  = fl[q];
+                        const double* r  = fr[q];
+                        const double  sc = q >= 2 ? fs : 0.0;
+                        double left  = weno_left_s(l[0], l[1], l[2], l[3], l[4], c->weights, sc);
+                        double right = weno_right_s(r[0], r[1], r[2], r[3], r[4], c->weights, sc);
                         f[base + q]  = left + right;
                     }
                 } else if (c->physics & NSO_PHYS_CONVECTION) {

@@ -110,13 +110,21 @@ double nso_mu_sutherland(double mu0, double C, double gamma, double rho, double 
 #define NSO_MLP_EPS 1e-6
 #define NSO_MLP_SMOOTH 3.0
 #define NSO_MLP_FLAG 1e3
-/* a stencil whose range is at most this share of its largest magnitude counts as constant */
+/* a stencil whose range is at most this share of its largest magnitude counts as constant; a
+ * momentum stencil of a component-wise face also when its range is at most this share of the
+ * face's momentum scale (nso_mlp_face_scale) */
 #define NSO_MLP_RANGE (1.0 / 8192.0)
 enum { NSO_REGIME_SMOOTH = 0, NSO_REGIME_MLP = 1, NSO_REGIME_FLAGGED = 2 };
 
 double nso_bf16(double v);
 /* x[0..4] = (v - min) / (max - min) (0 when max == min), x[5..7] = beta_0..2 of x */
 void   nso_mlp_features(const double v[5], double x[8]);
+/* the same with a scale (>= 0) the range is also measured against */
+void   nso_mlp_features_s(const double v[5], double scale, double x[8]);
+/* momentum scale of a component-wise face: the largest |split value| of the momentum
+ * components (stored positions 2..4) at the face's two cells, fl[p][2], fl[p][3] (fL of cells i,
+ * i+1) and fr[p][1], fr[p][2] (fR of cells i, i+1) */
+double nso_mlp_face_scale(const double fl[5][5], const double fr[5][5]);
 int    nso_mlp_regime(const double x[8]);
 /* (max beta + EPS) / (min beta + EPS) and the network's share of the reconstruction at it:
  * min(1, (r - SMOOTH) / SMOOTH) * min(1, (FLAG - r) / (0.75 FLAG)), clipped at 0 */
@@ -135,6 +143,8 @@ void   nso_weno5_learned(const double* params, int bf16, const double f[5], int 
 /* network used by nso_rhs / nso_step when cfg.weights == NSO_W_LEARNED */
 void   nso_set_mlp(const double* params, int bf16);
 double nso_weno5_learned_g(const double f[5], int side);
+/* the same with the constant-stencil scale of nso_mlp_features_s */
+double nso_weno5_learned_gs(const double f[5], int side, double scale);
 
 #ifdef __cplusplus
 }

@@ -207,6 +207,21 @@ class Oracle(_Lib):
         self.lib.nso_mlp_features(_ptr(v), _ptr(x))
         return x
 
+    def mlp_features_s(self, v, scale):
+        """mlp_features with the range also measured against `scale` (the momentum scale of a
+        component-wise face)"""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        x = np.zeros(8)
+        self.lib.nso_mlp_features_s(_ptr(v), ctypes.c_double(float(scale)), _ptr(x))
+        return x
+
+    def mlp_face_scale(self, fl, fr):
+        """momentum scale of a component-wise face from its split stencils fl, fr [5][5]"""
+        f = self.lib.nso_mlp_face_scale
+        f.restype = ctypes.c_double
+        return f(_ptr(np.ascontiguousarray(fl, dtype=np.float64)),
+                 _ptr(np.ascontiguousarray(fr, dtype=np.float64)))
+
     def mlp_regime(self, x):
         return self.lib.nso_mlp_regime(_ptr(np.ascontiguousarray(x, dtype=np.float64)))
 

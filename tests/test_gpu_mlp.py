@@ -134,6 +134,38 @@ def test_learned_rhs_matches_oracle(ns, oracle, conv):
     shu.close()
 
 
+@pytest.mark.parametrize("conv", ["weno5"])
+def test_noise_level_momentum_stays_classic(ns, oracle, conv):
+    # a spanwise velocity at round-off level (1e-12 of the flow's): every stencil of the rho w
+    # split fluxes is constant against the face's momentum scale, so the learned RHS of rho w is
+    # the classic one bitwise (EXACT math), as in the oracle, while the other components still
+    # take the network
+    # (a plane block: in a z sweep the rho w flux carries the pressure and is no noise)
+    print(f"[gpu] learned rhs, noise-level w ({conv})", flush=True)
+    C = cases_module()
+    case = dict(C.base_case(12, 10, 1), dim=2, conv=conv, weights="learned", diff_order=4)
+    W = C.random_smooth_W(case, seed=7)
+    W[..., 4] = 1e-12 * np.random.default_rng(5).standard_normal(W.shape[:3])
+    params = random_params(35)
+    U = oracle.p2c(case, W)
+    s = ns.Solver(case, math="exact")
+    s.set_mlp(params)
+    _, _, R = s.rhs(U, 1)
+    shu = ns.Solver(dict(case, weights="shu"), math="exact")
+    _, _, Rshu = shu.rhs(U, 1)
+    oracle.set_mlp(np.asarray(params, dtype=np.float32).astype(np.float64), bf16=True)
+    _, _, Rref = oracle.rhs(case, U, 1)
+    a, b, c = po.interior(R), po.interior(Rref), po.interior(Rshu)
+    np.testing.assert_array_equal(a[..., 4], c[..., 4])
+    np.testing.assert_array_equal(b[..., 4], c[..., 4])
+    assert np.abs(a[..., 2] - c[..., 2]).max() > 1e-6 * np.abs(c[..., 2]).max()
+    for q in range(4):
+        den = np.abs(b[..., q]).max()
+        assert np.abs(a[..., q] - b[..., q]).max() <= 2e-2 * den, q
+    s.close()
+    shu.close()
+
+
 @pytest.mark.parametrize("math", ["exact", "fp32"])
 def test_learned_ensemble_matches_solo(ns, math):
     print(f"[gpu] learned ensemble {math}", flush=True)

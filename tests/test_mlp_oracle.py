@@ -220,3 +220,38 @@ def test_roundoff_level_stencils_stay_classic(oracle):
     x = oracle.mlp_features(v)
     assert oracle.mlp_regime(x) == po.REGIMES["mlp"]
     np.testing.assert_allclose(x[:5], shape, atol=1e-9)
+
+
+def test_momentum_scale_holds_noise_stencils_constant(oracle):
+    # a momentum component that vanishes in exact arithmetic holds only round-off: its stencils,
+    # measured against the face's momentum scale, count as constant whatever shape the noise
+    # has; measured against their own magnitude alone they would look like structure
+    rng = np.random.default_rng(67)
+    shape = np.array([0.0, 0.1, 0.3, 0.35, 1.0])  # a network-regime shape
+    for mag in (1e-12, 1e-9, 1e-6):
+        v = mag * shape
+        assert oracle.mlp_regime(oracle.mlp_features(v)) == po.REGIMES["mlp"]
+        x = oracle.mlp_features_s(v, 1.0)
+        assert not x.any() and oracle.mlp_regime(x) == po.REGIMES["smooth"]
+    # the scale only ever adds the constant case: below the stencil's own magnitude it changes
+    # nothing, and a range above 2^-13 of the scale keeps the stencil's own features
+    for _ in range(200):
+        v = rng.standard_normal(5) * 10.0 ** rng.uniform(-3, 3)
+        own = np.abs(v).max()
+        np.testing.assert_array_equal(oracle.mlp_features_s(v, own * rng.uniform(0, 1)),
+                                      oracle.mlp_features(v))
+        r = v.max() - v.min()
+        x = oracle.mlp_features_s(v, r * 8192.0 * 0.999)
+        np.testing.assert_array_equal(x, oracle.mlp_features(v))
+        assert not oracle.mlp_features_s(v, r * 8192.0 * 1.001).any()
+    # the face scale: the largest |split value| of the momentum components (positions 2..4) at
+    # the face's two cells (fl cells 2, 3 and fr cells 1, 2 of the stencils)
+    fl = rng.standard_normal((5, 5))
+    fr = rng.standard_normal((5, 5))
+    want = max(np.abs(fl[2:, 2:4]).max(), np.abs(fr[2:, 1:3]).max())
+    assert oracle.mlp_face_scale(fl, fr) == want
+    fl2, fr2 = fl.copy(), fr.copy()
+    fl2[:2] = 1e6  # mass and energy do not enter
+    fl2[2:, [0, 1, 4]] = 1e6  # nor the outer stencil cells
+    fr2[2:, [0, 3, 4]] = 1e6
+    assert oracle.mlp_face_scale(fl2, fr2) == want
