@@ -236,6 +236,8 @@ void ensure_sink(nsgpu_ctx* c) {
     if (c->sink_tried || c->ph.weights != NSGPU_WEIGHTS_LEARNED || c->ph.conv != 1 || c->linked)
         return;
     c->sink_tried = true;
+    if (const char* e = std::getenv("NSGPU_LEARN_SINK"))  // 0: no buffers (in-kernel network)
+        if (std::atoi(e) == 0) return;
     int        vb  = 8;
     const long cap = c->L->learn_cap(c->g, &vb);
     const size_t vbytes = ((size_t)cap * 5 * vb + 255) / 256 * 256;

@@ -167,6 +167,28 @@ def test_noise_level_momentum_stays_classic(ns, oracle, conv):
 
 
 @pytest.mark.parametrize("math", ["exact", "fp32"])
+def test_network_pass_equals_in_kernel_network(ns, math, monkeypatch):
+    # a single block's learned component-wise sweeps hand their network stencils to a separate
+    # network pass; with NSGPU_LEARN_SINK=0 the face kernels evaluate the network themselves.
+    # Both must give the same steps bitwise
+    print(f"[gpu] network pass vs in-kernel network ({math})", flush=True)
+    C = cases_module()
+    case = dict(C.base_case(40, 24, 36), conv="weno5", weights="learned", diff_order=4)
+    W = C.random_smooth_W(case, seed=9)
+    params = random_params(38)
+    out = {}
+    for sink in ("0", "1"):
+        monkeypatch.setenv("NSGPU_LEARN_SINK", sink)
+        s = ns.Solver(case, math=math)
+        s.set_mlp(params)
+        s.set_state(W, "primitive")
+        s.step(3)
+        out[sink] = s.get_state("primitive")
+        s.close()
+    np.testing.assert_array_equal(po.interior(out["0"]), po.interior(out["1"]))
+
+
+@pytest.mark.parametrize("math", ["exact", "fp32"])
 def test_learned_ensemble_matches_solo(ns, math):
     print(f"[gpu] learned ensemble {math}", flush=True)
     C = cases_module()
