@@ -253,6 +253,10 @@ struct Launchers {
     // samples per side and set of the LearnSink the full-block sweeps of g need, and the bytes of
     // one sample value in this build
     long (*learn_cap)(Geo g, int* value_bytes);
+    // halo as above on the faces f0 .. f0 + nf - 1 only (W, E, N, S, F, B = 0 .. 5): the faces
+    // are disjoint and read interior cells only, so they may be filled in separate launches
+    void (*halo_faces)(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0, int nf,
+                       hipStream_t s);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

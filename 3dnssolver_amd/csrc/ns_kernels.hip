@@ -1365,8 +1365,9 @@ __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, 
     for (int q = 0; q < nB; q++) B[cd + q * g.P] = sB[cs + q * sP];
 }
 
-__global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b) {
-    halo_body(A, nA, B, nB, g, b, blockIdx.x, blockIdx.y, blockIdx.z);
+// faces f0 .. f0 + gridDim.z - 1
+__global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0) {
+    halo_body(A, nA, B, nB, g, b, blockIdx.x, blockIdx.y, f0 + (int)blockIdx.z);
 }
 
 // W = c2p(U) on the interior and the face halos (edges and corners are never read)
@@ -3011,13 +3012,18 @@ static void l_step_begin(const double* W, double* Ua, double* Un, double* Wst,
     k_step_begin<<<(unsigned)nb, 256, 0, s>>>(W, Ua, Un, Wst, dtmin, g, ph.gamma);
 }
 
-static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s) {
+static void l_halo_faces(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0, int nf,
+                         hipStream_t s) {
     // x, y faces: (k, tangential row, layer); z faces: ((j, layer), i)
     int  m1 = g.ny > g.nx ? g.ny : g.nx;
     int  mx = g.nz > NGC * g.ny ? g.nz : NGC * g.ny;
     dim3 blk(64, 4, 1);
-    dim3 grd((mx + 63) / 64, NGC * ((m1 + 3) / 4), 6);
-    k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b);
+    dim3 grd((mx + 63) / 64, NGC * ((m1 + 3) / 4), nf);
+    if (nf > 0) k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b, f0);
+}
+
+static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s) {
+    l_halo_faces(A, nA, B, nB, g, b, 0, 6, s);
 }
 
 static void l_p2c_faces(const double* W, double* U, Geo g, Phys ph, hipStream_t s) {
@@ -3453,7 +3459,8 @@ const Launchers& launchers() {
                                 l_residual_fd,  NS_FUSE_DIFF,
                                 // plane ranges of the cell kernels start on a march boundary
                                 (MARCH > FUSED_MARCH ? MARCH : FUSED_MARCH),
-                                l_zfd_kw,       l_residual_zfd, l_learn_cap};
+                                l_zfd_kw,       l_residual_zfd, l_learn_cap,
+                                l_halo_faces};
     return L;
 }
 

@@ -316,9 +316,12 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
 // them (a z tile block fits beside an x/y tile block on one CU: 44 + 66.5 KB of LDS); the
 // context's stream waits for both before the residual.  Timed runs keep everything on one
 // stream so that each kernel family is timed alone.
-void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
+// zsplit: the z-face halos of U and W are still to be filled; they go on the auxiliary stream,
+// ahead of the z sweep (only it and the work after the join read them)
+void stage_inputs(nsgpu_ctx* c, const double* U, const double* W, bool zsplit = false) {
     ensure_sink(c);
     if (c->timing || !c->aux) {
+        if (zsplit) c->L->halo_faces((double*)U, 5, (double*)W, 5, c->g, c->b, 4, 2, c->stream);
         convection_fluxes(c, U, W, 3, nullptr, zfd(c));
         diffusion_derivs(c, W);
         return;
@@ -326,6 +329,7 @@ void stage_inputs(nsgpu_ctx* c, const double* U, const double* W) {
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->aux, c->ev_fork, 0);
     const Geo& g = c->g;
+    if (zsplit) c->L->halo_faces((double*)U, 5, (double*)W, 5, g, c->b, 4, 2, c->aux);
     if (c->ph.conv && g.dim == 3 && !zfd(c))
         c->L->face_flux(2, U, W, c->F[2], g, c->ph, 3, 0, full_hi(c, 2), c->aux);
     if (c->ph.conv) {
@@ -726,10 +730,15 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
         double* wcur = c->Ws[0];
         double* wnxt = c->Ws[1];
         for (int st = 0; st < 3; st++) {
+            // the z faces of the halos go beside the x/y tiles (stage_inputs), which read only
+            // x and y halos
+            const bool zsplit = c->aux && !c->timing && !pipelined(c) && c->ph.conv &&
+                                c->g.dim == 3 && !zfd(c);
             {
                 // halos of U and of its primitives: the same copies (c2p acts per cell)
                 Scope sc(c, T_HALO);
-                L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
+                if (zsplit) L.halo_faces(cur, 5, wcur, 5, c->g, c->b, 0, 4, c->stream);
+                else L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
             }
             // the stage writes U and c2p(U) of the next stage; the last stage writes the
             // primitive state into the other state buffer (the residual reads wcur meanwhile)
@@ -739,7 +748,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
                 hipStreamWaitEvent(c->aux, c->ev_fork, 0);
                 stage_pipelined(c, cur, wcur, a);
             } else {
-                stage_inputs(c, cur, wcur);
+                stage_inputs(c, cur, wcur, zsplit);
                 Scope sc(c, T_RESIDUAL);
                 residual(c, wcur, nullptr, nullptr, &a, &c->dtmin[0], NSGPU_RHS_BOTH);
             }
