@@ -105,6 +105,13 @@ struct StageArgs {
     int           stage;  // 0, 1, 2
     unsigned*     flags;  // health of the stage output (may be null): bit 0 a non-finite value,
                           // bit 1 rho <= 0 or p <= 0; bits are only ever set (atomic OR)
+    // the next step's prologue, formed by the last stage of a step that another follows (null:
+    // none): U = p2c(Wout) into Unext, c2p(U) into Wnext, the CFL minimum of Wout into dtnext
+    // (atomic); Uout is then not written (the next step starts from Wout, as the reference's
+    // does, time_scheme.cpp:15-27)
+    double*             Unext = nullptr;
+    double*             Wnext = nullptr;
+    unsigned long long* dtnext = nullptr;
 };
 
 // ------------------------------------------------------------------------------------------

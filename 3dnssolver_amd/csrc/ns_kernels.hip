@@ -18,6 +18,7 @@
 //
 // Citations are paths relative to the reference's src/ directory.
 #include <algorithm>
+#include <type_traits>
 
 #include "ns_common.h"
 #include "ns_mlp.h"
@@ -1246,6 +1247,34 @@ __device__ __forceinline__ void block_min_atomic(double v, unsigned long long* o
     }
 }
 
+// the same from the calling lanes of one wavefront, without a barrier (the lanes of a workgroup
+// need not all be present): a butterfly when every lane takes part, else one atomic per lane
+__device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* out) {
+    if (__ballot(1) == ~0ull) {
+        for (int o = 32; o > 0; o >>= 1) {
+            double t = __shfl_xor(v, o, 64);
+            v        = (t < v) ? t : v;
+        }
+        if ((threadIdx.x & 63) == 0 && v < 1E300)
+            atomicMin(out, (unsigned long long)__double_as_longlong(v));
+    } else if (v < 1E300) {
+        atomicMin(out, (unsigned long long)__double_as_longlong(v));
+    }
+}
+
+// the next step's prologue of one cell from this step's final primitives w (step_begin_body's
+// arithmetic): U and c2p(U) stored at cell cc, the cell's CFL limit folded into dtl
+__device__ __forceinline__ void next_prologue(const StageArgs& st, const double w[5], long cc,
+                                              long P, const Geo& g, double gamma, double& dtl) {
+    const double d = local_dt(w, gamma, g.dx, g.dy, g.dz);
+    dtl            = (d < dtl) ? d : dtl;
+    double u[5], w2[5];
+    p2c(w, gamma, u);
+    c2p(u, gamma, w2);
+#pragma unroll
+    for (int q = 0; q < 5; q++) st.Unext[q * P + cc] = u[q], st.Wnext[q * P + cc] = w2[q];
+}
+
 __global__ void k_reset_min(unsigned long long* bits) {
     *bits = (unsigned long long)__double_as_longlong(1E10);  // max_dt = 1E10 (time_scheme.cpp:104)
 }
@@ -1959,7 +1988,8 @@ template <int ORDER, bool SUTH> constexpr int ring_vars() { return SUTH ? 7 : 6;
 template <int ORDER, bool SUTH> constexpr int ring_doubles() {
     return ring_vars<ORDER, SUTH>() * (ORDER + 1) * 256;
 }
-template <int ORDER, bool SQ, bool SUTH, bool LW>
+// NX: this stage also forms the next step's prologue (st.Unext, next_prologue)
+template <int ORDER, bool SQ, bool SUTH, bool LW, bool NX = false>
 __device__ __forceinline__ void residual_fd_stage_march(
     const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const StageArgs& st,
@@ -1972,6 +2002,7 @@ __device__ __forceinline__ void residual_fd_stage_march(
     const unsigned sx = (unsigned)g.sx;
     const int      sy = g.sy;
     const int      tid = threadIdx.x + threadIdx.y * 64;
+    double dtl = 1E300;  // the next step's CFL minimum over this thread's cells (st.Unext)
     const double* __restrict__ Un = st.Un;
     const double* __restrict__ Uc = st.Ucur;
     double* __restrict__       Uo = st.Uout;
@@ -2155,13 +2186,15 @@ __device__ __forceinline__ void residual_fd_stage_march(
 #else
             un[q] = unv[q] * m1 + ucv[q] * m2 + r * m3;
 #endif
-            Uo[q * P + cc] = un[q];
+            if constexpr (!NX) Uo[q * P + cc] = un[q];
         }
         c2p(un, ph.gamma, w);
 #pragma unroll
         for (int q = 0; q < 5; q++) Wo[q * P + cc] = w[q];
         if (st.flags) stage_health(st.flags, un, w);
+        if constexpr (NX) next_prologue(st, w, cc, P, g, ph.gamma, dtl);
     }
+    if constexpr (NX) wave_min_atomic(dtl, st.dtnext);
 }
 
 // 256 threads: kw lanes along k times 256/kw rows j; (bx, by, bz) = (k tile, j tile, chunk of
@@ -2172,7 +2205,7 @@ __device__ __forceinline__ void residual_fd_stage_march(
 // from fz, where the plane launch of that face stored it.  The same face_flux on the same split
 // values as the z sweep: the residual is bitwise the one the sweep's faces give.  Every thread
 // of the block reaches the two barriers of each plane; threads without a cell skip the work.
-template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false>
+template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false, bool NX = false>
 __device__ __forceinline__ void residual_fd_body(
     const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
@@ -2239,6 +2272,7 @@ __device__ __forceinline__ void residual_fd_body(
     const bool    rowz = j < g.ny;
     const int     mz   = (g.nz - bx * kw < kw) ? g.nz - bx * kw : kw;
     double dt = 0, m1 = 0, m2 = 0, m3 = 0;
+    double dtl = 1E300;  // the next step's CFL minimum over this thread's cells (st.Unext)
     if (st.Uout) {
         dt = __longlong_as_double((long long)*dtmin) * ph.cfl;
         if (st.stage == 0) {
@@ -2250,7 +2284,7 @@ __device__ __forceinline__ void residual_fd_body(
         }
     }
     if (ZC == 0 && st.Uout && !Rin && !Rout && conv && g.dim == 3 && (xper || i1 + H <= g.nx)) {
-        residual_fd_stage_march<ORDER, SQ, SUTH, LW>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1,
+        residual_fd_stage_march<ORDER, SQ, SUTH, LW, NX>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1,
                                                      wd, dyx, fxm, m1, m2, m3, ring);
         return;
     }
@@ -2407,16 +2441,18 @@ __device__ __forceinline__ void residual_fd_body(
 #else
             un[q] = (st.Un + q * P)[cc] * m1 + (st.Ucur + q * P)[cc] * m2 + r * m3;
 #endif
-            (st.Uout + q * P)[cc] = un[q];
+            if constexpr (!NX) (st.Uout + q * P)[cc] = un[q];
         }
         c2p(un, ph.gamma, w);
 #pragma unroll
         for (int q = 0; q < 5; q++) (st.Wout + q * P)[cc] = w[q];
         if (st.flags) stage_health(st.flags, un, w);
+        if constexpr (NX) next_prologue(st, w, cc, P, g, ph.gamma, dtl);
     }
+    if constexpr (NX) wave_min_atomic(dtl, st.dtnext);
 }
 
-template <int ORDER, bool SQ, bool SUTH>
+template <int ORDER, bool SQ, bool SUTH, bool NX = false>
 __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_residual_fd(
     const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
@@ -2426,7 +2462,7 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
     __shared__ double ring[NS_RING ? ring_doubles<ORDER, SUTH>() : 1];
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    residual_fd_body<ORDER, SQ, SUTH, 0, NS_RING != 0>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin,
+    residual_fd_body<ORDER, SQ, SUTH, 0, NS_RING != 0, NX>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin,
                                                        which, g, ph, bnd, bx, by, z0 + bz,
                                                        KW_CELL, FUSED_MARCH, nullptr, nullptr,
                                                        ring);
@@ -3223,17 +3259,23 @@ static void residual_fd_order(const flux_t* fx, const flux_t* fy, const flux_t* 
     const dim3 grd((g.nz + KW_CELL - 1) / KW_CELL, (g.ny + 256 / KW_CELL - 1) / (256 / KW_CELL),
                    (hi - lo + FUSED_MARCH - 1) / FUSED_MARCH);
     const int  z0 = lo / FUSED_MARCH;
-    if (ph.suth) {
-        if (g.square_xy)
-            k_residual_fd<ORDER, true, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
-        else
-            k_residual_fd<ORDER, false, true><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
-    } else {
-        if (g.square_xy)
-            k_residual_fd<ORDER, true, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
-        else
-            k_residual_fd<ORDER, false, false><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
-    }
+    auto go = [&](auto nx) {
+        constexpr bool NX = decltype(nx)::value;
+        if (ph.suth) {
+            if (g.square_xy)
+                k_residual_fd<ORDER, true, true, NX><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
+            else
+                k_residual_fd<ORDER, false, true, NX><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
+        } else {
+            if (g.square_xy)
+                k_residual_fd<ORDER, true, false, NX><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
+            else
+                k_residual_fd<ORDER, false, false, NX><<<grd, blk, 0, s>>>(fx, fy, fz, W, Dx, Rin, Rout, a, dtmin, which, g, ph, b, z0);
+        }
+    };
+    // the last stage of a step that another follows forms the next step's prologue
+    if (a.Unext && a.Uout) go(std::true_type{});
+    else go(std::false_type{});
 }
 
 static void l_residual_fd(const double* fx, const double* fy, const double* fz, const double* W,

@@ -512,7 +512,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     if (!rc) rc = alloc_array(&c->stage_aos, five, c->stream);
     c->partial_max = 4096;
     if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max, c->stream);
-    if (!rc) rc = alloc((double**)&c->dtmin, 2 * sizeof(unsigned long long), c->stream);
+    if (!rc) rc = alloc((double**)&c->dtmin, 3 * sizeof(unsigned long long), c->stream);
     if (!rc) rc = alloc((double**)&c->flags, sizeof(double), c->stream);
     if (!rc) rc = alloc((double**)&c->sflags, sizeof(double), c->stream);
     if (!rc) rc = alloc(&c->dtime, sizeof(double), c->stream);
@@ -717,18 +717,33 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
     if (time_io) HIPCHK(hipMemcpyAsync(c->dtime, time_io, sizeof(double), hipMemcpyHostToDevice, c->stream));
     const Launchers& L = *c->L;
     L.zero_flags(c->sflags, c->stream);
+    // the last stage of a step that another follows forms the next step's prologue (dt and U
+    // from its primitives, the stage-1 primitives c2p(U)) in place of the prologue kernel: the
+    // same arithmetic per cell, on values the stage holds already
+    const bool fuse = fused_diffusion(c) && !zfd(c) && !pipelined(c) && !c->timing;
+    bool       pro  = false;     // this step's prologue was formed by the previous step
+    double*    wfirst = c->Ws[0];  // the stage-1 primitives of this step
     for (int s = 0; s < nsteps; s++) {
+        const bool nx = fuse && s + 1 < nsteps;
         {
             Scope sc(c, T_STEP_BEGIN);
-            L.reset_min(&c->dtmin[0], c->stream);
-            L.step_begin(c->W, nullptr, c->Un, c->Ws[0], &c->dtmin[0], c->g, c->ph, c->stream);
+            if (pro) {
+                HIPCHK(hipMemcpyAsync(&c->dtmin[0], &c->dtmin[2], sizeof(unsigned long long),
+                                      hipMemcpyDeviceToDevice, c->stream));
+            } else {
+                wfirst = c->Ws[0];
+                L.reset_min(&c->dtmin[0], c->stream);
+                L.step_begin(c->W, nullptr, c->Un, wfirst, &c->dtmin[0], c->g, c->ph, c->stream);
+            }
             if (fixed_dt > 0.0) L.set_dt(&c->dtmin[0], fixed_dt, c->stream);
         }
-        // stage inputs Un -> Ua -> Ub: Un (Uprev of time_scheme.cpp:27) is never overwritten
+        // stage inputs Un -> Ua -> Ub: Un (Uprev of time_scheme.cpp:27) is read by every stage
+        // and overwritten (cell by cell, after its read) only by a last stage that forms the
+        // next step's prologue
         double* cur  = c->Un;
         double* nxt  = c->Ua;
-        double* wcur = c->Ws[0];
-        double* wnxt = c->Ws[1];
+        double* wcur = wfirst;
+        double* wnxt = (wfirst == c->Ws[0]) ? c->Ws[1] : c->Ws[0];
         for (int st = 0; st < 3; st++) {
             // the z faces of the halos go beside the x/y tiles (stage_inputs), which read only
             // x and y halos
@@ -743,6 +758,13 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             // the stage writes U and c2p(U) of the next stage; the last stage writes the
             // primitive state into the other state buffer (the residual reads wcur meanwhile)
             StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st, c->sflags};
+            if (st == 2 && nx) {  // wnxt is free in the last stage
+                L.reset_min(&c->dtmin[2], c->stream);
+                a.Unext  = c->Un;
+                a.Wnext  = wnxt;
+                a.dtnext = &c->dtmin[2];
+                wfirst   = wnxt;
+            }
             if (pipelined(c)) {
                 hipEventRecord(c->ev_fork, c->stream);  // orders aux after the halo fill
                 hipStreamWaitEvent(c->aux, c->ev_fork, 0);
@@ -759,11 +781,12 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             wcur      = wnxt;
             wnxt      = t;
         }
-        c->Ulast = cur;
+        c->Ulast = cur;  // (the last step of a call forms no prologue: Ulast is its U)
         double* t = c->W;
         c->W      = c->W2;
         c->W2     = t;
         L.time_accum(c->dtime, &c->dtmin[0], c->ph.cfl, c->stream);
+        pro = nx;
     }
     HIPCHK(hipGetLastError());
     if (nsteps > 0) c->dt_scale = c->ph.cfl;
