@@ -112,6 +112,20 @@ struct StageArgs {
     double*             Unext = nullptr;
     double*             Wnext = nullptr;
     unsigned long long* dtnext = nullptr;
+    // single-precision build: float copies of what the next stage's face sweeps read (Uout and
+    // Wout, or Unext and Wnext), interior cells; required for every stage of that build's
+    // single-block stage kernel
+    float* Uf = nullptr;
+    float* Wf = nullptr;
+};
+
+// float copies of a stage's U and primitives W (single-precision build), halos included, in the
+// element layout of the double arrays (P, sy, sx); the face kernels of that build convert every
+// loaded value to float, so they read these in place of the double arrays (same fluxes, half the
+// bytes)
+struct StateCopy {
+    float* U;
+    float* W;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -191,15 +205,17 @@ struct Launchers {
     void (*reset_min)(unsigned long long* bits, hipStream_t s);
     // dt from W, U = p2c(W) -> Ua and Un, Wst = c2p(U) (interior)
     void (*step_begin)(const double* W, double* Ua, double* Un, double* Wst,
-                       unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s);
+                       unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s,
+                       const StateCopy* cp);  // cp: float copies of Un and Wst (or null)
     // boundary conditions + periodic exchange on nA planes of A and nB planes of B (B may be null)
     void (*halo)(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s);
     void (*c2p_faces)(const double* U, double* W, Geo g, Phys ph, hipStream_t s);
     // parts: 1 the tile kernel (all faces but the last plane), 2 the last face plane, 3 both;
     // [lo, hi): the faces of the x sweep (the whole block: [0, nx + 1)), the planes i of the y
     // and z sweeps (the whole block: [0, nx); lo a multiple of 4)
+    // cp (single-precision build, or null): float copies of U and W that the kernels read instead
     void (*face_flux)(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                      int parts, int lo, int hi, hipStream_t s);
+                      int parts, int lo, int hi, hipStream_t s, const StateCopy* cp);
     // planes i in [lo, hi) (lo a multiple of the march length, 16).  Dy null: dW/dx of u and v
     // (Dx planes 2, 3) only (for residual_fd)
     void (*deriv1)(const double* W, double* Dx, double* Dy, Geo g, Phys ph, int lo, int hi,
@@ -261,9 +277,10 @@ struct Launchers {
     // one sample value in this build
     long (*learn_cap)(Geo g, int* value_bytes);
     // halo as above on the faces f0 .. f0 + nf - 1 only (W, E, N, S, F, B = 0 .. 5): the faces
-    // are disjoint and read interior cells only, so they may be filled in separate launches
+    // are disjoint and read interior cells only, so they may be filled in separate launches.
+    // cp (or null): the same faces of the float copies of A and B (5 planes each)
     void (*halo_faces)(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0, int nf,
-                       hipStream_t s);
+                       hipStream_t s, const StateCopy* cp);
 };
 // packs FP32 parameters (device) into the lane fragments of the network (ns_mlp.hip)
 void mlp_pack(const float* params, MlpPacked* P, hipStream_t s);

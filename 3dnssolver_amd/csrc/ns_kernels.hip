@@ -31,16 +31,21 @@
 #endif
 // arithmetic type of the reconstruction and the viscous fluxes: double for the EXACT and FAST
 // builds; float for the FP32 build (state, stage accumulation and reductions stay double)
-#ifndef NS_REAL
-#define NS_REAL double
+#ifndef NS_FP32
+#define NS_FP32 0
+#endif
+#ifndef NS_STATE_COPY
+#define NS_STATE_COPY 1  // the single-precision stage kernel writes the float copies of its output
 #endif
 #if NS_FP32
 #define NS_MODE fp32
 #define NS_REAL float
 #elif NS_FAST
 #define NS_MODE fast
+#define NS_REAL double
 #else
 #define NS_MODE exact
+#define NS_REAL double
 #endif
 
 namespace nsgpu {
@@ -235,6 +240,40 @@ __device__ __forceinline__ real learned_classic(const real v[5], int side, real*
                 : weno_left_c<false>(v[0], v[1], v[2], v[3], v[4], bo);
 }
 
+// min, max and largest magnitude of three values.  Single precision: one v_min3_f32 /
+// v_max3_f32 each (the magnitudes through the operands' abs modifiers).  For operands that are
+// not NaN they return the values the comparison chains of double precision return, up to the
+// sign of a zero result, which no regime decision below depends on (a zero range, magnitude or
+// indicator compares the same with either sign).
+__device__ __forceinline__ float min3r(float a, float b, float c) {
+    float r;
+    __asm__("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float max3r(float a, float b, float c) {
+    float r;
+    __asm__("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float maxabs3r(float a, float b, float c) {  // max(|a|, |b|, c)
+    float r;
+    __asm__("v_max3_f32 %0, |%1|, |%2|, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ double min3r(double a, double b, double c) {
+    const double t = b < a ? b : a;
+    return c < t ? c : t;
+}
+__device__ __forceinline__ double max3r(double a, double b, double c) {
+    const double t = b > a ? b : a;
+    return c > t ? c : t;
+}
+__device__ __forceinline__ double maxabs3r(double a, double b, double c) {
+    const double x = a < 0.0 ? -a : a, y = b < 0.0 ? -b : b;
+    const double t = x > y ? x : y;
+    return c > t ? c : t;
+}
+
 // regime of the stencil v (0 smooth, 1 network, 2 flagged discontinuity) from the smoothness
 // indicators b of its classic reconstruction: the indicators of the range-normalised stencil
 // (v - min) / r are b / r^2, so their ratio (max + EPS) / (min + EPS) is (max b + EPS r^2) /
@@ -245,17 +284,13 @@ __device__ __forceinline__ real learned_classic(const real v[5], int side, real*
 // against (the face's momentum scale for a momentum component, learned_face_scale)
 __device__ __forceinline__ int learned_regime(const real v[5], const real b[3],
                                               real scale = RL(0.0)) {
-    real m = v[0], M = v[0];
-#pragma unroll
-    for (int k = 1; k < 5; k++) m = v[k] < m ? v[k] : m, M = v[k] > M ? v[k] : M;
+    const real m  = min3r(min3r(v[0], v[1], v[2]), v[3], v[4]);
+    const real M  = max3r(max3r(v[0], v[1], v[2]), v[3], v[4]);
     const real r  = M - m;
-    const real am = (m < RL(0)) ? -m : m, aM = (M < RL(0)) ? -M : M;
-    real       mg = am > aM ? am : aM;
-    mg            = scale > mg ? scale : mg;
+    const real mg = maxabs3r(m, M, scale);
     if (!(r > (real)MLP_RANGE * mg)) return 0;
-    const real e  = (real)MLP_EPS * (r * r);
-    real       lo = b[0] < b[1] ? b[0] : b[1], hi = b[0] < b[1] ? b[1] : b[0];
-    lo = b[2] < lo ? b[2] : lo, hi = b[2] > hi ? b[2] : hi;
+    const real e   = (real)MLP_EPS * (r * r);
+    const real lo  = min3r(b[0], b[1], b[2]), hi = max3r(b[0], b[1], b[2]);
     const real num = hi + e, den = lo + e;
     return num < (real)MLP_SMOOTH * den ? 0 : (num > (real)MLP_FLAG * den ? 2 : 1);
 }
@@ -340,12 +375,7 @@ __device__ __forceinline__ real learned_face_scale(const S& st) {
 #pragma unroll
     for (int q = 2; q < 5; q++) {
 #pragma unroll
-        for (int ii = 2; ii < 4; ii++) {
-            const real a = st(ii, q), b = st(ii, 5 + q);
-            const real x = a < RL(0) ? -a : a, y = b < RL(0) ? -b : b;
-            s = x > s ? x : s;
-            s = y > s ? y : s;
-        }
+        for (int ii = 2; ii < 4; ii++) s = maxabs3r(st(ii, q), st(ii, 5 + q), s);
     }
     return s;
 }
@@ -541,8 +571,8 @@ __device__ __forceinline__ real weno_right(real a, real b, real c, real d, real 
 // Euler flux and Lax-Friedrichs split (convection_flux.cpp:23-61, weno-5.cpp:94-110):
 // s[0..4] = fL = (F + alpha U)/2, s[5..9] = fR = (F - alpha U)/2 of one cell, direction DIR
 // ------------------------------------------------------------------------------------------
-template <int DIR>
-__device__ __forceinline__ void split_cell(const double* __restrict__ U, const double* __restrict__ W,
+template <int DIR, class TS>
+__device__ __forceinline__ void split_cell(const TS* __restrict__ U, const TS* __restrict__ W,
                                            long c, long P, real gamma, real s[10]) {
     real rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
     real qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;  // u*nx + v*ny + w*nz
@@ -570,9 +600,9 @@ __device__ __forceinline__ void split_cell(const double* __restrict__ U, const d
 // the per-face Lax-Friedrichs constant (alpha_mode LOCAL_MAX): the split depends on the face, so
 // a cell stores s[0..4] = F, s[5..9] = U and s[10] = its eigenvalue max(|q_n - c|, |q_n + c|)
 // (the value convection_flux.cpp:235-238 computes and leaves unused)
-template <int DIR>
-__device__ __forceinline__ void split_cell_la(const double* __restrict__ U,
-                                              const double* __restrict__ W, long c, long P,
+template <int DIR, class TS>
+__device__ __forceinline__ void split_cell_la(const TS* __restrict__ U,
+                                              const TS* __restrict__ W, long c, long P,
                                               real gamma, real s[11]) {
     real rho = W[c], p = W[c + P], u = W[c + 2 * P], v = W[c + 3 * P], w = W[c + 4 * P];
     real qn  = (DIR == 0) ? u : (DIR == 1) ? v : w;
@@ -881,9 +911,11 @@ struct LdsSplit {  // split values in LDS, layout [cell][v][lane]
 // values stored per cell: the split (10), or F, U and the eigenvalue (11) for a per-face alpha
 template <int OPT> constexpr int split_nv() { return (OPT & 1) ? 11 : 10; }
 
-template <int DIR, int OPT>
-__device__ __forceinline__ void stage_cell(const double* __restrict__ U,
-                                           const double* __restrict__ W, long c, long P,
+// TS: the type of the stored state the cell is read from (double, or the float copies of the
+// single-precision build: every value is converted to real on its load either way)
+template <int DIR, int OPT, class TS>
+__device__ __forceinline__ void stage_cell(const TS* __restrict__ U,
+                                           const TS* __restrict__ W, long c, long P,
                                            real gamma, real* sv) {
     if constexpr (OPT & 1) split_cell_la<DIR>(U, W, c, P, gamma, sv);
     else split_cell<DIR>(U, W, c, P, gamma, sv);
@@ -933,9 +965,10 @@ template <int WT> constexpr int tile_waves(int base) {
 // f - 1 and f; hi <= nx), for the y sweep its planes i (a slab of the block); the full range is
 // [0, n) of the sweep resp. of the other index.
 // EMIT (learned weights, component-wise): the network samples go to *le (emit mode)
-template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S, bool EMIT = false>
-__device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
-                                                  const double* __restrict__ W,
+template <int DIR, int CONV, int WT, int OPT, int TS = TILE_S, bool EMIT = false,
+          class ST = double>
+__device__ __forceinline__ void flux_tile_xy_body(const ST* __restrict__ U,
+                                                  const ST* __restrict__ W,
                                                   flux_t* __restrict__ f, const Geo& g,
                                                   real gamma, int bx, int by, int bz, int kw,
                                                   real* __restrict__ tile,
@@ -997,9 +1030,9 @@ __device__ __forceinline__ void flux_tile_xy_body(const double* __restrict__ U,
     for (int q = 0; q < 5; q++) f[c + q * g.P] = out[q];
 }
 
-template <int DIR, int CONV, int WT, int OPT>
+template <int DIR, int CONV, int WT, int OPT, class ST>
 __global__ __launch_bounds__(64 * TILE_S1, tile_waves<WT>(NS_TILE1_WAVES)) void k_flux_tile_xy(
-    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
+    const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, const MlpPacked* mlp, int lo, int hi) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
     __shared__ real lq[WT == 2 ? TILE_S1 * LQ_REALS : 1];  // network queues (learned weights)
@@ -1013,9 +1046,9 @@ __global__ __launch_bounds__(64 * TILE_S1, tile_waves<WT>(NS_TILE1_WAVES)) void 
 // (bx, by, bz) = (segment, row tile, i).
 // the largest per-wave need (kw = 16: 4 rows of 21 cells)
 template <int OPT> constexpr int zlds_wave_max() { return 4 * (16 + 5) * split_nv<OPT>(); }
-template <int CONV, int WT, int OPT, bool EMIT = false>
-__device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
-                                                 const double* __restrict__ W,
+template <int CONV, int WT, int OPT, bool EMIT = false, class ST = double>
+__device__ __forceinline__ void flux_tile_z_body(const ST* __restrict__ U,
+                                                 const ST* __restrict__ W,
                                                  flux_t* __restrict__ f, const Geo& g,
                                                  real gamma, int bx, int by, int bz, int kw,
                                                  real* __restrict__ lds,
@@ -1076,9 +1109,9 @@ __device__ __forceinline__ void flux_tile_z_body(const double* __restrict__ U,
 }
 
 // planes i = lo + blockIdx.z
-template <int CONV, int WT, int OPT>
-__global__ __launch_bounds__(512, tile_waves<WT>(NS_TILE_WAVES)) void k_flux_tile_z(const double* __restrict__ U,
-                                                     const double* __restrict__ W,
+template <int CONV, int WT, int OPT, class ST>
+__global__ __launch_bounds__(512, tile_waves<WT>(NS_TILE_WAVES)) void k_flux_tile_z(const ST* __restrict__ U,
+                                                     const ST* __restrict__ W,
                                                      flux_t* __restrict__ f, Geo g,
                                                      real gamma, const MlpPacked* mlp, int lo) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<OPT>()];
@@ -1089,17 +1122,18 @@ __global__ __launch_bounds__(512, tile_waves<WT>(NS_TILE_WAVES)) void k_flux_til
 
 // emit-mode tiles of the learned component-wise sweeps (no network queue in LDS) and the
 // network pass over one side of their samples
-template <int DIR>
+template <int DIR, class ST>
 __global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy_emit(
-    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
+    const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, int lo, int hi, LearnEmit e) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<0>() * TILE_L];
     flux_tile_xy_body<DIR, 1, 2, 0, TILE_S1, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
                                                    blockIdx.z, TILE_L, tile, nullptr, lo, hi,
                                                    nullptr, &e);
 }
+template <class ST>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z_emit(
-    const double* __restrict__ U, const double* __restrict__ W, flux_t* __restrict__ f, Geo g,
+    const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, int lo, LearnEmit e) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<0>()];
     flux_tile_z_body<1, 2, 0, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
@@ -1119,9 +1153,9 @@ __global__ __launch_bounds__(256, NS_NET_WAVES) void k_learn_net(const MlpPacked
 
 // one plane of faces (sweep index `i`), one thread per face, split values in registers;
 // block (64, 4) over the tangential indices, (bx, by) = (fast, slow) tile
-template <int DIR, int CONV, int WT, int OPT>
-__device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
-                                                const double* __restrict__ W,
+template <int DIR, int CONV, int WT, int OPT, class ST = double>
+__device__ __forceinline__ void flux_plane_body(const ST* __restrict__ U,
+                                                const ST* __restrict__ W,
                                                 flux_t* __restrict__ f, const Geo& g, real gamma,
                                                 int i, int bx, int by,
                                                 const MlpPacked* __restrict__ mlp,
@@ -1159,9 +1193,9 @@ __device__ __forceinline__ void flux_plane_body(const double* __restrict__ U,
 }
 
 // by0: first row tile (of 4 rows of the slow tangential index) of the launch
-template <int DIR, int CONV, int WT, int OPT>
-__global__ __launch_bounds__(256) void k_flux_plane(const double* __restrict__ U,
-                                                    const double* __restrict__ W,
+template <int DIR, int CONV, int WT, int OPT, class ST>
+__global__ __launch_bounds__(256) void k_flux_plane(const ST* __restrict__ U,
+                                                    const ST* __restrict__ W,
                                                     flux_t* __restrict__ f, Geo g, real gamma,
                                                     int i, const MlpPacked* mlp, int by0) {
     __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
@@ -1264,6 +1298,8 @@ __device__ __forceinline__ void wave_min_atomic(double v, unsigned long long* ou
 
 // the next step's prologue of one cell from this step's final primitives w (step_begin_body's
 // arithmetic): U and c2p(U) stored at cell cc, the cell's CFL limit folded into dtl
+// CP: also their float copies into st.Uf, st.Wf
+template <bool CP>
 __device__ __forceinline__ void next_prologue(const StageArgs& st, const double w[5], long cc,
                                               long P, const Geo& g, double gamma, double& dtl) {
     const double d = local_dt(w, gamma, g.dx, g.dy, g.dz);
@@ -1272,7 +1308,10 @@ __device__ __forceinline__ void next_prologue(const StageArgs& st, const double 
     p2c(w, gamma, u);
     c2p(u, gamma, w2);
 #pragma unroll
-    for (int q = 0; q < 5; q++) st.Unext[q * P + cc] = u[q], st.Wnext[q * P + cc] = w2[q];
+    for (int q = 0; q < 5; q++) {
+        st.Unext[q * P + cc] = u[q], st.Wnext[q * P + cc] = w2[q];
+        if constexpr (CP) st.Uf[q * P + cc] = (float)u[q], st.Wf[q * P + cc] = (float)w2[q];
+    }
 }
 
 __global__ void k_reset_min(unsigned long long* bits) {
@@ -1288,11 +1327,14 @@ __global__ void k_zero_flags(unsigned* flags) { *flags = 0u; }
 
 // step prologue: dt from W (time_scheme.cpp:16, :98-136), U = p2c(W) (:25), Uprev = U (:27),
 // and the stage-1 primitives c2p(U) (:36).  Workgroup `wg` of `nwg` strides over the interior.
+// cp (may be null): float copies of Un and Wst (single-precision build)
 __device__ __forceinline__ void step_begin_body(const double* __restrict__ W,
                                                 double* __restrict__ Ua, double* __restrict__ Un,
                                                 double* __restrict__ Wst,
                                                 unsigned long long* dtmin, const Geo& g,
-                                                double gamma, int wg, int nwg) {
+                                                double gamma, int wg, int nwg,
+                                                float* __restrict__ Uf = nullptr,
+                                                float* __restrict__ Wf = nullptr) {
     long   n  = (long)g.nx * g.ny * g.nz;
     double dt = 1E300;
     for (long t = threadIdx.x + (long)wg * blockDim.x; t < n; t += (long)nwg * blockDim.x) {
@@ -1313,6 +1355,7 @@ __device__ __forceinline__ void step_begin_body(const double* __restrict__ W,
             if (Ua) Ua[c + q * g.P] = u[q];
             Un[c + q * g.P] = u[q];
             Wst[c + q * g.P] = w2[q];
+            if (Uf) Uf[c + q * g.P] = (float)u[q], Wf[c + q * g.P] = (float)w2[q];
         }
     }
     block_min_atomic(dt, dtmin);
@@ -1322,8 +1365,9 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
                                                     double* __restrict__ Ua,
                                                     double* __restrict__ Un,
                                                     double* __restrict__ Wst,
-                                                    unsigned long long* dtmin, Geo g, double gamma) {
-    step_begin_body(W, Ua, Un, Wst, dtmin, g, gamma, blockIdx.x, gridDim.x);
+                                                    unsigned long long* dtmin, Geo g, double gamma,
+                                                    float* Uf, float* Wf) {
+    step_begin_body(W, Ua, Un, Wst, dtmin, g, gamma, blockIdx.x, gridDim.x, Uf, Wf);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1336,7 +1380,9 @@ __global__ __launch_bounds__(256) void k_step_begin(const double* __restrict__ W
 // A linked face (FACE_LINK) copies the neighbour's interior planes next to it (`lk`: its arrays and
 // extent along the normal; same tangential layout), as Communicator::communicate does between
 // ranks (communicator.cpp:16-144): faces only, like every other face.
-__device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, const Geo& g,
+// T: double, or float for the float copies of the single-precision build (no linked faces)
+template <class T>
+__device__ __forceinline__ void halo_body(T* A, int nA, T* B, int nB, const Geo& g,
                                           const Bnd& b, int bx, int by, int bz,
                                           const HaloLink* lk = nullptr) {
     int face = bz;
@@ -1367,13 +1413,17 @@ __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, 
     }
     int dst = low ? h : NGC + n + h;
     int src;
-    const double* sA = A;
-    const double* sB = B;
-    int           sx = g.sx, sy = g.sy;  // strides of the source
-    long          sP = g.P;
+    const T* sA = A;
+    const T* sB = B;
+    int      sx = g.sx, sy = g.sy;  // strides of the source
+    long     sP = g.P;
     if (link) {
-        src = low ? lk->n + h : NGC + h, sA = lk->A, sB = lk->B;
-        sx = lk->sx, sy = lk->sy, sP = lk->P;
+        if constexpr (std::is_same<T, double>::value) {
+            src = low ? lk->n + h : NGC + h, sA = lk->A, sB = lk->B;
+            sx = lk->sx, sy = lk->sy, sP = lk->P;
+        } else {
+            return;
+        }
     } else if (b.periodic[axis]) {
         src = low ? n + h : NGC + h;
     } else {
@@ -1397,6 +1447,12 @@ __device__ __forceinline__ void halo_body(double* A, int nA, double* B, int nB, 
 // faces f0 .. f0 + gridDim.z - 1
 __global__ void k_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0) {
     halo_body(A, nA, B, nB, g, b, blockIdx.x, blockIdx.y, f0 + (int)blockIdx.z);
+}
+// the same, and the same copies on the float copies Af, Bf (5 planes each) of A and B
+__global__ void k_halo_cp(double* A, int nA, double* B, int nB, float* Af, float* Bf, Geo g,
+                          Bnd b, int f0) {
+    halo_body(A, nA, B, nB, g, b, blockIdx.x, blockIdx.y, f0 + (int)blockIdx.z);
+    halo_body(Af, 5, Bf, 5, g, b, blockIdx.x, blockIdx.y, f0 + (int)blockIdx.z);
 }
 
 // W = c2p(U) on the interior and the face halos (edges and corners are never read)
@@ -1988,8 +2044,10 @@ template <int ORDER, bool SUTH> constexpr int ring_vars() { return SUTH ? 7 : 6;
 template <int ORDER, bool SUTH> constexpr int ring_doubles() {
     return ring_vars<ORDER, SUTH>() * (ORDER + 1) * 256;
 }
-// NX: this stage also forms the next step's prologue (st.Unext, next_prologue)
-template <int ORDER, bool SQ, bool SUTH, bool LW, bool NX = false>
+// NX: this stage also forms the next step's prologue (st.Unext, next_prologue).  CP: float
+// copies of what the next stage's face sweeps read (the stage output U and c2p(U), or with NX
+// the next step's U and stage-1 primitives) go to st.Uf and st.Wf
+template <int ORDER, bool SQ, bool SUTH, bool LW, bool NX = false, bool CP = false>
 __device__ __forceinline__ void residual_fd_stage_march(
     const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const StageArgs& st,
@@ -2187,12 +2245,16 @@ __device__ __forceinline__ void residual_fd_stage_march(
             un[q] = unv[q] * m1 + ucv[q] * m2 + r * m3;
 #endif
             if constexpr (!NX) Uo[q * P + cc] = un[q];
+            if constexpr (!NX && CP) st.Uf[q * P + cc] = (float)un[q];
         }
         c2p(un, ph.gamma, w);
 #pragma unroll
-        for (int q = 0; q < 5; q++) Wo[q * P + cc] = w[q];
+        for (int q = 0; q < 5; q++) {
+            Wo[q * P + cc] = w[q];
+            if constexpr (!NX && CP) st.Wf[q * P + cc] = (float)w[q];
+        }
         if (st.flags) stage_health(st.flags, un, w);
-        if constexpr (NX) next_prologue(st, w, cc, P, g, ph.gamma, dtl);
+        if constexpr (NX) next_prologue<CP>(st, w, cc, P, g, ph.gamma, dtl);
     }
     if constexpr (NX) wave_min_atomic(dtl, st.dtnext);
 }
@@ -2205,7 +2267,8 @@ __device__ __forceinline__ void residual_fd_stage_march(
 // from fz, where the plane launch of that face stored it.  The same face_flux on the same split
 // values as the z sweep: the residual is bitwise the one the sweep's faces give.  Every thread
 // of the block reaches the two barriers of each plane; threads without a cell skip the work.
-template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false, bool NX = false>
+template <int ORDER, bool SQ, bool SUTH, int ZC = 0, bool LW = false, bool NX = false,
+          bool CP = false>
 __device__ __forceinline__ void residual_fd_body(
     const flux_t* __restrict__ fx, const flux_t* __restrict__ fy, const flux_t* __restrict__ fz,
     const double* __restrict__ W, const double* __restrict__ Dx, const double* __restrict__ Rin,
@@ -2284,7 +2347,7 @@ __device__ __forceinline__ void residual_fd_body(
         }
     }
     if (ZC == 0 && st.Uout && !Rin && !Rout && conv && g.dim == 3 && (xper || i1 + H <= g.nx)) {
-        residual_fd_stage_march<ORDER, SQ, SUTH, LW, NX>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1,
+        residual_fd_stage_march<ORDER, SQ, SUTH, LW, NX, CP>(fx, fy, fz, W, Dx, st, g, ph, row, i0, i1,
                                                      wd, dyx, fxm, m1, m2, m3, ring);
         return;
     }
@@ -2442,12 +2505,16 @@ __device__ __forceinline__ void residual_fd_body(
             un[q] = (st.Un + q * P)[cc] * m1 + (st.Ucur + q * P)[cc] * m2 + r * m3;
 #endif
             if constexpr (!NX) (st.Uout + q * P)[cc] = un[q];
+            if constexpr (!NX && CP) (st.Uf + q * P)[cc] = (float)un[q];
         }
         c2p(un, ph.gamma, w);
 #pragma unroll
-        for (int q = 0; q < 5; q++) (st.Wout + q * P)[cc] = w[q];
+        for (int q = 0; q < 5; q++) {
+            (st.Wout + q * P)[cc] = w[q];
+            if constexpr (!NX && CP) (st.Wf + q * P)[cc] = (float)w[q];
+        }
         if (st.flags) stage_health(st.flags, un, w);
-        if constexpr (NX) next_prologue(st, w, cc, P, g, ph.gamma, dtl);
+        if constexpr (NX) next_prologue<CP>(st, w, cc, P, g, ph.gamma, dtl);
     }
     if constexpr (NX) wave_min_atomic(dtl, st.dtnext);
 }
@@ -2462,10 +2529,11 @@ __global__ __launch_bounds__(256, (fused_waves<ORDER, SQ, SUTH>())) void k_resid
     __shared__ double ring[NS_RING ? ring_doubles<ORDER, SUTH>() : 1];
     int bx, by, bz;
     cell_tile(bx, by, bz);
-    residual_fd_body<ORDER, SQ, SUTH, 0, NS_RING != 0, NX>(fx, fy, fz, W, Dx, Rin, Rout, st, dtmin,
-                                                       which, g, ph, bnd, bx, by, z0 + bz,
-                                                       KW_CELL, FUSED_MARCH, nullptr, nullptr,
-                                                       ring);
+    constexpr bool CP = NS_FP32 && NS_STATE_COPY;
+    residual_fd_body<ORDER, SQ, SUTH, 0, NS_RING != 0, NX, CP>(fx, fy, fz, W, Dx, Rin, Rout, st,
+                                                             dtmin, which, g, ph, bnd, bx, by,
+                                                             z0 + bz, KW_CELL, FUSED_MARCH,
+                                                             nullptr, nullptr, ring);
 }
 
 // the residual + stage update with the z faces formed in the kernel (convection type ZC, Shu
@@ -3041,25 +3109,32 @@ static void l_set_dt(unsigned long long* bits, double dt_over_cfl, hipStream_t s
 }
 
 static void l_step_begin(const double* W, double* Ua, double* Un, double* Wst,
-                         unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s) {
+                         unsigned long long* dtmin, Geo g, Phys ph, hipStream_t s,
+                         const StateCopy* cp) {
     long n  = (long)g.nx * g.ny * g.nz;
     long nb = (n + 255) / 256;
     if (nb > 2048) nb = 2048;
-    k_step_begin<<<(unsigned)nb, 256, 0, s>>>(W, Ua, Un, Wst, dtmin, g, ph.gamma);
+    float* Uf = (cp && cp->U) ? cp->U : nullptr;
+    k_step_begin<<<(unsigned)nb, 256, 0, s>>>(W, Ua, Un, Wst, dtmin, g, ph.gamma, Uf,
+                                              Uf ? cp->W : nullptr);
 }
 
 static void l_halo_faces(double* A, int nA, double* B, int nB, Geo g, Bnd b, int f0, int nf,
-                         hipStream_t s) {
+                         hipStream_t s, const StateCopy* cp) {
     // x, y faces: (k, tangential row, layer); z faces: ((j, layer), i)
     int  m1 = g.ny > g.nx ? g.ny : g.nx;
     int  mx = g.nz > NGC * g.ny ? g.nz : NGC * g.ny;
     dim3 blk(64, 4, 1);
     dim3 grd((mx + 63) / 64, NGC * ((m1 + 3) / 4), nf);
-    if (nf > 0) k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b, f0);
+    if (nf <= 0) return;
+    if (cp && cp->U)
+        k_halo_cp<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, (float*)cp->U, (float*)cp->W, g, b,
+                                      f0);
+    else k_halo<<<grd, blk, 0, s>>>(A, nA, B, B ? nB : 0, g, b, f0);
 }
 
 static void l_halo(double* A, int nA, double* B, int nB, Geo g, Bnd b, hipStream_t s) {
-    l_halo_faces(A, nA, B, nB, g, b, 0, 6, s);
+    l_halo_faces(A, nA, B, nB, g, b, 0, 6, s, nullptr);
 }
 
 static void l_p2c_faces(const double* W, double* U, Geo g, Phys ph, hipStream_t s) {
@@ -3080,8 +3155,8 @@ static long learn_capr(long nb, long fb) { return (nb + LS_REGIONS - 1) / LS_REG
 
 // the tile launch of a learned component-wise sweep in emit mode: reset the counters of the
 // sweep's sample set, the tiles, then the network pass over the left and then the right samples
-template <int DIR>
-static bool flux_emit(const double* U, const double* W, flux_t* f, Geo g, double gamma,
+template <int DIR, class ST>
+static bool flux_emit(const ST* U, const ST* W, flux_t* f, Geo g, double gamma,
                       const MlpPacked* mlp, const LearnSink* sink, dim3 grd, dim3 blk, int lo,
                       int hi, hipStream_t s) {
     const long nb   = (long)grd.x * grd.y * grd.z;
@@ -3103,10 +3178,10 @@ static bool flux_emit(const double* U, const double* W, flux_t* f, Geo g, double
     return true;
 }
 
-template <int DIR, int CONV, int B, int OPT>
-static void flux_dir(const double* U, const double* W, flux_t* f, Geo g, double gamma,
-                     const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s,
-                     const LearnSink* sink = nullptr) {
+template <int DIR, int CONV, int B, int OPT, class ST>
+static void flux_dir_st(const ST* U, const ST* W, flux_t* f, Geo g, double gamma,
+                        const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s,
+                        const LearnSink* sink) {
     constexpr bool EMIT = (CONV == 1 && B == 2 && OPT == 0);
     if (!(parts & 1)) {
     } else if constexpr (DIR == 2) {
@@ -3143,6 +3218,25 @@ static void flux_dir(const double* U, const double* W, flux_t* f, Geo g, double 
     }
 }
 
+// cp: the float copies of (U, W) with their halos (single-precision build, null: none).  The
+// face kernels convert every value they load to real, so reading the copies gives the same
+// fluxes from half the bytes.  The per-face alpha and corrected projection kernels read the
+// double arrays.
+template <int DIR, int CONV, int B, int OPT>
+static void flux_dir(const double* U, const double* W, flux_t* f, Geo g, double gamma,
+                     const MlpPacked* mlp, int parts, int lo, int hi, hipStream_t s,
+                     const LearnSink* sink = nullptr, const StateCopy* cp = nullptr) {
+#if NS_FP32
+    if constexpr (OPT == 0) {
+        if (cp && cp->U) {
+            flux_dir_st<DIR, CONV, B, OPT>(cp->U, cp->W, f, g, gamma, mlp, parts, lo, hi, s, sink);
+            return;
+        }
+    }
+#endif
+    flux_dir_st<DIR, CONV, B, OPT>(U, W, f, g, gamma, mlp, parts, lo, hi, s, sink);
+}
+
 // kernel options of a configuration: bit 0 per-face alpha, bit 1 corrected characteristic
 // projection (characteristic sweeps only).  The learned weights run with neither (the host
 // refuses the combination).
@@ -3153,43 +3247,47 @@ static int flux_opt(const Phys& ph) {
 template <int DIR, int CONV, int B>
 static void flux_dir_opt(const double* U, const double* W, flux_t* f, Geo g, double gamma,
                          const MlpPacked* m, int opt, int parts, int lo, int hi, hipStream_t s,
-                         const LearnSink* sink = nullptr) {
+                         const LearnSink* sink, const StateCopy* cp) {
     if constexpr (B == 2) {
-        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s, sink);
+        flux_dir<DIR, CONV, 2, 0>(U, W, f, g, gamma, m, parts, lo, hi, s, sink, cp);
     } else if constexpr (CONV == 1) {
         if (opt & 1) flux_dir<DIR, 1, B, 1>(U, W, f, g, gamma, m, parts, lo, hi, s);
-        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else flux_dir<DIR, 1, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s, nullptr, cp);
     } else {
         if (opt == 1) flux_dir<DIR, 2, B, 1>(U, W, f, g, gamma, m, parts, lo, hi, s);
         else if (opt == 2) flux_dir<DIR, 2, B, 2>(U, W, f, g, gamma, m, parts, lo, hi, s);
         else if (opt == 3) flux_dir<DIR, 2, B, 3>(U, W, f, g, gamma, m, parts, lo, hi, s);
-        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s);
+        else flux_dir<DIR, 2, B, 0>(U, W, f, g, gamma, m, parts, lo, hi, s, nullptr, cp);
     }
 }
 
 template <int DIR>
 static void face_flux_dir(const double* U, const double* W, flux_t* f, Geo g, Phys ph,
-                          int parts, int lo, int hi, hipStream_t s) {
+                          int parts, int lo, int hi, hipStream_t s, const StateCopy* cp) {
     const MlpPacked* m   = ph.mlp;
     const int        opt = flux_opt(ph);
+    const LearnSink* sk  = ph.sink;
     if (ph.conv == 1) {
-        if (ph.weights == 1) flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        if (ph.weights == 1)
+            flux_dir_opt<DIR, 1, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, nullptr, cp);
         else if (ph.weights == 2)
-            flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, ph.sink);
-        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+            flux_dir_opt<DIR, 1, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, sk, cp);
+        else flux_dir_opt<DIR, 1, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, nullptr, cp);
     } else {
-        if (ph.weights == 1) flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
-        else if (ph.weights == 2) flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
-        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s);
+        if (ph.weights == 1)
+            flux_dir_opt<DIR, 2, 1>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, nullptr, cp);
+        else if (ph.weights == 2)
+            flux_dir_opt<DIR, 2, 2>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, nullptr, cp);
+        else flux_dir_opt<DIR, 2, 0>(U, W, f, g, ph.gamma, m, opt, parts, lo, hi, s, nullptr, cp);
     }
 }
 
 static void l_face_flux(int dir, const double* U, const double* W, double* f, Geo g, Phys ph,
-                        int parts, int lo, int hi, hipStream_t s) {
+                        int parts, int lo, int hi, hipStream_t s, const StateCopy* cp) {
     flux_t* ff = (flux_t*)f;  // the build's flux precision (see flux_t)
-    if (dir == 0) face_flux_dir<0>(U, W, ff, g, ph, parts, lo, hi, s);
-    else if (dir == 1) face_flux_dir<1>(U, W, ff, g, ph, parts, lo, hi, s);
-    else face_flux_dir<2>(U, W, ff, g, ph, parts, lo, hi, s);
+    if (dir == 0) face_flux_dir<0>(U, W, ff, g, ph, parts, lo, hi, s, cp);
+    else if (dir == 1) face_flux_dir<1>(U, W, ff, g, ph, parts, lo, hi, s, cp);
+    else face_flux_dir<2>(U, W, ff, g, ph, parts, lo, hi, s, cp);
 }
 
 // planes [lo, hi) (lo a multiple of MARCH)

@@ -89,6 +89,12 @@ struct nsgpu_ctx {
     LearnSink          sink{};                // sample buffers of the learned component-wise
     void*              sink_mem   = nullptr;  // sweeps (emit mode), allocated at the first
     bool               sink_tried = false;    // single-block learned sweep (ensure_sink)
+    // single-precision build: float copies of the stage state U and its primitives W, halos
+    // included, written by the stage kernel (and the step prologue) beside the double arrays
+    // and read by the next stage's face sweeps (use_copies)
+    StateCopy          cp{};
+    void*              cp_mem  = nullptr;
+    bool               cp_read = true;  // $NSGPU_STATE_COPY=0: the sweeps read the double arrays
     bool               timing = false;
     hipEvent_t         ev[2 * T_COUNT] = {};
     double             ms[T_COUNT]     = {};
@@ -267,22 +273,24 @@ void ensure_sink(nsgpu_ctx* c) {
 inline int full_hi(const nsgpu_ctx* c, int dir) { return dir == 0 ? c->g.nx + 1 : c->g.nx; }
 
 // convection face fluxes of the current (U, W); fz stays zero for dim == 2
+// cp: float copies of (U, W) the sweeps read instead (or null)
 void convection_fluxes(nsgpu_ctx* c, const double* U, const double* W, int parts = 3,
-                       hipStream_t st = nullptr, bool skip_z = false) {
+                       hipStream_t st = nullptr, bool skip_z = false,
+                       const StateCopy* cp = nullptr) {
     if (!c->ph.conv) return;
     ensure_sink(c);
     if (!st) st = c->stream;
     {
         Scope s(c, T_FLUX_X);
-        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, parts, 0, full_hi(c, 0), st);
+        c->L->face_flux(0, U, W, c->F[0], c->g, c->ph, parts, 0, full_hi(c, 0), st, cp);
     }
     {
         Scope s(c, T_FLUX_Y);
-        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, parts, 0, full_hi(c, 1), st);
+        c->L->face_flux(1, U, W, c->F[1], c->g, c->ph, parts, 0, full_hi(c, 1), st, cp);
     }
     if (c->g.dim == 3 && !skip_z) {
         Scope s(c, T_FLUX_Z);
-        c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, parts, 0, full_hi(c, 2), st);
+        c->L->face_flux(2, U, W, c->F[2], c->g, c->ph, parts, 0, full_hi(c, 2), st, cp);
     }
 }
 
@@ -318,29 +326,34 @@ void diffusion_derivs(nsgpu_ctx* c, const double* W, hipStream_t st = nullptr) {
 // stream so that each kernel family is timed alone.
 // zsplit: the z-face halos of U and W are still to be filled; they go on the auxiliary stream,
 // ahead of the z sweep (only it and the work after the join read them)
-void stage_inputs(nsgpu_ctx* c, const double* U, const double* W, bool zsplit = false) {
+// cp: the float copies of (U, W) (use_copies), or null.  With copies only the sweeps read the z
+// halos, and only from the copies: the z-face fill then covers the copies alone.
+void stage_inputs(nsgpu_ctx* c, const double* U, const double* W, bool zsplit = false,
+                  const StateCopy* cp = nullptr) {
     ensure_sink(c);
+    const int nz5 = cp ? 0 : 5;  // planes of U and W whose z halos are filled
     if (c->timing || !c->aux) {
-        if (zsplit) c->L->halo_faces((double*)U, 5, (double*)W, 5, c->g, c->b, 4, 2, c->stream);
-        convection_fluxes(c, U, W, 3, nullptr, zfd(c));
+        if (zsplit)
+            c->L->halo_faces((double*)U, nz5, (double*)W, nz5, c->g, c->b, 4, 2, c->stream, cp);
+        convection_fluxes(c, U, W, 3, nullptr, zfd(c), cp);
         diffusion_derivs(c, W);
         return;
     }
     hipEventRecord(c->ev_fork, c->stream);
     hipStreamWaitEvent(c->aux, c->ev_fork, 0);
     const Geo& g = c->g;
-    if (zsplit) c->L->halo_faces((double*)U, 5, (double*)W, 5, g, c->b, 4, 2, c->aux);
+    if (zsplit) c->L->halo_faces((double*)U, nz5, (double*)W, nz5, g, c->b, 4, 2, c->aux, cp);
     if (c->ph.conv && g.dim == 3 && !zfd(c))
-        c->L->face_flux(2, U, W, c->F[2], g, c->ph, 3, 0, full_hi(c, 2), c->aux);
+        c->L->face_flux(2, U, W, c->F[2], g, c->ph, 3, 0, full_hi(c, 2), c->aux, cp);
     if (c->ph.conv) {
-        c->L->face_flux(0, U, W, c->F[0], g, c->ph, 2, 0, full_hi(c, 0), c->aux);
-        c->L->face_flux(1, U, W, c->F[1], g, c->ph, 2, 0, full_hi(c, 1), c->aux);
+        c->L->face_flux(0, U, W, c->F[0], g, c->ph, 2, 0, full_hi(c, 0), c->aux, cp);
+        c->L->face_flux(1, U, W, c->F[1], g, c->ph, 2, 0, full_hi(c, 1), c->aux, cp);
     }
     diffusion_derivs(c, W, c->aux);
     hipEventRecord(c->ev_join, c->aux);
     if (c->ph.conv) {
-        c->L->face_flux(0, U, W, c->F[0], g, c->ph, 1, 0, full_hi(c, 0), c->stream);
-        c->L->face_flux(1, U, W, c->F[1], g, c->ph, 1, 0, full_hi(c, 1), c->stream);
+        c->L->face_flux(0, U, W, c->F[0], g, c->ph, 1, 0, full_hi(c, 0), c->stream, cp);
+        c->L->face_flux(1, U, W, c->F[1], g, c->ph, 1, 0, full_hi(c, 1), c->stream, cp);
     }
     hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
@@ -368,6 +381,13 @@ bool pipelined(const nsgpu_ctx* c) {
     return c->nslab > 1 && c->aux && !c->timing && c->ph.conv && fused_diffusion(c);
 }
 
+// the face sweeps of a step read the float copies: the single-precision build's fused stage
+// kernel writes them for every stage (not the z-face variant); a pipelined stage's residual
+// writes them while the next slab's sweeps still read, so those read the double arrays
+bool use_copies(const nsgpu_ctx* c) {
+    return c->cp.U && c->cp_read && fused_diffusion(c) && !zfd(c) && !pipelined(c);
+}
+
 // One stage in slabs of planes i.  The face sweeps and the derivative pass of slab s run on the
 // context's stream; the residual + stage update of slab s runs on the auxiliary stream as soon as
 // they are done, beside the sweeps of slab s + 1: the HBM-bound residual overlaps the FP64-bound
@@ -383,9 +403,11 @@ void stage_pipelined(nsgpu_ctx* c, const double* U, const double* W, const Stage
     const Geo&       g = c->g;
     for (int sl = 0; sl < c->nslab; sl++) {
         const int lo = c->slab[sl], hi = c->slab[sl + 1];
-        L.face_flux(0, U, W, c->F[0], g, c->ph, 3, sl == 0 ? 0 : lo + 1, hi + 1, c->stream);
-        L.face_flux(1, U, W, c->F[1], g, c->ph, 3, lo, hi, c->stream);
-        if (g.dim == 3 && !zfd(c)) L.face_flux(2, U, W, c->F[2], g, c->ph, 3, lo, hi, c->stream);
+        L.face_flux(0, U, W, c->F[0], g, c->ph, 3, sl == 0 ? 0 : lo + 1, hi + 1, c->stream,
+                    nullptr);
+        L.face_flux(1, U, W, c->F[1], g, c->ph, 3, lo, hi, c->stream, nullptr);
+        if (g.dim == 3 && !zfd(c))
+            L.face_flux(2, U, W, c->F[2], g, c->ph, 3, lo, hi, c->stream, nullptr);
         L.deriv1(W, c->Dx, nullptr, g, c->ph, lo, hi, c->stream);
         L.deriv_halo(c->Dx, nullptr, g, c->b, lo, hi, c->stream);
         hipEventRecord(c->ev_slab[sl], c->stream);
@@ -510,6 +532,16 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
     }
     if (!rc) rc = alloc_array(&c->R, five, c->stream);
     if (!rc) rc = alloc_array(&c->stage_aos, five, c->stream);
+    if (!rc && cfg->math == NSGPU_MATH_FP32) {
+        // two arrays of `five / 8` floats, each starting so that its first interior cell of a row
+        // (k = NGC) lies on a 128-byte line like the double arrays'
+        const size_t n = (five / sizeof(double) + 31) / 32 * 32;
+        rc = alloc((double**)&c->cp_mem, (2 * n + 64) * sizeof(float), c->stream);
+        if (!rc) {
+            c->cp.U = (float*)c->cp_mem + (32 - NGC);
+            c->cp.W = c->cp.U + n;
+        }
+    }
     c->partial_max = 4096;
     if (!rc) rc = alloc(&c->partial, sizeof(double) * 5 * c->partial_max, c->stream);
     if (!rc) rc = alloc((double**)&c->dtmin, 3 * sizeof(unsigned long long), c->stream);
@@ -534,6 +566,7 @@ int nsgpu_create(const nsgpu_config* cfg, int device, nsgpu_ctx** out) {
             rc = NSGPU_ERR_HIP;
     if (!rc) plan_slabs(c);
     if (const char* e = std::getenv("NSGPU_ZFD")) c->zfd_on = std::atoi(e) != 0;
+    if (const char* e = std::getenv("NSGPU_STATE_COPY")) c->cp_read = std::atoi(e) != 0;
     c->Ulast    = c->Ua;
     c->dt_scale = cfg->cfl;
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = NSGPU_ERR_HIP;
@@ -562,7 +595,8 @@ void nsgpu_destroy(nsgpu_ctx* c) {
     for (double* p : arrays) free_array(p);
     double* bufs[] = {c->partial, (double*)c->dtmin, (double*)c->flags, (double*)c->sflags,
                       c->dtime, c->sums,
-                      (double*)c->mlp_params, (double*)c->mlp_packed, (double*)c->sink_mem};
+                      (double*)c->mlp_params, (double*)c->mlp_packed, (double*)c->sink_mem,
+                      (double*)c->cp_mem};
     for (double* p : bufs)
         if (p) hipFree(p);
     for (int t = 0; t < 2 * T_COUNT; t++)
@@ -721,6 +755,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
     // from its primitives, the stage-1 primitives c2p(U)) in place of the prologue kernel: the
     // same arithmetic per cell, on values the stage holds already
     const bool fuse = fused_diffusion(c) && !zfd(c) && !pipelined(c) && !c->timing;
+    const StateCopy* cpp = use_copies(c) ? &c->cp : nullptr;
     bool       pro  = false;     // this step's prologue was formed by the previous step
     double*    wfirst = c->Ws[0];  // the stage-1 primitives of this step
     for (int s = 0; s < nsteps; s++) {
@@ -733,7 +768,8 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             } else {
                 wfirst = c->Ws[0];
                 L.reset_min(&c->dtmin[0], c->stream);
-                L.step_begin(c->W, nullptr, c->Un, wfirst, &c->dtmin[0], c->g, c->ph, c->stream);
+                L.step_begin(c->W, nullptr, c->Un, wfirst, &c->dtmin[0], c->g, c->ph, c->stream,
+                             cpp);
             }
             if (fixed_dt > 0.0) L.set_dt(&c->dtmin[0], fixed_dt, c->stream);
         }
@@ -752,12 +788,21 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
             {
                 // halos of U and of its primitives: the same copies (c2p acts per cell)
                 Scope sc(c, T_HALO);
-                if (zsplit) L.halo_faces(cur, 5, wcur, 5, c->g, c->b, 0, 4, c->stream);
-                else L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
+                if (cpp) {
+                    // the copies' halos for the sweeps; of the double arrays only those of the
+                    // primitives across x and y, which the residual and the derivative pass read
+                    L.halo_faces(cur, 0, wcur, 5, c->g, c->b, 0, 4, c->stream, cpp);
+                    if (!zsplit) L.halo_faces(cur, 0, wcur, 0, c->g, c->b, 4, 2, c->stream, cpp);
+                } else if (zsplit) {
+                    L.halo_faces(cur, 5, wcur, 5, c->g, c->b, 0, 4, c->stream, nullptr);
+                } else {
+                    L.halo(cur, 5, wcur, 5, c->g, c->b, c->stream);
+                }
             }
             // the stage writes U and c2p(U) of the next stage; the last stage writes the
             // primitive state into the other state buffer (the residual reads wcur meanwhile)
             StageArgs a{c->Un, cur, nxt, (st == 2) ? c->W2 : wnxt, st, c->sflags};
+            a.Uf = c->cp.U, a.Wf = c->cp.W;  // (the single-precision stage kernel needs them)
             if (st == 2 && nx) {  // wnxt is free in the last stage
                 L.reset_min(&c->dtmin[2], c->stream);
                 a.Unext  = c->Un;
@@ -770,7 +815,7 @@ int step_impl(nsgpu_ctx* c, int nsteps, double fixed_dt, double* time_io, double
                 hipStreamWaitEvent(c->aux, c->ev_fork, 0);
                 stage_pipelined(c, cur, wcur, a);
             } else {
-                stage_inputs(c, cur, wcur, zsplit);
+                stage_inputs(c, cur, wcur, zsplit, cpp);
                 Scope sc(c, T_RESIDUAL);
                 residual(c, wcur, nullptr, nullptr, &a, &c->dtmin[0], NSGPU_RHS_BOTH);
             }
