@@ -159,6 +159,8 @@ struct EnsBlock {
     unsigned long long* dtmin;
     double*  dtime;
     unsigned* flags;  // health word of the ensemble step (shared by all blocks)
+    float*   Uc;      // float copies of the stage U and W (single-precision build, StateCopy)
+    float*   Wc;
     int      nbr[6];  // member index of the neighbour behind each linked face, -1 elsewhere
     int      linked;  // the block has linked faces: the separate derivative path, exchanges
     EnsTiles t;
@@ -169,6 +171,8 @@ struct EnsTable {
     const EnsBlock* blk;    // device array [n]
     const int*      start;  // device array [EF_COUNT][n + 1] of workgroup prefix sums
     int             n;
+    int             copies;  // the sweeps read the blocks' float copies (every block has them
+                             // and takes the fused residual, which writes them)
 };
 
 inline int ens_cdiv(int a, int b) { return (a + b - 1) / b; }

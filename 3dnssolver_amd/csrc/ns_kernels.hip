@@ -18,6 +18,7 @@
 //
 // Citations are paths relative to the reference's src/ directory.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "ns_common.h"
@@ -2910,6 +2911,9 @@ __global__ void kb_time_accum(EnsTable T) {
     *E->dtime = *E->dtime + __longlong_as_double((long long)*E->dtmin) * E->ph.cfl;
 }
 
+// CPY: the blocks' float copies of the stage state are written (step prologue, residual),
+// filled (halo) and read by the sweeps
+template <bool CPY>
 __global__ __launch_bounds__(256) void kb_step_begin(EnsTable T, double* Ua) {
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_STEP_BEGIN, bx, by, bz);
@@ -2918,7 +2922,8 @@ __global__ __launch_bounds__(256) void kb_step_begin(EnsTable T, double* Ua) {
     const int       nwg = E->grid[EF_STEP_BEGIN][0];
     double *W = E->W, *Un = E->Un, *Ws = E->Ws[0];
     unsigned long long* dtmin = E->dtmin;
-    step_begin_body(W, Ua, Un, Ws, dtmin, g, gam, bx, nwg);
+    step_begin_body(W, Ua, Un, Ws, dtmin, g, gam, bx, nwg, CPY ? E->Uc : nullptr,
+                    CPY ? E->Wc : nullptr);
 }
 
 // the link of face `f` of block E: the neighbour's stage arrays and its extent along the normal
@@ -2931,12 +2936,20 @@ __device__ __forceinline__ HaloLink ens_link(const EnsTable& T, const EnsBlock* 
                     N->g.sx, N->g.sy, N->g.P};
 }
 
+// CPY: the copies' halos, and of the double arrays only those of the primitives across x and y
+// (the residual and the derivative pass read no others; the sweeps read the copies)
+template <bool CPY>
 __global__ void kb_halo(EnsTable T, int s) {
     int             bx, by, bz;
     const EnsBlock* E  = T.blk + ens_locate(T, EF_HALO, bx, by, bz);
     const Geo       g  = E->g;
     const Bnd       bd = E->b;
     double *U = ens_u(E, s), *Wp = ens_w(E, s);
+    if constexpr (CPY) {  // (no linked blocks)
+        halo_body(U, 0, Wp, bz < 4 ? 5 : 0, g, bd, bx, by, bz);
+        halo_body(E->Uc, 5, E->Wc, 5, g, bd, bx, by, bz);
+        return;
+    }
     if (E->linked) {
         const HaloLink lk = ens_link(T, E, bz, s);  // bz is the face
         halo_body(U, 5, Wp, 5, g, bd, bx, by, bz, &lk);
@@ -2945,7 +2958,7 @@ __global__ void kb_halo(EnsTable T, int s) {
     }
 }
 
-template <int DIR, int CONV, int WT, int OPT>
+template <int DIR, int CONV, int WT, int OPT, bool CPY>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T, int s) {
     __shared__ real tile[(TILE_S + 5) * split_nv<OPT>() * TILE_L];
     __shared__ real lq[WT == 2 ? TILE_S * LQ_REALS : 1];
@@ -2958,11 +2971,15 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_xy(EnsTable T
     flux_t*       f = (flux_t*)E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
     // the whole block: faces 0 .. nx-1 of the x sweep, planes 0 .. nx-1 of the y sweep
-    flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx,
-                                          WT == 2 ? lq : nullptr);
+    if constexpr (CPY)
+        flux_tile_xy_body<DIR, CONV, WT, OPT>(E->Uc, E->Wc, f, g, gam, bx, by, bz, kw, tile, mlp,
+                                              0, g.nx, WT == 2 ? lq : nullptr);
+    else
+        flux_tile_xy_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp, 0, g.nx,
+                                              WT == 2 ? lq : nullptr);
 }
 
-template <int CONV, int WT, int OPT>
+template <int CONV, int WT, int OPT, bool CPY>
 __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T, int s) {
     __shared__ real tile[8 * zlds_wave_max<OPT>()];
     __shared__ real lq[WT == 2 ? 8 * LQ_REALS : 1];
@@ -2974,11 +2991,15 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void kb_flux_tile_z(EnsTable T,
     const double *U = ens_u(E, s), *Wp = ens_w(E, s);
     flux_t*       f = (flux_t*)E->F[2];
     const MlpPacked* mlp = E->ph.mlp;
-    flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp,
-                                    WT == 2 ? lq : nullptr);
+    if constexpr (CPY)
+        flux_tile_z_body<CONV, WT, OPT>(E->Uc, E->Wc, f, g, gam, bx, by, bz, kw, tile, mlp,
+                                        WT == 2 ? lq : nullptr);
+    else
+        flux_tile_z_body<CONV, WT, OPT>(U, Wp, f, g, gam, bx, by, bz, kw, tile, mlp,
+                                        WT == 2 ? lq : nullptr);
 }
 
-template <int DIR, int CONV, int WT, int OPT>
+template <int DIR, int CONV, int WT, int OPT, bool CPY>
 __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     int             bx, by, bz;
     const EnsBlock* E   = T.blk + ens_locate(T, EF_PLANE_X + DIR, bx, by, bz);
@@ -2989,8 +3010,12 @@ __global__ __launch_bounds__(256) void kb_flux_plane(EnsTable T, int s) {
     flux_t*       f = (flux_t*)E->F[DIR];
     const MlpPacked* mlp = E->ph.mlp;
     __shared__ real lq[WT == 2 ? 4 * LQ_REALS : 1];
-    flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp,
-                                        WT == 2 ? lq : nullptr);
+    if constexpr (CPY)
+        flux_plane_body<DIR, CONV, WT, OPT>(E->Uc, E->Wc, f, g, gam, NGC + n - 1, bx, by, mlp,
+                                            WT == 2 ? lq : nullptr);
+    else
+        flux_plane_body<DIR, CONV, WT, OPT>(U, Wp, f, g, gam, NGC + n - 1, bx, by, mlp,
+                                            WT == 2 ? lq : nullptr);
 }
 
 // ensemble members take the same derivative split as single blocks (k_residual_fd): dW/dx of
@@ -3038,7 +3063,7 @@ __global__ void kb_deriv_halo(EnsTable T) {
 
 // which / Rin / Rout are launch arguments (3, null, null) rather than constants, so that the body
 // keeps the control flow of the single-block kernel
-template <int ORDER, bool SUTH>
+template <int ORDER, bool SUTH, bool CPY>
 __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
                                                    const double* Rin, double* Rout) {
     int             bx, by, bz;
@@ -3052,16 +3077,19 @@ __global__ __launch_bounds__(256) void kb_residual(EnsTable T, int s, int which,
     const double* Wp = ens_w(E, s);
     StageArgs     a{E->Un, ens_u(E, s), s == 1 ? E->Ub : E->Ua,
                     s == 0 ? E->Ws[1] : (s == 1 ? E->Ws[0] : E->W), s, E->flags};
+    a.Uf = E->Uc, a.Wf = E->Wc;
     const unsigned long long* dtmin = E->dtmin;
     if constexpr (ORDER > 0) {
         if (ens_fused(g, E->linked)) {
             const Bnd bd = E->b;
             if (g.square_xy)
-                residual_fd_body<ORDER, true, SUTH>(fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin, which,
-                                                    g, ph, bd, bx, by, bz, kw, march);
+                residual_fd_body<ORDER, true, SUTH, 0, false, false, CPY>(
+                    fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin, which, g, ph, bd, bx, by, bz, kw,
+                    march);
             else
-                residual_fd_body<ORDER, false, SUTH>(fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin,
-                                                     which, g, ph, bd, bx, by, bz, kw, march);
+                residual_fd_body<ORDER, false, SUTH, 0, false, false, CPY>(
+                    fx, fy, fz, Wp, Dx, Rin, Rout, a, dtmin, which, g, ph, bd, bx, by, bz, kw,
+                    march);
             return;
         }
     }
@@ -3171,8 +3199,13 @@ static bool flux_emit(const ST* U, const ST* W, flux_t* f, Geo g, double gamma,
     else k_flux_tile_xy_emit<DIR><<<grd, blk, 0, s>>>(U, W, f, g, gamma, lo, hi, e);
     // enough wavefronts to cover the occupied chunks of a typical launch several times over;
     // each strides over the rest
+    static const long max_blk = [] {
+        const char* e = std::getenv("NSGPU_NET_BLOCKS");
+        const long  v = e ? std::atol(e) : 0;
+        return v > 0 ? v : 2048L;
+    }();
     const long chunks = (capr * LS_REGIONS + 63) / 64;
-    const long nblk   = std::min<long>(2048, std::max<long>(1, (chunks + 3) / 4));
+    const long nblk   = std::min<long>(max_blk, std::max<long>(1, (chunks + 3) / 4));
     k_learn_net<<<(unsigned)nblk, 256, 0, s>>>(mlp, e, 0, f);
     k_learn_net<<<(unsigned)nblk, 256, 0, s>>>(mlp, e, 1, f);
     return true;
@@ -3509,59 +3542,79 @@ static void l_selftest_div(long n, unsigned long long seed, unsigned long long* 
     k_selftest_div<<<1024, 256, 0, s>>>(n, seed, count);
 }
 
-template <int CONV, int B, int OPT>
+template <int CONV, int B, int OPT, bool CPY>
 static void ens_fluxes_t(const EnsTable& T, const int* total, int s, hipStream_t st) {
     dim3 tb(TILE_L, 8, 1), pb(64, 4, 1);
-    if (total[EF_FLUX_X]) kb_flux_tile_xy<0, CONV, B, OPT><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_X]) kb_flux_plane<0, CONV, B, OPT><<<total[EF_PLANE_X], pb, 0, st>>>(T, s);
-    if (total[EF_FLUX_Y]) kb_flux_tile_xy<1, CONV, B, OPT><<<total[EF_FLUX_Y], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_Y]) kb_flux_plane<1, CONV, B, OPT><<<total[EF_PLANE_Y], pb, 0, st>>>(T, s);
-    if (total[EF_FLUX_Z]) kb_flux_tile_z<CONV, B, OPT><<<total[EF_FLUX_Z], tb, 0, st>>>(T, s);
-    if (total[EF_PLANE_Z]) kb_flux_plane<2, CONV, B, OPT><<<total[EF_PLANE_Z], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_X])
+        kb_flux_tile_xy<0, CONV, B, OPT, CPY><<<total[EF_FLUX_X], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_X])
+        kb_flux_plane<0, CONV, B, OPT, CPY><<<total[EF_PLANE_X], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Y])
+        kb_flux_tile_xy<1, CONV, B, OPT, CPY><<<total[EF_FLUX_Y], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Y])
+        kb_flux_plane<1, CONV, B, OPT, CPY><<<total[EF_PLANE_Y], pb, 0, st>>>(T, s);
+    if (total[EF_FLUX_Z]) kb_flux_tile_z<CONV, B, OPT, CPY><<<total[EF_FLUX_Z], tb, 0, st>>>(T, s);
+    if (total[EF_PLANE_Z])
+        kb_flux_plane<2, CONV, B, OPT, CPY><<<total[EF_PLANE_Z], pb, 0, st>>>(T, s);
 }
 
-template <int CONV, int B>
+// the copies serve the kernels without options (as for single blocks)
+template <int CONV, int B, bool CPY>
 static void ens_fluxes(const EnsTable& T, const int* total, int s, int opt, hipStream_t st) {
     if constexpr (B == 2) {
-        ens_fluxes_t<CONV, 2, 0>(T, total, s, st);
+        ens_fluxes_t<CONV, 2, 0, CPY>(T, total, s, st);
     } else if constexpr (CONV == 1) {
-        if (opt & 1) ens_fluxes_t<1, B, 1>(T, total, s, st);
-        else ens_fluxes_t<1, B, 0>(T, total, s, st);
+        if (opt & 1) ens_fluxes_t<1, B, 1, false>(T, total, s, st);
+        else ens_fluxes_t<1, B, 0, CPY>(T, total, s, st);
     } else {
-        if (opt == 1) ens_fluxes_t<2, B, 1>(T, total, s, st);
-        else if (opt == 2) ens_fluxes_t<2, B, 2>(T, total, s, st);
-        else if (opt == 3) ens_fluxes_t<2, B, 3>(T, total, s, st);
-        else ens_fluxes_t<2, B, 0>(T, total, s, st);
+        if (opt == 1) ens_fluxes_t<2, B, 1, false>(T, total, s, st);
+        else if (opt == 2) ens_fluxes_t<2, B, 2, false>(T, total, s, st);
+        else if (opt == 3) ens_fluxes_t<2, B, 3, false>(T, total, s, st);
+        else ens_fluxes_t<2, B, 0, CPY>(T, total, s, st);
     }
 }
 
-template <int ORDER, bool SUTH>
+template <int ORDER, bool SUTH, bool CPY>
 static void ens_stage(const EnsTable& T, const int* total, const Phys& ph, int s, hipStream_t st) {
-    kb_halo<<<total[EF_HALO], dim3(64, 4, 1), 0, st>>>(T, s);
+    kb_halo<CPY><<<total[EF_HALO], dim3(64, 4, 1), 0, st>>>(T, s);
     const int opt = flux_opt(ph);
     if (ph.conv == 1) {
-        if (ph.weights == 1) ens_fluxes<1, 1>(T, total, s, opt, st);
-        else if (ph.weights == 2) ens_fluxes<1, 2>(T, total, s, opt, st);
-        else ens_fluxes<1, 0>(T, total, s, opt, st);
+        if (ph.weights == 1) ens_fluxes<1, 1, CPY>(T, total, s, opt, st);
+        else if (ph.weights == 2) ens_fluxes<1, 2, CPY>(T, total, s, opt, st);
+        else ens_fluxes<1, 0, CPY>(T, total, s, opt, st);
     } else if (ph.conv == 2) {
-        if (ph.weights == 1) ens_fluxes<2, 1>(T, total, s, opt, st);
-        else if (ph.weights == 2) ens_fluxes<2, 2>(T, total, s, opt, st);
-        else ens_fluxes<2, 0>(T, total, s, opt, st);
+        if (ph.weights == 1) ens_fluxes<2, 1, CPY>(T, total, s, opt, st);
+        else if (ph.weights == 2) ens_fluxes<2, 2, CPY>(T, total, s, opt, st);
+        else ens_fluxes<2, 0, CPY>(T, total, s, opt, st);
     }
     if constexpr (ORDER > 0) {
         kb_deriv1<ORDER><<<total[EF_DERIV1], 256, 0, st>>>(T, s);
         kb_deriv_halo<<<total[EF_DERIV_HALO], dim3(64, 4, 1), 0, st>>>(T);
     }
-    kb_residual<ORDER, SUTH><<<total[EF_RESIDUAL], 256, 0, st>>>(T, s, 3, nullptr, nullptr);
+    kb_residual<ORDER, SUTH, CPY><<<total[EF_RESIDUAL], 256, 0, st>>>(T, s, 3, nullptr, nullptr);
 }
 
-template <int ORDER, bool SUTH>
-static void ens_step_t(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {
+template <int ORDER, bool SUTH, bool CPY>
+static void ens_step_c(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {
     const unsigned nb = (unsigned)((T.n + 63) / 64);
     kb_reset_min<<<nb, 64, 0, st>>>(T);
-    kb_step_begin<<<total[EF_STEP_BEGIN], 256, 0, st>>>(T, nullptr);
-    for (int s = 0; s < 3; s++) ens_stage<ORDER, SUTH>(T, total, ph, s, st);
+    kb_step_begin<CPY><<<total[EF_STEP_BEGIN], 256, 0, st>>>(T, nullptr);
+    for (int s = 0; s < 3; s++) ens_stage<ORDER, SUTH, CPY>(T, total, ph, s, st);
     kb_time_accum<<<nb, 64, 0, st>>>(T);
+}
+
+// the float copies: single-precision build, diffusion on (the fused residual writes them)
+template <int ORDER, bool SUTH>
+static void ens_step_t(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {
+#if NS_FP32
+    if constexpr (ORDER > 0) {
+        if (T.copies) {
+            ens_step_c<ORDER, SUTH, true>(T, total, ph, st);
+            return;
+        }
+    }
+#endif
+    ens_step_c<ORDER, SUTH, false>(T, total, ph, st);
 }
 
 static void l_ens_step(const EnsTable& T, const int* total, const Phys& ph, hipStream_t st) {

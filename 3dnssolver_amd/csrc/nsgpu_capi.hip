@@ -998,6 +998,7 @@ struct nsgpu_ensemble {
     // decompositions (nsgpu_domain): neighbour member per face and the shared CFL minimum
     std::vector<std::array<int, 6>> nbr;
     unsigned long long*             shared_dt = nullptr;
+    int                             copies    = 0;  // EnsTable::copies
 };
 
 namespace {
@@ -1027,9 +1028,15 @@ int ens_build(nsgpu_ensemble* e) {
         k.flags = e->flags;
         k.linked = c->linked;
         for (int f = 0; f < 6; f++) k.nbr[f] = e->nbr.empty() ? -1 : e->nbr[b][f];
+        k.Uc = c->cp.U, k.Wc = c->cp.W;
         k.t     = e->tiles[b];
         ens_grids(k);
     }
+    // the sweeps read the float copies when every block has them and takes the fused residual
+    // (which writes them), as for a single block (use_copies)
+    e->copies = 1;
+    for (nsgpu_ctx* c : e->m)
+        if (!(c->cp.U && c->cp_read && fused_diffusion(c))) e->copies = 0;
     for (int f = 0; f < EF_COUNT; f++) {
         long acc = 0;
         for (int b = 0; b < n; b++) {
@@ -1049,7 +1056,9 @@ int ens_build(nsgpu_ensemble* e) {
     return NSGPU_OK;
 }
 
-EnsTable ens_table(const nsgpu_ensemble* e) { return EnsTable{e->dblk, e->dstart, (int)e->m.size()}; }
+EnsTable ens_table(const nsgpu_ensemble* e) {
+    return EnsTable{e->dblk, e->dstart, (int)e->m.size(), e->copies};
+}
 
 void ens_enqueue(nsgpu_ensemble* e) {
     const nsgpu_ctx* c0 = e->m[0];

@@ -3,14 +3,15 @@ primitives) that the stage kernel writes beside its FP64 output; the halo fill c
 their z halos only for the copies.  The sweeps convert every loaded value to float either way, so
 the steps must be bitwise those of the sweeps reading the double arrays ($NSGPU_STATE_COPY=0):
 characteristic and component-wise WENO-5, periodic and walled faces, both viscosity laws, the
-learned weights, and calls of one step (prologue kernel only) and of several (the prologue
-formed by the last stage).
+learned weights, calls of one step (prologue kernel only) and of several (the prologue formed by
+the last stage), and ensembles of uneven blocks (whose batched kernels use the members' copies).
 """
 import numpy as np
 import pytest
 
 import pyoracle as po
 from conftest import cases_module, package
+from test_gpu_ensemble import _members
 from test_mlp_oracle import random_params
 
 pytestmark = pytest.mark.gpu
@@ -62,3 +63,28 @@ def test_state_copies_learned_weights(ns, monkeypatch):
     b = _run(ns, case, W, monkeypatch, "1", (2, 2), params)
     for x, y in zip(a, b):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("conv", ["weno5cw", "weno5"])
+def test_state_copies_in_ensembles(ns, monkeypatch, conv):
+    print(f"[gpu] state copies ensemble {conv}", flush=True)
+    members = _members(conv=conv)
+    out = {}
+    for copy in ("0", "1"):
+        monkeypatch.setenv("NSGPU_STATE_COPY", copy)
+        solvers = []
+        for case, W in members:
+            s = ns.Solver(case, math="fp32")
+            s.set_state(W, "primitive")
+            solvers.append(s)
+        ens = ns.Ensemble(solvers)
+        ens.step(2)
+        ens.step(1, graph=False)
+        out[copy] = [(po.interior(s.get_state("primitive")),
+                      po.interior(s.get_state("conservative"))) for s in solvers]
+        ens.close()
+        for s in solvers:
+            s.close()
+    for a, b in zip(out["0"], out["1"]):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
