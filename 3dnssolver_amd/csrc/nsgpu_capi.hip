@@ -381,12 +381,20 @@ bool pipelined(const nsgpu_ctx* c) {
     return c->nslab > 1 && c->aux && !c->timing && c->ph.conv && fused_diffusion(c);
 }
 
-// the face sweeps of a step read the float copies: the single-precision build's fused stage
-// kernel writes them for every stage (not the z-face variant); a pipelined stage's residual
-// writes them while the next slab's sweeps still read, so those read the double arrays
-bool use_copies(const nsgpu_ctx* c) {
-    return c->cp.U && c->cp_read && fused_diffusion(c) && !zfd(c) && !pipelined(c);
+// a context whose sweeps may read its float copies: the single-precision build's fused stage
+// kernel writes them for every stage, and the sweep kernels without options read them (the
+// per-face alpha and the corrected projection read the double arrays, whose halos then have to
+// be filled in full)
+bool copies_ok(const nsgpu_ctx* c) {
+    const bool opt = c->ph.alpha_mode == NSGPU_ALPHA_LOCAL_MAX ||
+                     (c->ph.conv == 2 && (c->ph.physics & NSGPU_PHYS_CONVECTION));
+    return c->cp.U && c->cp_read && fused_diffusion(c) && !opt;
 }
+
+// the face sweeps of a step read the float copies (not with the z faces formed in the residual,
+// which reads the double arrays; a pipelined stage's residual writes the copies while the next
+// slab's sweeps still read, so those read the double arrays)
+bool use_copies(const nsgpu_ctx* c) { return copies_ok(c) && !zfd(c) && !pipelined(c); }
 
 // One stage in slabs of planes i.  The face sweeps and the derivative pass of slab s run on the
 // context's stream; the residual + stage update of slab s runs on the auxiliary stream as soon as
@@ -1036,7 +1044,7 @@ int ens_build(nsgpu_ensemble* e) {
     // (which writes them), as for a single block (use_copies)
     e->copies = 1;
     for (nsgpu_ctx* c : e->m)
-        if (!(c->cp.U && c->cp_read && fused_diffusion(c))) e->copies = 0;
+        if (!copies_ok(c)) e->copies = 0;
     for (int f = 0; f < EF_COUNT; f++) {
         long acc = 0;
         for (int b = 0; b < n; b++) {

@@ -53,6 +53,47 @@ def test_state_copies_give_the_same_steps(ns, monkeypatch, conv, bc, viscosity):
     assert np.all(np.isfinite(a[0]))
 
 
+@pytest.mark.parametrize("conv,alpha,physics", [("weno5", "local_max", "reference"),
+                                                ("weno5cw", "local_max", "reference"),
+                                                ("weno5cw", "const3", "corrected_convection")])
+def test_sweeps_with_options_keep_the_double_halos(ns, monkeypatch, conv, alpha, physics):
+    # the per-face alpha and the corrected projection read the double arrays: their halos
+    # (U, and the z faces) must still be filled when the context holds copies
+    print(f"[gpu] state copies with options {conv} {alpha} {physics}", flush=True)
+    C = cases_module()
+    case = dict(C.base_case(19, 13, 11), conv=conv, alpha_mode=alpha, physics=physics,
+                L=(1.0, 0.8, 0.7), bc=WALLS)
+    W = C.random_smooth_W(case, seed=29)
+    a = _run(ns, case, W, monkeypatch, "0", (1, 3))
+    b = _run(ns, case, W, monkeypatch, "1", (1, 3))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    # and the same steps in EXACT math agree to the single-precision round-off
+    monkeypatch.setenv("NSGPU_STATE_COPY", "1")
+    s = ns.Solver(case, math="exact")
+    s.set_state(W, "primitive")
+    s.step(1)
+    s.step(3)
+    e = po.interior(s.get_state("primitive"))
+    s.close()
+    d = np.abs(e - b[0]).reshape(-1, 5).max(axis=0) / np.abs(e).reshape(-1, 5).max(axis=0)
+    assert d.max() < 1e-4, d
+
+
+@pytest.mark.parametrize("nz", [1, 2])
+def test_state_copies_two_dimensional(ns, monkeypatch, nz):
+    # a 2-D block (no z sweep; its z halos are thinner than the halo for nz < 3)
+    print(f"[gpu] state copies 2-D nz={nz}", flush=True)
+    C = cases_module()
+    case = dict(C.base_case(32, 24, nz), dim=2, L=(1.0, 1.0, nz / 32.0), conv="weno5cw",
+                bc=WALLS)
+    W = C.random_smooth_W(case, seed=23)
+    a = _run(ns, case, W, monkeypatch, "0", (1, 3))
+    b = _run(ns, case, W, monkeypatch, "1", (1, 3))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
 def test_state_copies_learned_weights(ns, monkeypatch):
     print("[gpu] state copies learned", flush=True)
     C = cases_module()
@@ -65,10 +106,11 @@ def test_state_copies_learned_weights(ns, monkeypatch):
         np.testing.assert_array_equal(x, y)
 
 
-@pytest.mark.parametrize("conv", ["weno5cw", "weno5"])
-def test_state_copies_in_ensembles(ns, monkeypatch, conv):
-    print(f"[gpu] state copies ensemble {conv}", flush=True)
-    members = _members(conv=conv)
+@pytest.mark.parametrize("conv,alpha", [("weno5cw", "const3"), ("weno5", "const3"),
+                                        ("weno5cw", "local_max")])
+def test_state_copies_in_ensembles(ns, monkeypatch, conv, alpha):
+    print(f"[gpu] state copies ensemble {conv} {alpha}", flush=True)
+    members = [(dict(case, alpha_mode=alpha), W) for case, W in _members(conv=conv)]
     out = {}
     for copy in ("0", "1"):
         monkeypatch.setenv("NSGPU_STATE_COPY", copy)
