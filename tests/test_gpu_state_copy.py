@@ -53,6 +53,19 @@ def test_state_copies_give_the_same_steps(ns, monkeypatch, conv, bc, viscosity):
     assert np.all(np.isfinite(a[0]))
 
 
+@pytest.mark.parametrize("conv", ["weno5cw", "weno5"])
+def test_state_copies_odd_block(ns, monkeypatch, conv):
+    # extents that leave partial k segments, row tiles and sweep tiles in every kernel
+    print(f"[gpu] state copies odd block {conv}", flush=True)
+    C = cases_module()
+    case = dict(C.base_case(67, 45, 130), conv=conv, viscosity="sutherland", L=(1.3, 0.9, 2.1))
+    W = C.random_smooth_W(case, seed=31)
+    a = _run(ns, case, W, monkeypatch, "0", (3,))
+    b = _run(ns, case, W, monkeypatch, "1", (3,))
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
 @pytest.mark.parametrize("conv,alpha,physics", [("weno5", "local_max", "reference"),
                                                 ("weno5cw", "local_max", "reference"),
                                                 ("weno5cw", "const3", "corrected_convection")])
