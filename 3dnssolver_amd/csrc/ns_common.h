@@ -72,8 +72,12 @@ constexpr int LS_STRIDE  = 32;  // counters one 128-byte line apart
 struct LearnSink {
     void*     v[2][2];     // [set][side]: samples, planes [5][cap] of real
     unsigned* dest[2][2];  // [set][side]: flux index of each sample
-    unsigned* cnt[2];      // [set]: counters [side][LS_REGIONS] at stride LS_STRIDE
+    unsigned* cnt[2][2];   // [set][parity]: counters [side][LS_REGIONS] at stride LS_STRIDE
     long      cap;         // samples per side and set
+    // the counters of a set alternate between two arrays from one launch to the next: a launch
+    // counts in cnt[set][parity[set]] and its first workgroup zeroes the other array, which the
+    // previous launch's network pass has finished reading (one stream per set)
+    mutable int parity[2];
 };
 
 struct Phys {

@@ -222,9 +222,17 @@ struct LearnEmit {
     real*     v[2];     // per side: planes [5][cap] (stencil in left-biased order)
     unsigned* dest[2];  // per side: flux index of each sample
     unsigned* cnt;      // counters [side][LS_REGIONS] at stride LS_STRIDE
+    unsigned* zero;     // the set's other counters, zeroed by the launch's first workgroup
     long      cap;      // samples per side
     long      capr;     // samples per region of this launch
 };
+
+// the first workgroup of an emit-mode launch zeroes the counters of the set's next launch
+__device__ __forceinline__ void learn_zero_next(const LearnEmit& e) {
+    if (blockIdx.x != 0 || blockIdx.y != 0 || blockIdx.z != 0) return;
+    const int t = threadIdx.x + threadIdx.y * blockDim.x;
+    if (t < 2 * LS_REGIONS) e.zero[(long)t * LS_STRIDE] = 0u;
+}
 struct Learn {
     const MlpPacked* P;
     bool             active;
@@ -439,13 +447,18 @@ __device__ __forceinline__ void face_flux_learned_emit(const S& st, real out[5],
         if (n1)
             t1 = (int)atomicAdd(E.cnt + (long)(LS_REGIONS + L.region) * LS_STRIDE, (unsigned)n1);
     }
-    long b0 = (long)L.region * E.capr + (unsigned)__shfl(t0, 0);
-    long b1 = (long)L.region * E.capr + (unsigned)__shfl(t1, 0);
+    const unsigned s0 = (unsigned)__shfl(t0, 0), s1 = (unsigned)__shfl(t1, 0);
+    // the buffers hold every reconstruction of the region's faces, so a reservation never ends
+    // past its region while the counters start each launch at zero; a wavefront whose
+    // reservation would end past it writes none of that side's samples
+    const bool ok0 = (long)s0 + n0 <= E.capr, ok1 = (long)s1 + n1 <= E.capr;
+    long b0 = (long)L.region * E.capr + s0;
+    long b1 = (long)L.region * E.capr + s1;
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         __asm__ __volatile__("" ::: "memory");  // (as above: the stencil is read again)
-        if (m[r] == 0) continue;  // wavefront-uniform
         const int side = r & 1;
+        if (m[r] == 0 || !(side ? ok1 : ok0)) continue;  // wavefront-uniform
         if ((m[r] >> lane) & 1ull) {
             real v[5];
             learned_stencil(st, r, v);
@@ -472,7 +485,8 @@ __device__ __forceinline__ void learn_net_body(const MlpPacked* __restrict__ P,
     const long     gw   = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const long     nw   = ((long)gridDim.x * blockDim.x) >> 6;
     const unsigned* cnt = E.cnt + (long)side * LS_REGIONS * LS_STRIDE;
-    unsigned       mx   = (l < LS_REGIONS) ? cnt[(long)l * LS_STRIDE] : 0u;
+    const unsigned cr   = (unsigned)E.capr;  // (a region's count never exceeds it)
+    unsigned       mx   = (l < LS_REGIONS) ? min(cnt[(long)l * LS_STRIDE], cr) : 0u;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const unsigned t = (unsigned)__shfl_xor((int)mx, o, 64);
@@ -484,7 +498,7 @@ __device__ __forceinline__ void learn_net_body(const MlpPacked* __restrict__ P,
     for (long ch = gw; ch < limit; ch += nw) {
         const int      r  = (int)(ch % LS_REGIONS);
         const long     lc = ch / LS_REGIONS;
-        const unsigned n  = cnt[(long)r * LS_STRIDE];
+        const unsigned n  = min(cnt[(long)r * LS_STRIDE], cr);
         if (lc * 64 >= (long)n) continue;  // wavefront-uniform
         const bool has = lc * 64 + l < (long)n;
         const long s   = (long)r * E.capr + lc * 64 + l;
@@ -1128,6 +1142,7 @@ __global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy_e
     const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, int lo, int hi, LearnEmit e) {
     __shared__ real tile[(TILE_S1 + 5) * split_nv<0>() * TILE_L];
+    learn_zero_next(e);
     flux_tile_xy_body<DIR, 1, 2, 0, TILE_S1, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
                                                    blockIdx.z, TILE_L, tile, nullptr, lo, hi,
                                                    nullptr, &e);
@@ -1137,12 +1152,9 @@ __global__ __launch_bounds__(512, NS_TILE_WAVES) void k_flux_tile_z_emit(
     const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, int lo, LearnEmit e) {
     __shared__ real tile[8 * (TILE_L + 5) * split_nv<0>()];
+    learn_zero_next(e);
     flux_tile_z_body<1, 2, 0, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y, lo + blockIdx.z,
                                     TILE_L, tile, nullptr, nullptr, &e);
-}
-__global__ void k_learn_reset(unsigned* cnt) {
-    const int t = threadIdx.x;
-    if (t < 2 * LS_REGIONS) cnt[(long)t * LS_STRIDE] = 0u;
 }
 #ifndef NS_NET_WAVES
 #define NS_NET_WAVES 5  // minimum waves per SIMD requested for the network pass (95 VGPRs)
@@ -3192,9 +3204,11 @@ static bool flux_emit(const ST* U, const ST* W, flux_t* f, Geo g, double gamma,
     if (!sink || capr * LS_REGIONS > sink->cap || 4 * g.P + (long)g.NX * g.sx > 0xffffffffL)
         return false;
     const int set = (DIR == 2) ? 1 : 0;
+    const int par = sink->parity[set];
+    sink->parity[set] ^= 1;
     LearnEmit e{{(real*)sink->v[set][0], (real*)sink->v[set][1]},
-                {sink->dest[set][0], sink->dest[set][1]}, sink->cnt[set], sink->cap, capr};
-    k_learn_reset<<<1, 2 * LS_REGIONS, 0, s>>>(sink->cnt[set]);
+                {sink->dest[set][0], sink->dest[set][1]}, sink->cnt[set][par],
+                sink->cnt[set][par ^ 1], sink->cap, capr};
     if constexpr (DIR == 2) k_flux_tile_z_emit<<<grd, blk, 0, s>>>(U, W, f, g, gamma, lo, e);
     else k_flux_tile_xy_emit<DIR><<<grd, blk, 0, s>>>(U, W, f, g, gamma, lo, hi, e);
     // enough wavefronts to cover the occupied chunks of a typical launch several times over;

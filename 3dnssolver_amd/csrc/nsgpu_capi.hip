@@ -249,7 +249,7 @@ void ensure_sink(nsgpu_ctx* c) {
     const size_t vbytes = ((size_t)cap * 5 * vb + 255) / 256 * 256;
     const size_t dbytes = ((size_t)cap * 4 + 255) / 256 * 256;
     const size_t cbytes = (size_t)2 * LS_REGIONS * LS_STRIDE * sizeof(unsigned);
-    const size_t total  = 4 * (vbytes + dbytes) + 2 * cbytes;
+    const size_t total  = 4 * (vbytes + dbytes) + 4 * cbytes;
     if (total > SINK_MAX_BYTES) return;
     char* m = nullptr;
     if (hipMalloc((void**)&m, total) != hipSuccess) {
@@ -262,8 +262,19 @@ void ensure_sink(nsgpu_ctx* c) {
             c->sink.v[set][side] = m, m += vbytes;
             c->sink.dest[set][side] = (unsigned*)m, m += dbytes;
         }
-        c->sink.cnt[set] = (unsigned*)m, m += cbytes;
+        c->sink.cnt[set][0] = (unsigned*)m, m += cbytes;
+        c->sink.cnt[set][1] = (unsigned*)m, m += cbytes;
     }
+    // every counter starts at zero; from then on each emit launch zeroes the next launch's
+    for (int set = 0; set < 2; set++) {  // (the two arrays of a set are adjacent)
+        if (hipMemsetAsync(c->sink.cnt[set][0], 0, 2 * cbytes, c->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            hipFree(c->sink_mem);
+            c->sink_mem = nullptr;
+            return;
+        }
+    }
+    c->sink.parity[0] = c->sink.parity[1] = 0;
     c->sink.cap = cap;
     c->ph.sink  = &c->sink;
 }
