@@ -949,16 +949,17 @@ __device__ __forceinline__ int learn_region() {
 
 constexpr int TILE_S = 8;    // x/y tiles: faces along the sweep per block (one wave each)
 constexpr int TILE_L = 64;   // lanes along k
-// sweep tile of the single-block x/y kernels: 16 faces (1024-thread blocks) in single precision,
-// where the 32 KB tile of 8 faces leaves room and the 5 extra cells per tile weigh more
-#ifndef NS_TILE_S1
-#if NS_FP32
-#define NS_TILE_S1 16
+// sweep tile of the single-block x/y kernels of convection type CONV: 8 faces per block (one wave
+// each); the single-precision characteristic sweeps take 16 (1024-thread blocks), where the 32 KB
+// tile of 8 faces leaves room and the 5 extra cells per tile weigh more (the component-wise
+// sweeps, with less arithmetic per face, run faster with 8)
+template <int CONV> constexpr int tile_s1() {
+#ifdef NS_TILE_S1
+    return NS_TILE_S1;
 #else
-#define NS_TILE_S1 8
+    return (NS_FP32 && CONV == 2) ? 16 : 8;
 #endif
-#endif
-constexpr int TILE_S1 = NS_TILE_S1;
+}
 #ifndef NS_TILE_WAVES
 #define NS_TILE_WAVES 1      // minimum waves per SIMD requested for the tile kernels
 #endif
@@ -1046,12 +1047,13 @@ __device__ __forceinline__ void flux_tile_xy_body(const ST* __restrict__ U,
 }
 
 template <int DIR, int CONV, int WT, int OPT, class ST>
-__global__ __launch_bounds__(64 * TILE_S1, tile_waves<WT>(NS_TILE1_WAVES)) void k_flux_tile_xy(
+__global__ __launch_bounds__(64 * tile_s1<CONV>(), tile_waves<WT>(NS_TILE1_WAVES)) void k_flux_tile_xy(
     const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, const MlpPacked* mlp, int lo, int hi) {
-    __shared__ real tile[(TILE_S1 + 5) * split_nv<OPT>() * TILE_L];
-    __shared__ real lq[WT == 2 ? TILE_S1 * LQ_REALS : 1];  // network queues (learned weights)
-    flux_tile_xy_body<DIR, CONV, WT, OPT, TILE_S1>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
+    constexpr int TS = tile_s1<CONV>();
+    __shared__ real tile[(TS + 5) * split_nv<OPT>() * TILE_L];
+    __shared__ real lq[WT == 2 ? TS * LQ_REALS : 1];  // network queues (learned weights)
+    flux_tile_xy_body<DIR, CONV, WT, OPT, TS>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
                                                    blockIdx.z, TILE_L, tile, mlp, lo, hi,
                                                    WT == 2 ? lq : nullptr);
 }
@@ -1138,12 +1140,12 @@ __global__ __launch_bounds__(512, tile_waves<WT>(NS_TILE_WAVES)) void k_flux_til
 // emit-mode tiles of the learned component-wise sweeps (no network queue in LDS) and the
 // network pass over one side of their samples
 template <int DIR, class ST>
-__global__ __launch_bounds__(64 * TILE_S1, NS_TILE1_WAVES) void k_flux_tile_xy_emit(
+__global__ __launch_bounds__(64 * tile_s1<1>(), NS_TILE1_WAVES) void k_flux_tile_xy_emit(
     const ST* __restrict__ U, const ST* __restrict__ W, flux_t* __restrict__ f, Geo g,
     real gamma, int lo, int hi, LearnEmit e) {
-    __shared__ real tile[(TILE_S1 + 5) * split_nv<0>() * TILE_L];
+    __shared__ real tile[(tile_s1<1>() + 5) * split_nv<0>() * TILE_L];
     learn_zero_next(e);
-    flux_tile_xy_body<DIR, 1, 2, 0, TILE_S1, true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
+    flux_tile_xy_body<DIR, 1, 2, 0, tile_s1<1>(), true>(U, W, f, g, gamma, blockIdx.x, blockIdx.y,
                                                    blockIdx.z, TILE_L, tile, nullptr, lo, hi,
                                                    nullptr, &e);
 }
@@ -3238,13 +3240,15 @@ static void flux_dir_st(const ST* U, const ST* W, flux_t* f, Geo g, double gamma
             k_flux_tile_z<CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo);
     } else if constexpr (DIR == 0) {
         const int fe = hi < g.nx ? hi : g.nx;  // faces of the tile kernel: [lo, fe)
-        dim3      blk(TILE_L, TILE_S1, 1);
-        dim3      grd((g.nz + TILE_L - 1) / TILE_L, g.ny, (fe - lo + TILE_S1 - 1) / TILE_S1);
+        constexpr int TS = tile_s1<CONV>();
+        dim3      blk(TILE_L, TS, 1);
+        dim3      grd((g.nz + TILE_L - 1) / TILE_L, g.ny, (fe - lo + TS - 1) / TS);
         if (fe > lo && !(EMIT && flux_emit<DIR>(U, W, f, g, gamma, mlp, sink, grd, blk, lo, fe, s)))
             k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, fe);
     } else {
-        dim3 blk(TILE_L, TILE_S1, 1);
-        dim3 grd((g.nz + TILE_L - 1) / TILE_L, hi - lo, (g.ny + TILE_S1 - 1) / TILE_S1);
+        constexpr int TS = tile_s1<CONV>();
+        dim3 blk(TILE_L, TS, 1);
+        dim3 grd((g.nz + TILE_L - 1) / TILE_L, hi - lo, (g.ny + TS - 1) / TS);
         if (hi > lo && !(EMIT && flux_emit<DIR>(U, W, f, g, gamma, mlp, sink, grd, blk, lo, hi, s)))
             k_flux_tile_xy<DIR, CONV, B, OPT><<<grd, blk, 0, s>>>(U, W, f, g, gamma, mlp, lo, hi);
     }
@@ -3651,8 +3655,9 @@ static void l_mlp_recon(const MlpPacked* P, long n, const double* S, int side, d
 static long l_learn_cap(Geo g, int* value_bytes) {
     if (value_bytes) *value_bytes = (int)sizeof(real);
     const long kt = (g.nz + TILE_L - 1) / TILE_L;
-    const long fx = learn_capr(kt * g.ny * ((g.nx + TILE_S1 - 1) / TILE_S1), 64L * TILE_S1);
-    const long fy = learn_capr(kt * g.nx * ((g.ny + TILE_S1 - 1) / TILE_S1), 64L * TILE_S1);
+    constexpr long TS = tile_s1<1>();  // (the emit-mode sweeps are component-wise)
+    const long fx = learn_capr(kt * g.ny * ((g.nx + TS - 1) / TS), 64L * TS);
+    const long fy = learn_capr(kt * g.nx * ((g.ny + TS - 1) / TS), 64L * TS);
     const long fz = learn_capr(kt * ((g.ny + 7) / 8) * g.nx, 512L);
     return std::max(fx, std::max(fy, fz)) * LS_REGIONS;
 }
