@@ -552,10 +552,13 @@ def fp32_form(ns, case, W0, args, fp64_value, U_fp64, math="fp32"):
             "accumulation and reductions" if math == "fp32" else
             "the headline workload in FP64 with fused multiply-add within expressions and hoisted "
             "reciprocals (round-off of the reference's order, not bitwise)")
+    tr = step_traffic(None, argparse.Namespace(math=math, viscosity=args.viscosity), n)
     return {"workload": what,
             "value": round(v, 6), "ms_per_step": round(el / args.steps * 1e3, 4),
             "speedup_vs_fp64": round(v / fp64_value, 4),
-            "errors_vs_fp64": err, "steps_compared": args.warmup + args.steps, "health": rc}
+            "errors_vs_fp64": err, "steps_compared": args.warmup + args.steps, "health": rc,
+            "traffic": tr["bytes_per_step"], "traffic_bytes_per_cell_step": tr["bytes_per_cell_step"],
+            "traffic_source": tr["source"]}
 
 
 def learned_form(ns, C, args):
